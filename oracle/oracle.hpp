@@ -1,0 +1,829 @@
+// CPU oracle for the CongFD right-hand side and SSP-RK3 time advance.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a plain C++ restatement of the
+// reference solver's arithmetic (wanghongfei23/CFD_inWSL, snapshot zone_FD,
+// plus the CongFD1.0 accuracy-test path), kept in the reference's evaluation
+// order so that a GPU result can be compared with it bit for bit.  Only the
+// test suite, the smoke check and the CPU-baseline leg of the benchmark load
+// it; the product library never links or calls it.
+//
+// Citations are relative to the reference repository root.
+//
+// Extensions that the reference does not have (3D, true periodic halo,
+// component-wise Euler) are marked "EXTENSION" and are pinned by invariants
+// (3D with w=0 equals 2D bit for bit, conservation, free-stream preservation,
+// mirror symmetry) rather than by reference vectors.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace oracle {
+
+// zone_FD/src/include/macro.hpp:28 (#define GAMMA 1.4)
+constexpr double GAMMA = 1.4;
+
+// Interpolation schemes (zone_FD/src/include/interScheme.hpp).
+enum Scheme { S_JS = 0, S_Z = 1, S_TENO5 = 2, S_TENO5_CT4 = 3, S_TENO5_CT7 = 4, S_TENOS = 5, S_LINEAR = 6 };
+enum Flux { F_HLLC = 0, F_ROE = 1 };
+
+template <class T> inline T sq(const T& a) { return a * a; }
+// std::min / std::max semantics: min(a,b) = (b < a) ? b : a, max(a,b) = (a < b) ? b : a
+template <class T> inline T smin(const T& a, const T& b) { return (b < a) ? b : a; }
+template <class T> inline T smax(const T& a, const T& b) { return (a < b) ? b : a; }
+
+// Smoothness indicators shared by every 5-point scheme
+// (zone_FD/src/include/interScheme.hpp:20-27, 384-392).
+template <class T> inline void betas(const T q[5], T b[3]) {
+  using std::abs;
+  b[0] = sq(q[0] - 2.0 * q[1] + q[2]) + 0.25 * sq(q[0] - 4.0 * q[1] + 3.0 * q[2]);
+  b[1] = sq(q[1] - 2.0 * q[2] + q[3]) + 0.25 * sq(q[1] - q[3]);
+  b[2] = sq(q[2] - 2.0 * q[3] + q[4]) + 0.25 * sq(3.0 * q[2] - 4.0 * q[3] + q[4]);
+}
+
+// Candidate stencils u0,u1,u2 (interScheme.hpp:29-32).
+template <class T> inline T cand0(const T q[5]) { return 0.375 * q[0] - 1.25 * q[1] + 1.875 * q[2]; }
+template <class T> inline T cand1(const T q[5]) { return -0.125 * q[1] + 0.75 * q[2] + 0.375 * q[3]; }
+template <class T> inline T cand2(const T q[5]) { return 0.375 * q[2] + 0.75 * q[3] - 0.125 * q[4]; }
+
+// TENO flag -> fixed stencil (interScheme.hpp:417-451).
+template <class T> inline T teno_select(unsigned flag, const T q[5]) {
+  switch (flag) {
+  case 0: return 0.0234375 * q[0] - 0.15625 * q[1] + 0.703125 * q[2] + 0.46875 * q[3] - 0.0390625 * q[4];
+  case 1: return -0.0625 * q[1] + 0.5625 * q[2] + 0.5625 * q[3] - 0.0625 * q[4];
+  case 2:
+  case 3: return cand2(q);
+  case 4: return 0.0625 * q[0] - 0.3125 * q[1] + 0.9375 * q[2] + 0.3125 * q[3];
+  case 5: return cand1(q);
+  case 6: return cand0(q);
+  default: return q[2];
+  }
+}
+
+// weno5_JSchen (interScheme.hpp:17-43)
+template <class T> inline T weno5_js(const T q[5]) {
+  T b[3];
+  betas(q, b);
+  const double d[3] = {0.0625, 0.625, 0.3125};
+  T u[3] = {cand0(q), cand1(q), cand2(q)};
+  T a[3];
+  for (int i = 0; i < 3; i++) a[i] = d[i] / sq(1e-6 + b[i]);
+  T s = a[0] + a[1] + a[2];
+  T r = a[0] * u[0] + a[1] * u[1] + a[2] * u[2];
+  return r / s;
+}
+
+// weno5_Z (interScheme.hpp:85-114)
+template <class T> inline T weno5_z(const T q[5]) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  const double d[3] = {0.0625, 0.625, 0.3125};
+  T u[3] = {cand0(q), cand1(q), cand2(q)};
+  T tau = abs(b[2] - b[0]);
+  T a[3];
+  for (int i = 0; i < 3; i++) a[i] = d[i] * (1.0 + sq(tau / (b[i] + 1e-40)));
+  T s = a[0] + a[1] + a[2];
+  T r = a[0] * u[0] + a[1] * u[1] + a[2] * u[2];
+  return r / s;
+}
+
+// Teno5_Z (interScheme.hpp:384-452); CT4 / CT7 variants (:116, :184) differ
+// only in the cut-off constant.
+template <class T> inline T teno5(const T q[5], double ct) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  T g[3];
+  for (int i = 0; i < 3; i++) {
+    T t = 1.0 + tau / (b[i] + 1e-40);
+    t = t * t;
+    g[i] = t * t * t;
+  }
+  T CT = ct * (g[0] + g[1] + g[2]);
+  unsigned flag = 0;
+  if (g[0] < CT) flag += 1;
+  if (g[1] < CT) flag += 2;
+  if (g[2] < CT) flag += 4;
+  return teno_select(flag, q);
+}
+
+// Teno5_CongZ, "TENO-S" (interScheme.hpp:494-585)
+template <class T> inline T tenos(const T q[5]) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  int m = 0;  // std::min_element: first minimal index
+  if (b[1] < b[m]) m = 1;
+  if (b[2] < b[m]) m = 2;
+  const double CT = 0.15704178024750198;
+  const double CT_1 = 1 - CT;
+  T tau = abs(b[2] - b[0]);
+  T rr = CT * tau - CT_1 * b[m];
+  T ll = tau * b[m];
+  unsigned flag = 0;
+  if (ll < rr * b[0]) flag += 1;
+  if (ll < rr * b[1]) flag += 2;
+  if (ll < rr * b[2]) flag += 4;
+  return teno_select(flag, q);
+}
+
+// Linear5 (interScheme.hpp:1450-1453)
+template <class T> inline T linear5(const T q[5]) { return teno_select(0u, q); }
+
+template <class T> inline T inter5(int s, const T q[5]) {
+  switch (s) {
+  case S_JS: return weno5_js(q);
+  case S_Z: return weno5_z(q);
+  case S_TENO5: return teno5(q, 1e-5);
+  case S_TENO5_CT4: return teno5(q, 1e-4);
+  case S_TENO5_CT7: return teno5(q, 1e-7);
+  case S_TENOS: return tenos(q);
+  default: return linear5(q);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Euler point kernels.  Variable order: prim = (rho, u[, v[, w]], p),
+// cons = (rho, rho u[, rho v[, rho w]], rho E).  NV = D + 2.
+// The face normal is the unit vector of axis `ax`.
+// ---------------------------------------------------------------------------
+
+// Equation::consToPrimEuler1D / Euler2D (zone_FD/src/src/equation.cpp:37-99);
+// EXTENSION: the 3D form adds w*w after the 2D sum.
+template <class T, int D> inline void cons_to_prim(const T* c, T* p) {
+  T r = c[0];
+  T u = c[1] / r;
+  if (D == 1) {
+    T E = c[2] / r;
+    T e = -u * u / 2 + E;
+    T RT = (GAMMA - 1) * e;
+    p[0] = r; p[1] = u; p[2] = r * RT;
+  } else if (D == 2) {
+    T v = c[2] / r;
+    T E = c[3] / r;
+    T q2 = (u * u + v * v) / 2;
+    T e = -q2 + E;
+    T RT = (GAMMA - 1) * e;
+    p[0] = r; p[1] = u; p[2] = v; p[3] = r * RT;
+  } else {
+    T v = c[2] / r;
+    T w = c[3] / r;
+    T E = c[4] / r;
+    T q2 = (u * u + v * v + w * w) / 2;
+    T e = -q2 + E;
+    T RT = (GAMMA - 1) * e;
+    p[0] = r; p[1] = u; p[2] = v; p[3] = w; p[4] = r * RT;
+  }
+}
+
+// Conservative state rebuilt from a primitive state exactly as
+// eigensystemEuler{1D,2D}::primToChar does before projecting
+// (zone_FD/src/src/eigenSystem.cpp:91-93, 254-256).
+template <class T, int D> inline void prim_to_cons(const T* p, T* c) {
+  T r = p[0];
+  if (D == 1) {
+    T ek = (p[1] * p[1]) / 2;
+    c[0] = r; c[1] = r * p[1]; c[2] = p[2] / (GAMMA - 1) + r * ek;
+  } else if (D == 2) {
+    T ek = (p[1] * p[1] + p[2] * p[2]) / 2;
+    c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = p[3] / (GAMMA - 1) + r * ek;
+  } else {
+    T ek = (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]) / 2;
+    c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = r * p[3];
+    c[4] = p[4] / (GAMMA - 1) + r * ek;
+  }
+}
+
+// Roe-averaged characteristic system at one interface.
+// 1D: eigensystemEuler1D(priml, primr) (eigenSystem.cpp:217-250).
+// 2D: eigensystemEuler2D(priml, primr, norm) (eigenSystem.cpp:13-86).
+// 3D (EXTENSION): waves (shear1, entropy, Vn-c, Vn+c, shear2); for the x and y
+// normals shear1 is the 2D shear wave and every sum keeps the 2D order with
+// the w term inserted so that w = 0 reproduces 2D bit for bit.
+template <class T, int D> struct Eigen {
+  static constexpr int NV = D + 2;
+  T L[NV][NV];  // rows: waves; columns: cons components
+  T R[NV][NV];  // rows: cons components; columns: waves
+
+  void build(const T* pl, const T* pr, int ax) {
+    using std::sqrt;
+    if (D == 1) {
+      T rl = pl[0], ul = pl[1], ppl = pl[2];
+      T rr = pr[0], ur = pr[1], ppr = pr[2];
+      T HL = (ul * ul) / 2 + ppl / rl * GAMMA / (GAMMA - 1);
+      T HR = (ur * ur) / 2 + ppr / rr * GAMMA / (GAMMA - 1);
+      T c1 = sqrt(rl), c2 = sqrt(rr);
+      T divisor = 1.0 / (sqrt(rl) + sqrt(rr));
+      T u = (c1 * ul + c2 * ur) * divisor;
+      T ht = (c1 * HL + c2 * HR) * divisor;
+      T ek = (u * u) / 2;
+      T h = ht - ek;
+      T c = sqrt((GAMMA - 1) * h);
+      const double g1 = GAMMA - 1;
+      T l[9] = {ht + c * (u - c) / g1, -u - c / g1, T(1.0),
+                -2.0 * ht + 4.0 * c * c / g1, 2.0 * u, T(-2.0),
+                ht - c * (u + c) / g1, -u + c / g1, T(1.0)};
+      T f = 0.5 * g1 / (c * c);
+      for (int i = 0; i < 9; i++) L[i / 3][i % 3] = l[i] * f;
+      T r9[9] = {T(1.0), T(1.0), T(1.0), u - c, u, u + c, ht - u * c, 0.5 * u * u, ht + u * c};
+      for (int i = 0; i < 9; i++) R[i / 3][i % 3] = r9[i];
+      return;
+    }
+    // 2D / 3D Roe average (eigenSystem.cpp:21-51)
+    T rl = pl[0], rr = pr[0];
+    T ul = pl[1], ur = pr[1], vl = pl[2], vr = pr[2];
+    T wl = (D == 3) ? pl[3] : T(0.0), wr = (D == 3) ? pr[3] : T(0.0);
+    T ppl = pl[NV - 1], ppr = pr[NV - 1];
+    T HL, HR;
+    if (D == 2) {
+      HL = (ul * ul + vl * vl) / 2 + ppl / rl * GAMMA / (GAMMA - 1);
+      HR = (ur * ur + vr * vr) / 2 + ppr / rr * GAMMA / (GAMMA - 1);
+    } else {
+      HL = (ul * ul + vl * vl + wl * wl) / 2 + ppl / rl * GAMMA / (GAMMA - 1);
+      HR = (ur * ur + vr * vr + wr * wr) / 2 + ppr / rr * GAMMA / (GAMMA - 1);
+    }
+    T c1 = sqrt(rl), c2 = sqrt(rr);
+    T divisor = 1.0 / (c1 + c2);
+    T u = (c1 * ul + c2 * ur) * divisor;
+    T v = (c1 * vl + c2 * vr) * divisor;
+    T w = (D == 3) ? (c1 * wl + c2 * wr) * divisor : T(0.0);
+    T ht = (c1 * HL + c2 * HR) * divisor;
+    const double nx = ax == 0 ? 1.0 : 0.0, ny = ax == 1 ? 1.0 : 0.0, nz = ax == 2 ? 1.0 : 0.0;
+    T Vn, ek;
+    if (D == 2) {
+      Vn = nx * u + ny * v;
+      ek = (u * u + v * v) / 2;
+    } else {
+      Vn = nx * u + ny * v + nz * w;
+      ek = (u * u + v * v + w * w) / 2;
+    }
+    T h = ht - ek;
+    T c = sqrt((GAMMA - 1) * h);
+    if (D == 2) {
+      // leftEig / rightEig (eigenSystem.cpp:53-85)
+      T l[16] = {(-ny * u + nx * v), T(ny), T(-nx), T(0.0),
+                 (h - ek), u, v, T(-1.0),
+                 (Vn / c + ek / h) / 2, (-nx / c - u / h) / 2, (-ny / c - v / h) / 2, 1.0 / (2.0 * h),
+                 (-Vn / c + ek / h) / 2, (nx / c - u / h) / 2, (ny / c - v / h) / 2, 1.0 / (2.0 * h)};
+      T r[16] = {T(0.0), 1.0 / h, T(1.0), T(1.0),
+                 T(ny), u / h, u - nx * c, u + nx * c,
+                 T(-nx), v / h, v - ny * c, v + ny * c,
+                 ny * u - nx * v, ek / h, h + ek - Vn * c, h + ek + Vn * c};
+      for (int i = 0; i < 16; i++) { L[i / 4][i % 4] = l[i]; R[i / 4][i % 4] = r[i]; }
+      return;
+    }
+    // 3D: tangents t1 (2D shear direction for x/y normals), t2.
+    // x: t1=(0,-1,0) t2=(0,0,1); y: t1=(1,0,0) t2=(0,0,1); z: t1=(1,0,0) t2=(0,1,0)
+    double t1[3], t2[3];
+    if (ax == 0) { t1[0] = 0; t1[1] = -1; t1[2] = 0; t2[0] = 0; t2[1] = 0; t2[2] = 1; }
+    else if (ax == 1) { t1[0] = 1; t1[1] = 0; t1[2] = 0; t2[0] = 0; t2[1] = 0; t2[2] = 1; }
+    else { t1[0] = 1; t1[1] = 0; t1[2] = 0; t2[0] = 0; t2[1] = 1; t2[2] = 0; }
+    // shear-1 row/column: for x/y normals the literal 2D expressions
+    T l0rho, r0E;
+    if (ax < 2) { l0rho = (-ny * u + nx * v); r0E = ny * u - nx * v; }
+    else { l0rho = -u; r0E = u; }
+    T t2u = (ax == 2) ? v : w;  // t2 . velocity
+    // rows of L: (rho, rho u, rho v, rho w, rho E)
+    L[0][0] = l0rho; L[0][1] = T(ax < 2 ? ny : t1[0]); L[0][2] = T(ax < 2 ? -nx : t1[1]); L[0][3] = T(0.0); L[0][4] = T(0.0);
+    L[1][0] = h - ek; L[1][1] = u; L[1][2] = v; L[1][3] = w; L[1][4] = T(-1.0);
+    L[2][0] = (Vn / c + ek / h) / 2; L[2][1] = (-nx / c - u / h) / 2; L[2][2] = (-ny / c - v / h) / 2;
+    L[2][3] = (-nz / c - w / h) / 2; L[2][4] = 1.0 / (2.0 * h);
+    L[3][0] = (-Vn / c + ek / h) / 2; L[3][1] = (nx / c - u / h) / 2; L[3][2] = (ny / c - v / h) / 2;
+    L[3][3] = (nz / c - w / h) / 2; L[3][4] = 1.0 / (2.0 * h);
+    L[4][0] = -t2u; L[4][1] = T(t2[0]); L[4][2] = T(t2[1]); L[4][3] = T(t2[2]); L[4][4] = T(0.0);
+    // columns of R (stored R[component][wave])
+    R[0][0] = T(0.0); R[0][1] = 1.0 / h; R[0][2] = T(1.0); R[0][3] = T(1.0); R[0][4] = T(0.0);
+    R[1][0] = T(ax < 2 ? ny : t1[0]); R[1][1] = u / h; R[1][2] = u - nx * c; R[1][3] = u + nx * c; R[1][4] = T(t2[0]);
+    R[2][0] = T(ax < 2 ? -nx : t1[1]); R[2][1] = v / h; R[2][2] = v - ny * c; R[2][3] = v + ny * c; R[2][4] = T(t2[1]);
+    R[3][0] = T(t1[2]); R[3][1] = w / h; R[3][2] = w - nz * c; R[3][3] = w + nz * c; R[3][4] = T(t2[2]);
+    R[4][0] = r0E; R[4][1] = ek / h; R[4][2] = h + ek - Vn * c; R[4][3] = h + ek + Vn * c; R[4][4] = t2u;
+  }
+
+  // primToChar applied to a conservative state built by prim_to_cons
+  // (eigenSystem.cpp:97-107, 260-264).
+  void to_char(const T* c, T* ch) const {
+    for (int k = 0; k < NV; k++) {
+      if (D == 1) ch[k] = c[0] * L[k][0] + c[1] * L[k][1] + c[2] * L[k][2];
+      else if (D == 2) ch[k] = c[1] * L[k][1] + c[2] * L[k][2] + c[3] * L[k][3] + c[0] * L[k][0];
+      else ch[k] = c[1] * L[k][1] + c[2] * L[k][2] + c[3] * L[k][3] + c[4] * L[k][4] + c[0] * L[k][0];
+    }
+  }
+  // charToCons (eigenSystem.cpp:139-149 summation order ch3,ch4,ch2,ch1; 1D :300-302)
+  void to_cons(const T* ch, T* c) const {
+    for (int i = 0; i < NV; i++) {
+      if (D == 1) c[i] = ch[0] * R[i][0] + ch[1] * R[i][1] + ch[2] * R[i][2];
+      else if (D == 2) c[i] = ch[2] * R[i][2] + ch[3] * R[i][3] + ch[1] * R[i][1] + ch[0] * R[i][0];
+      else c[i] = ch[2] * R[i][2] + ch[3] * R[i][3] + ch[1] * R[i][1] + ch[0] * R[i][0] + ch[4] * R[i][4];
+    }
+  }
+};
+
+// eigensystemEuler2D::charToPrim tail (eigenSystem.cpp:163-167);
+// 1D eigensystemEuler1D::consToPrim (:309-322).
+template <class T, int D> inline void eig_cons_to_prim(const T* c, T* p) {
+  T rt = c[0];
+  if (D == 1) {
+    T ut = c[1] / rt;
+    T ekt = (ut * ut) / 2;
+    p[0] = rt; p[1] = ut; p[2] = (GAMMA - 1) * (c[2] - rt * ekt);
+  } else if (D == 2) {
+    T ut = c[1] / rt, vt = c[2] / rt;
+    T rekt = (c[1] * c[1] + c[2] * c[2]) / rt / 2;
+    p[0] = rt; p[1] = ut; p[2] = vt; p[3] = (GAMMA - 1) * (c[3] - rekt);
+  } else {
+    T ut = c[1] / rt, vt = c[2] / rt, wt = c[3] / rt;
+    T rekt = (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) / rt / 2;
+    p[0] = rt; p[1] = ut; p[2] = vt; p[3] = wt; p[4] = (GAMMA - 1) * (c[4] - rekt);
+  }
+}
+
+// variable_positive_limiter (zone_FD/src/include/positive_preserving.hpp:14-46)
+// with the density (q=rho) and rho*c^2 (q=gamma p) functionals used by
+// Recon5Order1DEulerEig::reconI (reconstructor5order.hpp:204-232).
+template <class T> inline T lim_rho(const T* W) { return W[0]; }
+template <class T> inline T lim_rhoc2(const T* W) {
+  T u = W[1] / W[0];
+  T ekin = 0.5 * u * u;
+  T p = (GAMMA - 1) * (W[2] - W[0] * ekin);
+  return GAMMA * p;
+}
+template <class T> inline void positive_limit(const T* W, T* Wi, int which) {
+  T qi = which == 0 ? lim_rho(Wi) : lim_rhoc2(Wi);
+  if (qi >= 1e-10) return;
+  T qw = which == 0 ? lim_rho(W) : lim_rhoc2(W);
+  if (qw < 1e-10) {
+    for (int i = 0; i < 3; i++) Wi[i] = W[i];
+    return;
+  }
+  T theta = (1e-10 - qw) / (qi - qw);
+  for (int i = 0; i < 3; i++) Wi[i] = (1 - theta) * W[i] + theta * Wi[i];
+}
+
+// Half-node reconstruction at one interface from the 6-cell window
+// w[0..5] = cells i-3 .. i+2 (interface between w[2] and w[3]).
+// Component-wise: Recon5OrderFaceCenter::reconI (reconstructor5order.hpp:128-151).
+// Characteristic: Recon5Order{1D,2D}EulerEig::reconI (:154-244, :247-356).
+template <class T, int D>
+inline void recon_interface(int scheme, bool charwise, const T* const* wp, const T* const* wc, int ax,
+                            T* outL, T* outR) {
+  constexpr int NV = D + 2;
+  if (!charwise) {
+    for (int i = 0; i < NV; i++) {
+      T qL[5] = {wp[0][i], wp[1][i], wp[2][i], wp[3][i], wp[4][i]};
+      T qR[5] = {wp[5][i], wp[4][i], wp[3][i], wp[2][i], wp[1][i]};
+      outL[i] = inter5(scheme, qL);
+      outR[i] = inter5(scheme, qR);
+    }
+    return;
+  }
+  const T* pL = wp[2];
+  const T* pR = wp[3];
+  Eigen<T, D> eig;
+  eig.build(pL, pR, ax);
+  T ch[6][NV];
+  for (int j = 0; j < 6; j++) eig.to_char(wc[j], ch[j]);
+  T QL[NV], QR[NV];
+  for (int k = 0; k < NV; k++) {
+    T qL[5] = {ch[0][k], ch[1][k], ch[2][k], ch[3][k], ch[4][k]};
+    T qR[5] = {ch[5][k], ch[4][k], ch[3][k], ch[2][k], ch[1][k]};
+    QL[k] = inter5(scheme, qL);
+    QR[k] = inter5(scheme, qR);
+  }
+  T cL[NV], cR[NV];
+  eig.to_cons(QL, cL);
+  eig.to_cons(QR, cR);
+  if (D == 1) {
+    T consL[3], consR[3];
+    prim_to_cons<T, 1>(pL, consL);
+    prim_to_cons<T, 1>(pR, consR);
+    positive_limit(consL, cL, 0);
+    positive_limit(consR, cR, 0);
+    positive_limit(consL, cL, 1);
+    positive_limit(consR, cR, 1);
+    eig_cons_to_prim<T, 1>(cL, outL);
+    eig_cons_to_prim<T, 1>(cR, outR);
+    return;
+  }
+  eig_cons_to_prim<T, D>(cL, outL);
+  eig_cons_to_prim<T, D>(cR, outR);
+  // simple positivity fallback (reconstructor5order.hpp:301-302); the right
+  // state keeps the reference's use of the left cell's velocities.
+  if (outL[0] < 0 || outL[NV - 1] < 0)
+    for (int i = 0; i < NV; i++) outL[i] = pL[i];
+  if (outR[0] < 0 || outR[NV - 1] < 0) {
+    outR[0] = pR[0];
+    for (int i = 1; i < NV - 1; i++) outR[i] = pL[i];
+    outR[NV - 1] = pR[NV - 1];
+  }
+}
+
+// HLLCFlux1D (zone_FD/src/include/fluxSchemes.hpp:74-125)
+template <class T> inline void hllc1d(const T* L, const T* Rr, T* res) {
+  using std::sqrt;
+  T rl = L[0], ul = L[1], pl = L[2], rr = Rr[0], ur = Rr[1], pr = Rr[2];
+  T HL = pl / rl * GAMMA / (GAMMA - 1) + (ul * ul) / 2;
+  T HR = pr / rr * GAMMA / (GAMMA - 1) + (ur * ur) / 2;
+  T cl = sqrt(GAMMA * (pl / rl));
+  T cr = sqrt(GAMMA * (pr / rr));
+  T SL = smin(ur - cr, ul - cl);
+  T SR = smax(ul + cl, ur + cr);
+  T Sstar = ((pr - pl) + (rl * ul * (SL - ul) - rr * ur * (SR - ur))) / (rl * (SL - ul) - rr * (SR - ur));
+  if (SL >= 0) {
+    res[0] = rl * ul; res[1] = rl * ul * ul + pl; res[2] = rl * HL * ul;
+  } else if (Sstar >= 0) {
+    T pStar = pl + rl * (SL - ul) * (Sstar - ul);
+    T U[3] = {rl, rl * ul, pl / (GAMMA - 1) + rl * ul * ul / 2};
+    T F[3] = {rl * ul, rl * ul * ul + pl, rl * HL * ul};
+    T Dd[3] = {T(0.0), T(1.0), Sstar};
+    for (int i = 0; i < 3; i++) res[i] = (Sstar * (SL * U[i] - F[i]) + SL * pStar * Dd[i]) / (SL - Sstar);
+  } else if (SR >= 0) {
+    T pStar = pr + rr * (SR - ur) * (Sstar - ur);
+    T U[3] = {rr, rr * ur, pr / (GAMMA - 1) + rr * ur * ur / 2};
+    T F[3] = {rr * ur, rr * ur * ur + pr, rr * HR * ur};
+    T Dd[3] = {T(0.0), T(1.0), Sstar};
+    for (int i = 0; i < 3; i++) res[i] = (Sstar * (SR * U[i] - F[i]) + SR * pStar * Dd[i]) / (SR - Sstar);
+  } else {
+    res[0] = rr * ur; res[1] = rr * ur * ur + pr; res[2] = rr * HR * ur;
+  }
+}
+
+// roeFlux1D2 (fluxSchemes.hpp:13-71)
+template <class T> inline void roe1d(const T* L, const T* Rr, T* res) {
+  using std::abs;
+  using std::sqrt;
+  T rl = L[0], ul = L[1], pl = L[2], rr = Rr[0], ur = Rr[1], pr = Rr[2];
+  T HL = pl / rl * GAMMA / (GAMMA - 1) + (ul * ul) / 2;
+  T HR = pr / rr * GAMMA / (GAMMA - 1) + (ur * ur) / 2;
+  T rBar = sqrt(rl * rr);
+  T uBar = (ul * sqrt(rl) + ur * sqrt(rr)) / (sqrt(rl) + sqrt(rr));
+  T HBar = (HL * sqrt(rl) + HR * sqrt(rr)) / (sqrt(rl) + sqrt(rr));
+  T cBar = sqrt((GAMMA - 1) * (HBar - uBar * uBar / 2));
+  T cBar2 = (GAMMA - 1) * (HBar - uBar * uBar / 2);
+  T dr = rr - rl, du = ur - ul, dp = pr - pl;
+  T lam[3] = {abs(uBar), abs(uBar + cBar), abs(uBar - cBar)};
+  T eps = 0.1 * (abs(uBar) + cBar);
+  for (int i = 0; i < 3; i++)
+    if (lam[i] < eps) lam[i] = (lam[i] * lam[i] + eps * eps) / (2.0 * eps);
+  T a0 = lam[0] * (dr - dp / cBar2);
+  T a1 = lam[1] / (2.0 * cBar2) * (dp + rBar * cBar * du);
+  T a2 = lam[2] / (2.0 * cBar2) * (dp - rBar * cBar * du);
+  T a3 = a0 + a1 + a2;
+  T a4 = cBar * (a1 - a2);
+  T FL[3] = {rl * ul, rl * ul * ul + pl, rl * HL * ul};
+  T FR[3] = {rr * ur, rr * ur * ur + pr, rr * HR * ur};
+  for (int i = 0; i < 3; i++) res[i] = (FL[i] + FR[i]) / 2.0;
+  res[0] -= 0.5 * a3;
+  res[1] -= 0.5 * (a3 * uBar + a4);
+  res[2] -= 0.5 * (HBar * a3 + uBar * a4 - cBar2 * a0 / (GAMMA - 1));
+}
+
+// HLLCFlux2D2 (fluxSchemes.hpp:202-327); EXTENSION to 3D with the tangential
+// momentum of the star state written as Sstar*n_k + u_k*(sum of the other n's).
+template <class T, int D> inline void hllc_md(const T* Lp, const T* Rp, int ax, T* res) {
+  using std::sqrt;
+  constexpr int NV = D + 2;
+  constexpr int E = NV - 1;
+  double n[3] = {ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0};
+  T rl = Lp[0], rr = Rp[0], pl = Lp[E], pr = Rp[E];
+  T vel_l[3], vel_r[3];
+  for (int k = 0; k < D; k++) { vel_l[k] = Lp[1 + k]; vel_r[k] = Rp[1 + k]; }
+  T HL, HR;
+  if (D == 2) {
+    HL = (vel_l[0] * vel_l[0] + vel_l[1] * vel_l[1]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
+    HR = (vel_r[0] * vel_r[0] + vel_r[1] * vel_r[1]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+  } else {
+    HL = (vel_l[0] * vel_l[0] + vel_l[1] * vel_l[1] + vel_l[2] * vel_l[2]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
+    HR = (vel_r[0] * vel_r[0] + vel_r[1] * vel_r[1] + vel_r[2] * vel_r[2]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+  }
+  T VnL = vel_l[ax], VnR = vel_r[ax];
+  T cl = sqrt(pl / rl * GAMMA);
+  T cr = sqrt(pr / rr * GAMMA);
+  T rBar = (rl + rr) / 2, cBar = (cl + cr) / 2;
+  T ps = smax(T(0.0), (pl + pr) / 2 - rBar * cBar * (VnR - VnL) / 2);
+  T ql = (ps <= pl) ? T(1.0) : sqrt(1.0 + (GAMMA + 1.0) / (2.0 * GAMMA) * (ps / pl - 1.0));
+  T qr = (ps <= pr) ? T(1.0) : sqrt(1.0 + (GAMMA + 1.0) / (2.0 * GAMMA) * (ps / pr - 1.0));
+  T SL = VnL - cl * ql, SR = VnR + cr * qr;
+  T Sstar = ((pr - pl) + (rl * VnL * (SL - VnL) - rr * VnR * (SR - VnR))) / (rl * (SL - VnL) - rr * (SR - VnR));
+  auto side_flux = [&](T r, const T* vel, T p, T H, T Vn, T* F) {
+    F[0] = r * Vn;
+    for (int k = 0; k < D; k++) F[1 + k] = r * vel[k] * Vn + p * n[k];
+    F[E] = r * H * Vn;
+  };
+  // star-region flux F + S*(U* - U) of one side
+  auto star_flux = [&](T r, const T* vel, T p, T H, T Vn, T S, T* out) {
+    T U[NV], F[NV], Us[NV];
+    U[0] = r;
+    for (int k = 0; k < D; k++) U[1 + k] = r * vel[k];
+    U[E] = r * H - p;
+    side_flux(r, vel, p, H, Vn, F);
+    T Usf = (S - Vn) / (S - Sstar);
+    Us[0] = Usf * r;
+    for (int k = 0; k < D; k++) {
+      double other = 0.0;  // sum of the other normal components (the 2D n[1]/n[0] swap)
+      for (int m = 0; m < D; m++)
+        if (m != k) other += n[m];
+      Us[1 + k] = Usf * (Sstar * n[k] + vel[k] * other) * r;
+    }
+    Us[E] = Usf * (U[E] + (Sstar - Vn) * (r * Sstar + p / (S - Vn)));
+    for (int i = 0; i < NV; i++) out[i] = F[i] + S * (Us[i] - U[i]);
+  };
+  if (SL > 0) {
+    side_flux(rl, vel_l, pl, HL, VnL, res);
+  } else if (Sstar > 0) {
+    star_flux(rl, vel_l, pl, HL, VnL, SL, res);
+  } else if (Sstar == 0) {
+    T a[NV], b[NV];
+    star_flux(rl, vel_l, pl, HL, VnL, SL, a);
+    star_flux(rr, vel_r, pr, HR, VnR, SR, b);
+    for (int i = 0; i < NV; i++) res[i] = a[i] / 2;
+    for (int i = 0; i < NV; i++) res[i] += b[i] / 2;
+  } else if (SR > 0) {
+    star_flux(rr, vel_r, pr, HR, VnR, SR, res);
+  } else {
+    side_flux(rr, vel_r, pr, HR, VnR, res);
+  }
+}
+
+// roeFlux2DSym (fluxSchemes.hpp:420-553); EXTENSION to 3D: every tangential
+// velocity contributes to the shear part in the order (u, v, w).
+template <class T, int D> inline void roe_md(const T* Lp, const T* Rp, int ax, T* res) {
+  using std::abs;
+  using std::sqrt;
+  constexpr int NV = D + 2;
+  constexpr int E = NV - 1;
+  double n[3] = {ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0};
+  T rl = Lp[0], rr = Rp[0], pl = Lp[E], pr = Rp[E];
+  T vl[3], vr[3];
+  for (int k = 0; k < D; k++) { vl[k] = Lp[1 + k]; vr[k] = Rp[1 + k]; }
+  T HL, HR;
+  if (D == 2) {
+    HL = (vl[0] * vl[0] + vl[1] * vl[1]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
+    HR = (vr[0] * vr[0] + vr[1] * vr[1]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+  } else {
+    HL = (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
+    HR = (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+  }
+  T VnL = vl[ax], VnR = vr[ax];
+  T FcL[NV], FcR[NV];
+  FcL[0] = rl * VnL;
+  FcR[0] = rr * VnR;
+  for (int k = 0; k < D; k++) {
+    FcL[1 + k] = rl * vl[k] * VnL + pl * n[k];
+    FcR[1 + k] = rr * vr[k] * VnR + pr * n[k];
+  }
+  FcL[E] = rl * HL * VnL;
+  FcR[E] = rr * HR * VnR;
+  T c1 = sqrt(rl), c2 = sqrt(rr);
+  T divisor = 1.0 / (sqrt(rl) + sqrt(rr));
+  T rhoAvg = sqrt(rl * rr);
+  T va[3];
+  for (int k = 0; k < D; k++) va[k] = (c1 * vl[k] + c2 * vr[k]) * divisor;
+  T HAvg = (c1 * HL + c2 * HR) * divisor;
+  T q2;
+  if (D == 2) q2 = (va[0] * va[0] + va[1] * va[1]) / 2;
+  else q2 = (va[0] * va[0] + va[1] * va[1] + va[2] * va[2]) / 2;
+  T cAvg = sqrt((GAMMA - 1) * (HAvg - q2));
+  T VnAvg = va[ax];
+  T lam[3] = {abs(VnAvg - cAvg), abs(VnAvg), abs(VnAvg + cAvg)};
+  T eps = 0.05 * (abs(VnAvg) + cAvg);
+  for (int i = 0; i < 3; i++)
+    if (lam[i] < eps) lam[i] = (lam[i] * lam[i] + eps * eps) / (2.0 * eps);
+  T dP = pr - pl, dVn = VnR - VnL, dRho = rr - rl;
+  T dv[3];
+  for (int k = 0; k < D; k++) dv[k] = vr[k] - vl[k];
+  T FD[NV];
+  T cf1 = (dP - rhoAvg * cAvg * dVn) / (2.0 * cAvg * cAvg);
+  FD[0] = lam[0] * (cf1 * 1);
+  for (int k = 0; k < D; k++) FD[1 + k] = lam[0] * (cf1 * (va[k] - cAvg * n[k]));
+  FD[E] = lam[0] * (cf1 * (HAvg - cAvg * VnAvg));
+  T cf = (dP + rhoAvg * cAvg * dVn) / (2.0 * cAvg * cAvg);
+  FD[0] += lam[2] * (cf * 1);
+  for (int k = 0; k < D; k++) FD[1 + k] += lam[2] * (cf * (va[k] + cAvg * n[k]));
+  FD[E] += lam[2] * (cf * (HAvg + cAvg * VnAvg));
+  T cf2 = dRho - dP / (cAvg * cAvg);
+  FD[0] += lam[1] * (cf2 * 1.0 + rhoAvg * 0.0);
+  for (int k = 0; k < D; k++) FD[1 + k] += lam[1] * (cf2 * va[k] + rhoAvg * (k == ax ? T(0.0) : dv[k]));
+  // shear energy: sum over tangential components in (u, v, w) order
+  T shear = T(0.0);
+  bool first = true;
+  for (int k = 0; k < D; k++) {
+    if (k == ax) continue;
+    if (first) { shear = va[k] * dv[k]; first = false; }
+    else shear = shear + va[k] * dv[k];
+  }
+  FD[E] += lam[1] * (cf2 * q2 + rhoAvg * shear);
+  for (int i = 0; i < NV; i++) res[i] = (FcL[i] + FcR[i] - FD[i]) / 2;
+}
+
+// Physical flux at a cell: fluxSolveEuler1D / 2D (fluxSchemes.hpp:572-599).
+template <class T, int D> inline void node_flux(const T* p, int ax, T* F) {
+  if (D == 1) {
+    T r = p[0], u = p[1], pp = p[2];
+    F[0] = r * u;
+    F[1] = r * u * u + pp;
+    F[2] = (GAMMA / (GAMMA - 1) * pp + r * (u * u) / 2) * u;
+    return;
+  }
+  constexpr int NV = D + 2;
+  constexpr int E = NV - 1;
+  double n[3] = {ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0};
+  T r = p[0], pp = p[E];
+  T Vn, k2;
+  if (D == 2) {
+    Vn = (p[1] * n[0] + p[2] * n[1]);
+    k2 = (p[1] * p[1] + p[2] * p[2]);
+  } else {
+    Vn = (p[1] * n[0] + p[2] * n[1] + p[3] * n[2]);
+    k2 = (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+  }
+  F[0] = r * Vn;
+  for (int k = 0; k < D; k++) F[1 + k] = r * p[1 + k] * Vn + n[k] * pp;
+  F[E] = (k2 / 2 * r + GAMMA / (GAMMA - 1) * pp) * Vn;
+}
+
+template <class T, int D> inline void riemann(int flux, const T* L, const T* R, int ax, T* res) {
+  if (D == 1) {
+    if (flux == F_HLLC) hllc1d(L, R, res);
+    else roe1d(L, R, res);
+  } else {
+    if (flux == F_HLLC) hllc_md<T, D>(L, R, ax, res);
+    else roe_md<T, D>(L, R, ax, res);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary ("halo") fill per pencil end: OneDBnd::update
+// (zone_FD/src/src/oneDBnd.cpp:31-114) with the wiring of
+// Initializer::initBnds (initializer.cpp:743-952).  Ghost k = 0 is adjacent to
+// the domain.  Codes follow the reference BndType (macro.hpp:33-34) plus
+// EXTENSION codes for a true periodic Euler halo and a z symmetry plane.
+// ---------------------------------------------------------------------------
+enum Bc {
+  BC_PERIODIC = 1,       // true periodic (LINEARCONV1D wiring, initializer.cpp:769-773)
+  BC_SYMMETRY1D = 2,     // mirror, flip u
+  BC_DIRICHLET = 3,      // fixed primitive state
+  BC_OUTFLOW = 7,        // SUPERSONICOUTLET: copy the edge cell
+  BC_SYMMETRYX = 10,     // mirror, flip u
+  BC_SYMMETRYY = 11,     // mirror, flip v
+  BC_SYMMETRYZ = 101,    // EXTENSION: mirror, flip w
+  BC_MIRROR_COPY = 102,  // the reference's 2D Euler PERIODIC1D wiring: mirror without flip
+};
+
+template <class T, int D>
+inline void fill_ghost(int bc, const double* dir_prim, const T* prim_pencil, int n, int stride_v, bool left,
+                       T* ghost /* 4 x NV */) {
+  constexpr int NV = D + 2;
+  for (int k = 0; k < 4; k++) {
+    int src;
+    int flip = -1;
+    switch (bc) {
+    case BC_PERIODIC: src = left ? ((n - 1 - k) % n + n) % n : k % n; break;
+    case BC_OUTFLOW: src = left ? 0 : n - 1; break;
+    case BC_SYMMETRY1D:
+    case BC_SYMMETRYX: src = left ? k : n - 1 - k; flip = 1; break;
+    case BC_SYMMETRYY: src = left ? k : n - 1 - k; flip = 2; break;
+    case BC_SYMMETRYZ: src = left ? k : n - 1 - k; flip = 3; break;
+    case BC_MIRROR_COPY: src = left ? k : n - 1 - k; break;
+    default: src = -1; break;
+    }
+    for (int v = 0; v < NV; v++) {
+      T val;
+      if (bc == BC_DIRICHLET) val = T(dir_prim[v]);
+      else {
+        val = prim_pencil[(long)src * stride_v + v];
+        if (v == flip) val = -val;
+      }
+      ghost[k * NV + v] = val;
+    }
+  }
+}
+
+struct Grid {
+  int dim;
+  int n[3];
+  double h[3];
+  int scheme;
+  int flux;
+  int charwise;
+  int bc[6];
+  double bc_prim[6][5];
+};
+
+// One direction sweep: SpDistributor::rhsSolve body for direction `ax`
+// (zone_FD/src/src/sp_distributor.cpp:9-78): per pencil, reconstruct n+3
+// half nodes from 4+n+4 cells, Riemann flux at each, node flux at cells
+// -1..n, then MidNodeAndNodeDif6 (zone_FD/src/src/differ.cpp:54-103):
+// rhs_i += (w1 (f_{i+1}-f_{i-1}) + w0 (F_{i+1/2}-F_{i-1/2}) + w2 (F_{i+3/2}-F_{i-3/2})) / h.
+template <class T, int D>
+void sweep(const Grid& g, int ax, const T* prim, T* rhs, int nthreads) {
+  constexpr int NV = D + 2;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const long sx = 1, sy = nx, sz = (long)nx * ny;
+  const int n = g.n[ax];
+  const long sa = ax == 0 ? sx : ax == 1 ? sy : sz;
+  int m1, m2;
+  long s1, s2;
+  if (ax == 0) { m1 = ny; m2 = nz; s1 = sy; s2 = sz; }
+  else if (ax == 1) { m1 = nx; m2 = nz; s1 = sx; s2 = sz; }
+  else { m1 = nx; m2 = ny; s1 = sx; s2 = sy; }
+  const double hh = g.h[ax];
+  const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
+  const long npencil = (long)m1 * m2;
+#pragma omp parallel num_threads(nthreads)
+  {
+    std::vector<T> pp((n + 8) * NV), cc((n + 8) * NV), fh((n + 3) * NV), fn((n + 2) * NV);
+#pragma omp for schedule(static)
+    for (long ip = 0; ip < npencil; ip++) {
+      long i1 = ip % m1, i2 = ip / m1;
+      long base = i1 * s1 + i2 * s2;
+      // gather pencil with ghosts: pp[(j+4)*NV + v] for cell j in [-4, n+4)
+      for (int j = 0; j < n; j++)
+        for (int v = 0; v < NV; v++) pp[(j + 4) * NV + v] = prim[(base + j * sa) * NV + v];
+      T gl[4 * NV], gr[4 * NV];
+      fill_ghost<T, D>(g.bc[2 * ax], g.bc_prim[2 * ax], &pp[4 * NV], n, NV, true, gl);
+      fill_ghost<T, D>(g.bc[2 * ax + 1], g.bc_prim[2 * ax + 1], &pp[4 * NV], n, NV, false, gr);
+      for (int k = 0; k < 4; k++)
+        for (int v = 0; v < NV; v++) {
+          pp[(3 - k) * NV + v] = gl[k * NV + v];
+          pp[(n + 4 + k) * NV + v] = gr[k * NV + v];
+        }
+      for (int j = 0; j < n + 8; j++) prim_to_cons<T, D>(&pp[j * NV], &cc[j * NV]);
+      // interfaces: half node m (0..n+2) lies between cells m-2 and m-1
+      for (int m = 0; m < n + 3; m++) {
+        const T* wp[6];
+        const T* wc[6];
+        for (int j = 0; j < 6; j++) {
+          int cell = m - 4 + j;  // cells m-4 .. m+1
+          wp[j] = &pp[(cell + 4) * NV];
+          wc[j] = &cc[(cell + 4) * NV];
+        }
+        T L[NV], R[NV];
+        recon_interface<T, D>(g.scheme, g.charwise != 0, wp, wc, ax, L, R);
+        riemann<T, D>(g.flux, L, R, ax, &fh[m * NV]);
+      }
+      for (int m = 0; m < n + 2; m++) node_flux<T, D>(&pp[(m - 1 + 4) * NV], ax, &fn[m * NV]);
+      for (int i = 0; i < n; i++) {
+        T* r = &rhs[(base + i * sa) * NV];
+        for (int v = 0; v < NV; v++) {
+          r[v] += (w1 * (fn[(i + 2) * NV + v] - fn[i * NV + v]) + w0 * (fh[(i + 2) * NV + v] - fh[(i + 1) * NV + v]) +
+                   w2 * (fh[(i + 3) * NV + v] - fh[i * NV + v])) /
+                  hh;
+        }
+      }
+    }
+  }
+}
+
+// Full RHS: setZeros, consToPrim, ghost fill, sweeps x, y, z (rhs = +sum dF/dx).
+template <class T, int D> void rhs_eval(const Grid& g, const T* cons, T* prim, T* rhs, int nthreads) {
+  constexpr int NV = D + 2;
+  const long N = (long)g.n[0] * g.n[1] * g.n[2];
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (long i = 0; i < N; i++) {
+    cons_to_prim<T, D>(&cons[i * NV], &prim[i * NV]);
+    for (int v = 0; v < NV; v++) rhs[i * NV + v] = T(0.0);
+  }
+  for (int ax = 0; ax < D; ax++) sweep<T, D>(g, ax, prim, rhs, nthreads);
+}
+
+// BlockSolver::RK3_SSP (zone_FD/src/src/blockSolver.cpp:43-87).
+template <class T, int D> void rk3_step(const Grid& g, T* cons, double dt, int nthreads) {
+  constexpr int NV = D + 2;
+  const long N = (long)g.n[0] * g.n[1] * g.n[2] * NV;
+  std::vector<T> u0(cons, cons + N), prim(N), rhs(N);
+  rhs_eval<T, D>(g, cons, prim.data(), rhs.data(), nthreads);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (long i = 0; i < N; i++) cons[i] = u0[i] - dt * rhs[i];
+  rhs_eval<T, D>(g, cons, prim.data(), rhs.data(), nthreads);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (long i = 0; i < N; i++) cons[i] = 0.75 * u0[i] - 0.25 * dt * rhs[i] + 0.25 * cons[i];
+  rhs_eval<T, D>(g, cons, prim.data(), rhs.data(), nthreads);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (long i = 0; i < N; i++) cons[i] = 1.0 / 3.0 * u0[i] - 2.0 / 3.0 * dt * rhs[i] + 2.0 / 3.0 * cons[i];
+}
+
+// BlockSolver::getTimeIntervalExplicit (zone_FD/src/src/blockSolver.cpp:318-348):
+// dt = CFL / sum_d max_i (sqrt(gamma p / rho) + |u_d|) / h_d.
+template <class T, int D> double cfl_dt(const Grid& g, const T* cons, double cfl, int nthreads) {
+  constexpr int NV = D + 2;
+  const long N = (long)g.n[0] * g.n[1] * g.n[2];
+  double lam = 0;
+  for (int ax = 0; ax < D; ax++) {
+    double mx = 0;
+#pragma omp parallel for num_threads(nthreads) reduction(max : mx)
+    for (long i = 0; i < N; i++) {
+      T p[NV];
+      cons_to_prim<T, D>(&cons[i * NV], p);
+      double li = (std::sqrt(GAMMA * double(p[NV - 1]) / double(p[0])) + std::abs(double(p[ax + 1]))) / g.h[ax];
+      mx = std::max(mx, li);
+    }
+    lam += mx;
+  }
+  return cfl / lam;
+}
+
+}  // namespace oracle

@@ -1,0 +1,322 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+Every check here is chosen so that a plausible slip in the oracle (a dropped
+term, a wrong sign or index, a transposed operand) fails: the reference's own
+accuracy-test output, the reference's bitwise mirror-symmetry checks,
+closed forms (exactness on polynomials, L.R = I, consistency F(q,q) = F(q))
+and conservation / free-stream invariants.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+SCHEMES = [O.S_JS, O.S_Z, O.S_TENO5, O.S_TENO5_CT4, O.S_TENO5_CT7, O.S_TENOS, O.S_LINEAR]
+
+
+# ---------------------------------------------------------------- accuracy
+def _g(x, k):
+    return math.pow(x, k) * math.exp(0.75 * (x - 1))
+
+
+def _dg(x, k):
+    return math.exp(0.75 * (-1 + x)) * k * math.pow(x, -1 + k) + 0.75 * math.exp(0.75 * (-1 + x)) * math.pow(x, k)
+
+
+def _accuracy_errors(h, k, scheme=O.S_TENO5):
+    """CongFD1.0/AccuracyTest.cpp:14-83 (getError) driven through the oracle."""
+    n = int((1.0 - (-1.0)) / h)
+    gl = [_g(-1.0 - h / 2 - i * h, k) for i in range(10)]
+    gr = [_g(1.0 + h / 2 + i * h, k) for i in range(10)]
+    cells = [_g(-1.0 + h / 2 + i * h, k) for i in range(n)]
+    hh = (1.0 - (-1.0)) / ((n + 1) - 1)
+    r = O.accuracy_rhs(scheme, cells, gl, gr, hh)
+    e = [abs(r[i] - _dg(-1.0 + h / 2 + i * h, k)) for i in range(n)]
+    return sum(e) / n, math.sqrt(sum(x * x for x in e) / n), max(e)
+
+
+def test_accuracy_test_matches_reference_output():
+    rows = [l.split() for l in open(os.path.join(GOLD, "accuracyTest.dat")).read().splitlines()[2:]]
+    checked = 0
+    for row in rows:
+        h = float(row[0])
+        if int(round(2.0 / h)) > 60:
+            continue  # rows n >= 70 are not reproducible (see tests/golden/README.md)
+        got = _accuracy_errors(h, 2.0)
+        for g, ref in zip(got, row[1:4]):
+            assert abs(g / float(ref) - 1) < 2e-7, (h, got, row)
+        checked += 1
+    assert checked == 6
+
+
+def test_accuracy_order_fifth():
+    hs = [2.0 / n for n in (10, 20, 30, 40)]
+    errs = [_accuracy_errors(h, 2.0)[0] for h in hs]
+    orders = [math.log(errs[i] / errs[i + 1]) / math.log(hs[i] / hs[i + 1]) for i in range(3)]
+    ref = [float(l.split()[1]) for l in open(os.path.join(GOLD, "accuracy_order.txt")).read().splitlines()[1:4]]
+    np.testing.assert_allclose(orders, ref, rtol=1e-6)
+
+
+def test_accuracy_test_detects_planted_slip():
+    # the same pipeline with WENO-JS instead of TENO5 must not match the file
+    row = open(os.path.join(GOLD, "accuracyTest.dat")).read().splitlines()[2].split()
+    got = _accuracy_errors(float(row[0]), 2.0, scheme=O.S_JS)
+    assert abs(got[0] / float(row[1]) - 1) > 1e-3
+
+
+# ------------------------------------------------------------ interpolation
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_inter5_exact_for_quadratics(scheme):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        a, b, c = rng.normal(size=3)
+        x = np.arange(5.0)
+        q = a + b * x + c * x * x
+        xe = 2.5
+        want = a + b * xe + c * xe * xe
+        assert abs(O.inter5(scheme, q) - want) < 1e-12 * (1 + abs(want) + abs(c) * 10)
+
+
+@pytest.mark.parametrize("scheme", [O.S_LINEAR, O.S_TENO5])
+def test_inter5_fifth_order_exact_for_quartics(scheme):
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        co = rng.normal(size=5)
+        # resolved data (spacing 0.05): TENO5 keeps every stencil and reduces
+        # to the linear 5th-order interpolant, exact for quartics
+        x = 0.3 + 0.05 * np.arange(5.0)
+        q = np.polyval(co, x)
+        want = np.polyval(co, 0.3 + 0.05 * 2.5)
+        assert abs(O.inter5(scheme, q) - want) < 1e-13 * (1 + np.abs(co).sum())
+
+
+def test_teno_step_selects_smooth_candidate():
+    # step (0,0,0,1,1): stencils 1 and 2 cross the jump -> flag 6 -> u0 = 0
+    assert O.inter5(O.S_TENO5, [0, 0, 0, 1, 1]) == 0.0
+    # mirrored step (1,1,0,0,0) seen from the other side -> u2 of (0,0,0) = 0
+    assert O.inter5(O.S_TENO5, [1, 1, 0, 0, 0]) == 0.0
+    # all three stencils smooth -> linear 5th-order weights
+    q = [1.0, 2.0, 3.0, 4.0, 5.0]
+    assert O.inter5(O.S_TENO5, q) == O.inter5(O.S_LINEAR, q) == 3.5
+
+
+def test_weno_weights_js_known_value():
+    # step data: JS weights give a known convex blend (hand-evaluated formula)
+    q = [0.0, 0.0, 0.0, 1.0, 1.0]
+    b = [0.0, 1.25, 3.25]
+    d = [1 / 16, 5 / 8, 5 / 16]
+    u = [0.0, 0.375, 0.375 * 0 + 0.75 - 0.125]
+    a = [d[i] / (1e-6 + b[i]) ** 2 for i in range(3)]
+    want = sum(a[i] * u[i] for i in range(3)) / sum(a)
+    assert abs(O.inter5(O.S_JS, q) - want) < 1e-15
+
+
+# -------------------------------------------------------------- eigensystem
+def _rand_prim(rng, dim):
+    p = [rng.uniform(0.3, 2.0)] + list(rng.uniform(-1.5, 1.5, size=dim)) + [rng.uniform(0.3, 2.0)]
+    return np.array(p)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_eigen_left_right_inverse(dim):
+    rng = np.random.default_rng(3 + dim)
+    for ax in range(dim):
+        for _ in range(10):
+            pl, pr = _rand_prim(rng, dim), _rand_prim(rng, dim)
+            L, R = O.eigen(dim, pl, pr, ax)
+            np.testing.assert_allclose(L @ R, np.eye(dim + 2), atol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_char_roundtrip_identity(dim):
+    rng = np.random.default_rng(7 + dim)
+    for ax in range(dim):
+        for _ in range(10):
+            pl, pr, p = _rand_prim(rng, dim), _rand_prim(rng, dim), _rand_prim(rng, dim)
+            _, back = O.prim_char_roundtrip(dim, pl, pr, ax, p)
+            np.testing.assert_allclose(back, p, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_eigenvalues_of_roe_matrix(dim):
+    # R diag(lambda) L must equal the Roe-averaged flux Jacobian: check that
+    # L A R is diagonal with (Vn, Vn, Vn-c, Vn+c[, Vn]) where A is built from
+    # finite differences of the physical flux at the Roe state (pl == pr).
+    rng = np.random.default_rng(11)
+    for ax in range(dim):
+        p = _rand_prim(rng, dim)
+        L, R = O.eigen(dim, p, p, ax)
+        nv = dim + 2
+
+        def cons(pp):
+            r = pp[0]
+            vel = pp[1:1 + dim]
+            return np.concatenate([[r], r * vel, [pp[-1] / 0.4 + 0.5 * r * vel @ vel]])
+
+        def flux_of_cons(u):
+            return O.node_flux(dim, O.cons_to_prim(dim, u), ax)
+
+        u0 = cons(p)
+        A = np.zeros((nv, nv))
+        for j in range(nv):
+            e = np.zeros(nv)
+            e[j] = 1e-6 * max(1.0, abs(u0[j]))
+            A[:, j] = (flux_of_cons(u0 + e) - flux_of_cons(u0 - e)) / (2 * e[j])
+        Lam = L @ A @ R
+        c = math.sqrt(1.4 * p[-1] / p[0])
+        vn = p[1 + ax]
+        want = [vn, vn, vn - c, vn + c] + ([vn] if dim == 3 else [])
+        np.testing.assert_allclose(np.diag(Lam), want, atol=2e-6)
+        off = Lam - np.diag(np.diag(Lam))
+        assert np.abs(off).max() < 2e-6
+
+
+# ------------------------------------------------------- reference symmetry
+def test_reference_symmetry_checks_bitwise():
+    """zone_FD/symmetrytest.cpp:11-82 with its own states, bit for bit."""
+    priml = np.array([0.18217903560635557, 0.8040523419013228, 1.206, 0.07123495167920027])
+    primr = np.array([0.52330217813773561, 0.0097235893105526473, 1.2060000000000002, 0.294926010838852])
+    prim = np.array([1.0, 0.05, 0.8, 0.8])
+    sw = lambda s: np.array([s[0], s[2], s[1], s[3]])
+    primlm, primrm, primm = sw(priml), sw(primr), sw(prim)
+
+    Char, primTrans = O.prim_char_roundtrip(2, priml, priml, 0, prim)
+    Charm, primTransm = O.prim_char_roundtrip(2, primlm, primlm, 1, primm)
+    assert Char[0] == -Charm[0] and Char[1] == Charm[1] and Char[2] == Charm[2] and Char[3] == Charm[3]
+    assert primTrans[0] == primTransm[0] and primTrans[2] == primTransm[1] and primTrans[3] == primTransm[3]
+
+    flux = O.riemann(2, O.F_ROE, priml, primr, 0)
+    neg_n = lambda s: np.array([s[0], s[1], -s[2], s[3]])
+    fluxm = O.riemann(2, O.F_ROE, neg_n(primrm), neg_n(primlm), 1)
+    assert flux[0] == -fluxm[0] and flux[2] == -fluxm[1] and flux[3] == -fluxm[3]
+    assert abs(flux[1] - fluxm[2]) <= 1e-15 * abs(flux[1])
+
+    fluxInv = O.riemann(2, O.F_HLLC, primr, priml, 0)
+    fluxInvm = O.riemann(2, O.F_HLLC, primrm, primlm, 1)
+    assert fluxInv[0] == fluxInvm[0] and fluxInv[2] == fluxInvm[1] and fluxInv[3] == fluxInvm[3]
+
+    fluxs = O.node_flux(2, primr, 0)
+    fluxsm = O.node_flux(2, primrm, 1)
+    assert fluxs[0] == fluxsm[0] and fluxs[2] == fluxsm[1] and fluxs[3] == fluxsm[3]
+
+
+# ---------------------------------------------------------- Riemann fluxes
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("flux", [O.F_HLLC, O.F_ROE])
+def test_riemann_consistency(dim, flux):
+    rng = np.random.default_rng(13)
+    for ax in range(dim):
+        for _ in range(10):
+            p = _rand_prim(rng, dim)
+            np.testing.assert_allclose(O.riemann(dim, flux, p, p, ax), O.node_flux(dim, p, ax), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_hllc_supersonic_upwind(dim):
+    rng = np.random.default_rng(17)
+    for ax in range(dim):
+        pl, pr = _rand_prim(rng, dim), _rand_prim(rng, dim)
+        pl[1 + ax] = 6.0
+        pr[1 + ax] = 6.5  # both states supersonic to the right
+        np.testing.assert_allclose(O.riemann(dim, O.F_HLLC, pl, pr, ax), O.node_flux(dim, pl, ax), rtol=1e-13)
+        pl[1 + ax], pr[1 + ax] = -6.5, -6.0
+        np.testing.assert_allclose(O.riemann(dim, O.F_HLLC, pl, pr, ax), O.node_flux(dim, pr, ax), rtol=1e-13)
+
+
+# ---------------------------------------------------------- 3D extension
+@pytest.mark.parametrize("flux", [O.F_HLLC, O.F_ROE])
+@pytest.mark.parametrize("scheme", [O.S_JS, O.S_Z, O.S_TENO5])
+def test_3d_reduces_to_2d_bitwise(flux, scheme):
+    rng = np.random.default_rng(19)
+    nx, ny = 12, 10
+    p2 = np.stack([rng.uniform(0.5, 1.5, (ny, nx)), rng.uniform(-0.5, 0.5, (ny, nx)),
+                   rng.uniform(-0.5, 0.5, (ny, nx)), rng.uniform(0.5, 1.5, (ny, nx))], -1)
+    c2 = np.empty_like(p2)
+    c2[..., 0] = p2[..., 0]
+    c2[..., 1] = p2[..., 0] * p2[..., 1]
+    c2[..., 2] = p2[..., 0] * p2[..., 2]
+    c2[..., 3] = p2[..., 3] / 0.4 + 0.5 * p2[..., 0] * (p2[..., 1] ** 2 + p2[..., 2] ** 2)
+    c3 = np.zeros((1, ny, nx, 5))
+    c3[0, ..., [0, 1, 2, 4]] = np.moveaxis(c2, -1, 0)
+    g2 = O.make_grid(2, (nx, ny), 0.1, scheme, flux, 1, [1, 1, 1, 1])
+    g3 = O.make_grid(3, (nx, ny, 1), 0.1, scheme, flux, 1, [1] * 6)
+    r2, _ = O.rhs(g2, c2.reshape(-1, 4))
+    r3, _ = O.rhs(g3, c3.reshape(-1, 5))
+    assert np.array_equal(r3[:, [0, 1, 2, 4]], r2)
+    assert np.all(r3[:, 3] == 0)
+
+
+# ----------------------------------------------- whole-RHS invariants
+def _smooth_state(dim, shape, rng, amp=0.1):
+    grids = np.meshgrid(*[np.arange(s) * 2 * np.pi / s for s in shape], indexing="ij")
+    def f():
+        ph = rng.uniform(0, 2 * np.pi, dim)
+        k = rng.integers(1, 3, dim)
+        return np.sin(sum(k[i] * grids[i] + ph[i] for i in range(dim)))
+    rho = 1 + amp * f()
+    vel = [0.3 * f() for _ in range(dim)]
+    p = 1 + amp * f()
+    c = [rho] + [rho * v for v in vel] + [p / 0.4 + 0.5 * rho * sum(v * v for v in vel)]
+    arr = np.stack(c, -1)  # indexing (x, y, z): reorder to x fastest
+    arr = np.transpose(arr, list(range(dim))[::-1] + [dim])
+    return np.ascontiguousarray(arr.reshape(-1, dim + 2))
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("flux", [O.F_HLLC, O.F_ROE])
+@pytest.mark.parametrize("charwise", [0, 1])
+def test_conservation_periodic(dim, flux, charwise):
+    rng = np.random.default_rng(23)
+    shape = {1: (32,), 2: (16, 12), 3: (8, 6, 10)}[dim]
+    c = _smooth_state(dim, shape, rng)
+    g = O.make_grid(dim, shape, 0.05, O.S_TENO5, flux, charwise, [1] * 6)
+    r, _ = O.rhs(g, c)
+    tot = r.sum(0)
+    scale = np.abs(r).sum(0) + 1e-30
+    assert np.all(np.abs(tot) / scale < 1e-13)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("scheme", SCHEMES[:6])
+def test_freestream_preserved(dim, scheme):
+    shape = {1: (16,), 2: (9, 7), 3: (5, 6, 7)}[dim]
+    n = int(np.prod(shape))
+    vel = [0.3, -0.2, 0.1][:dim]
+    c = np.array([1.2] + [1.2 * v for v in vel] + [1.0 / 0.4 + 0.5 * 1.2 * sum(v * v for v in vel)])
+    cons = np.tile(c, (n, 1))
+    for flux in (O.F_HLLC, O.F_ROE):
+        for bc in (1, 7):
+            g = O.make_grid(dim, shape, 0.1, scheme, flux, 1, [bc] * 6)
+            r, _ = O.rhs(g, cons)
+            assert np.all(r == 0.0)
+
+
+def test_rhs_transpose_symmetry_2d():
+    """Swapping x<->y (and u<->v) maps the RHS onto itself."""
+    rng = np.random.default_rng(29)
+    n = 14
+    c = _smooth_state(2, (n, n), rng).reshape(n, n, 4)  # [y][x][v]
+    ct = np.transpose(c, (1, 0, 2))[..., [0, 2, 1, 3]].copy()
+    for flux in (O.F_HLLC, O.F_ROE):
+        g = O.make_grid(2, (n, n), 0.1, O.S_TENO5, flux, 1, [1] * 4)
+        r, _ = O.rhs(g, c.reshape(-1, 4))
+        rt, _ = O.rhs(g, ct.reshape(-1, 4))
+        back = np.transpose(rt.reshape(n, n, 4), (1, 0, 2))[..., [0, 2, 1, 3]]
+        np.testing.assert_allclose(back, r.reshape(n, n, 4), rtol=0, atol=1e-12 * np.abs(r).max())
+
+
+def test_mnd6_exact_for_quartic_flux():
+    # Linear5 half nodes are exact for degree <= 4 and MND6 differentiates
+    # exactly, so the derivative of a quartic is recovered to round-off.
+    co = np.array([0.3, -0.2, 0.5, 1.0, 2.0])
+    h = 0.05
+    n = 20
+    xc = -0.5 + h / 2 + h * np.arange(n)
+    gl = np.polyval(co, -0.5 - h / 2 - h * np.arange(10))
+    gr = np.polyval(co, 0.5 + h / 2 + h * np.arange(10))
+    r = O.accuracy_rhs(O.S_LINEAR, np.polyval(co, xc), gl, gr, h)
+    np.testing.assert_allclose(r, np.polyval(np.polyder(co), xc), rtol=0, atol=1e-11)
