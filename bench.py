@@ -1,0 +1,147 @@
+"""Benchmark: Mcell-stage/s of the CongFD RHS + SSP-RK3 step on one MI355X.
+
+Workload (default): 2D isentropic vortex on a periodic 2048^2 grid, FP64,
+WCNS5-Z characteristic-wise + Roe flux + MND6, SSP-RK3 with the CFL time step
+computed on the device every step.  A "step" is one full RK3 step (three
+stages of consToPrim + ghost fill + x/y sweeps + stage update, plus the CFL
+reduction).  Mcell-stage/s = cells * 3 * steps / seconds / 1e6.
+
+The same JSON line carries the 3D 256^3 periodic-flow figure (TENO5
+characteristic-wise + HLLC) under "secondary", the live roofline of the
+sweep kernel (HIP events on the solver's stream) and a CPU baseline (the
+in-repo oracle on the host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from cfd_inwsl_amd import BC, FluxMethod, InterMethod, Recon, Solver, cases  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measure(sol, nv, dim, ncell, steps, warmup, cfl):
+    sol.rk3_steps_cfl(cfl, warmup, want_time=False)
+    sol.synchronize()
+    t0 = time.perf_counter()
+    sol.rk3_steps_cfl(cfl, steps, want_time=False)
+    sol.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    value = ncell * 3 * steps / sec / 1e6
+    # dominant kernel: the direction sweep; algorithmic bytes per cell and
+    # direction = 24*nv (direction-split streams, FP64): per-launch bytes
+    # = 24*nv*ncell, averaged over the dim sweep launches of a stage
+    ms = sol.time_sweeps(reps=5)
+    mean_ms = float(np.mean(ms))
+    bytes_per_launch = 24 * nv * ncell
+    achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "sweep_kernel", "sweep_ms": [round(float(x), 4) for x in ms],
+            "bytes_per_launch": bytes_per_launch}
+    return value, sec * 1e3 / steps, roof
+
+
+def cpu_baseline(seconds_target=15.0):
+    """In-repo oracle (OpenMP over pencils, like SpDistributor::rhsSolve) timed
+    on the host cores on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    n = 2048
+    cons, h = cases.vortex(n)
+    grid = O.make_grid(2, (n, n), h, O.S_Z, O.F_ROE, 1, [1] * 4)
+    # one step first to size the sample
+    t0 = time.perf_counter()
+    cur, _ = O.rk3(grid, cons, cfl=0.5, nsteps=1, nthreads=threads)
+    one = time.perf_counter() - t0
+    steps = max(1, min(8, int(seconds_target / max(one, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    O.rk3(grid, cur, cfl=0.5, nsteps=steps, nthreads=threads)
+    sec = time.perf_counter() - t0
+    val = n * n * 3 * steps / sec / 1e6
+    return {"value": round(val, 3), "unit": "Mcell-stage/s", "cores": threads, "kind": "port",
+            "sample": f"vortex 2048^2 WCNS5-Z char + Roe, {steps} CFL RK3 steps after 1 untimed"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D 256^3 secondary figure")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the path shards by slabs with a halo exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    cfl = 0.5
+    n = 2048
+    log(f"[bench] 2D vortex {n}^2 setup")
+    cons, h = cases.vortex(n)
+    sol = Solver(2, (n, n), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, [BC.PERIODIC] * 4)
+    sol.set_state(cons)
+    log(f"[bench] timing {args.steps} steps after {args.warmup} warmup")
+    value, ms_step, roof = measure(sol, 4, 2, n * n, args.steps, args.warmup, cfl)
+    sol.close()
+    log(f"[bench] 2D: {value:.1f} Mcell-stage/s, {ms_step:.3f} ms/step, roofline {roof['frac']:.3f}")
+
+    secondary = None
+    if not args.no_3d:
+        n3 = 256
+        log(f"[bench] 3D periodic {n3}^3 setup")
+        c3, h3 = cases.periodic3d(n3)
+        s3 = Solver(3, (n3, n3, n3), h3, InterMethod.TCNS5, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 6)
+        s3.set_state(c3)
+        st3 = max(2, args.steps // 4)
+        v3, ms3, roof3 = measure(s3, 5, 3, n3 ** 3, st3, max(1, args.warmup // 2), cfl)
+        s3.close()
+        log(f"[bench] 3D: {v3:.1f} Mcell-stage/s, {ms3:.3f} ms/step, roofline {roof3['frac']:.3f}")
+        secondary = {"workload": "3D periodic random-phase flow 256^3, TENO5 char-wise + HLLC, FP64",
+                     "value": round(v3, 2), "unit": "Mcell-stage/s", "ms_per_step": round(ms3, 4),
+                     "steps": st3, "roofline": roof3}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline()
+        log(f"[bench] CPU: {cpu['value']} Mcell-stage/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "Mcell-stage/s at 2D 2048² & 3D 256³ (% HBM roofline); adjoint÷forward time",
+        "value": round(value, 2),
+        "unit": "Mcell-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D isentropic vortex 2048^2 periodic, WCNS5-Z char-wise + Roe + MND6, "
+                               "SSP-RK3 with device CFL dt (CFL 0.5)",
+                   "cells": n * n, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,298 @@
+// Direction-sweep kernel: one pass of SpDistributor::rhsSolve along axis AX
+// (zone_FD/src/src/sp_distributor.cpp:9-78) fused with the ghost fill
+// (OneDBnd::update, zone_FD/src/src/oneDBnd.cpp:31-114), the per-stage
+// consToPrim (zone_FD/src/src/equation.cpp:8-99), the MND6 flux derivative
+// (MidNodeAndNodeDif6::solve, zone_FD/src/src/differ.cpp:54-103) and, on the
+// last direction, the SSP-RK3 stage update (BlockSolver::RK3_SSP,
+// zone_FD/src/src/blockSolver.cpp:43-87).
+//
+// Work decomposition (MI355X, wave64):
+//   * a workgroup of 256 lanes owns CP pencils and a segment [c0, c1) of
+//     their cells along AX, and marches along AX in chunks of CA interfaces;
+//   * lane (a, p) loads cell m0+1+a of pencil p, converts it to primitive
+//     variables once and keeps (prim, rebuilt cons) in an LDS ring buffer;
+//   * lane (a, p) then computes interface m0+a (characteristic projection,
+//     nonlinear interpolation, Riemann flux) from six ring cells, writes the
+//     flux to an LDS flux ring, and finally updates cell m0-3+a.
+//   For AX = 0 the lanes run along x (CA = 32, CP = 8), for AX = 1, 2 they run
+//   across x-adjacent pencils (CA = 16, CP = 16): every global access is a
+//   coalesced row segment.  A segment computes n+3 interfaces for n cells
+//   (the reference's per-pencil count), so redundancy is 3 interfaces per
+//   segment.
+#pragma once
+#include "euler_point.hpp"
+
+namespace cfd {
+
+enum BcCode {
+  BC_PERIODIC = 1,
+  BC_SYMMETRY1D = 2,
+  BC_DIRICHLET = 3,
+  BC_OUTFLOW = 7,
+  BC_SYMMETRYX = 10,
+  BC_SYMMETRYY = 11,
+  BC_SYMMETRYZ = 101,
+  BC_MIRROR_COPY = 102,
+};
+
+// flags of a sweep launch
+enum SweepFlags { SW_READ_R = 1, SW_WRITE_R = 2, SW_RK = 4 };
+
+struct SweepArgs {
+  const double* U;   // state whose RHS is evaluated (SoA planes of N)
+  double* R;         // RHS accumulator (SoA)
+  const double* U0;  // RK: state at the start of the step
+  double* Uout;      // RK: stage output
+  const double* dtp; // RK: device time step
+  long N;            // cells per variable plane
+  int n[3];          // cells per direction
+  int flags;
+  int stage;         // 1, 2, 3
+  int seg_len;       // cells per segment along AX
+  double h;          // spacing along AX
+  double ct;         // TENO cut-off constant
+  int bc_lo, bc_hi;
+  double prim_lo[5], prim_hi[5];
+};
+
+template <int AX> struct Tile;
+template <> struct Tile<0> { static constexpr int CA = 32, CP = 8; };
+template <> struct Tile<1> { static constexpr int CA = 16, CP = 16; };
+template <> struct Tile<2> { static constexpr int CA = 16, CP = 16; };
+
+template <int D, int AX, int SCH, int FLX, bool CHARW>
+__global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
+  constexpr int NV = D + 2;
+  constexpr int CA = Tile<AX>::CA, CP = Tile<AX>::CP;
+  constexpr int RC = CA + 5;  // cell ring slots
+  constexpr int FR = CA + 3;  // flux ring slots
+  constexpr int NCV = CHARW ? 2 * NV : NV;  // per cell: prim [+ rebuilt cons]
+  // LDS layout: [var][slot][pencil] for AX>0 (pencil fastest = lane fastest),
+  //             [var][pencil][slot] for AX=0 (slot fastest = lane fastest).
+  __shared__ double cell[NCV * RC * CP];
+  __shared__ double flux[NV * FR * CP];
+  auto cidx = [](int v, int slot, int p) -> int {
+    return AX == 0 ? (v * CP + p) * RC + slot : (v * RC + slot) * CP + p;
+  };
+  auto fidx = [](int v, int slot, int p) -> int {
+    return AX == 0 ? (v * CP + p) * FR + slot : (v * FR + slot) * CP + p;
+  };
+
+  const int lane = threadIdx.x;
+  const int la = AX == 0 ? (lane & (CA - 1)) : (lane / CP);
+  const int lp = AX == 0 ? (lane / CA) : (lane & (CP - 1));
+
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int n = a.n[AX];
+  const long sa = AX == 0 ? 1 : (AX == 1 ? (long)nx : (long)nx * ny);
+  const long npencil = AX == 0 ? (long)ny * nz : (AX == 1 ? (long)nx * nz : (long)nx * ny);
+  const long gp = (long)blockIdx.x * CP + lp;
+  const bool pvalid = gp < npencil;
+  long base = 0;
+  if (pvalid) {
+    if (AX == 0) base = gp * nx;
+    else if (AX == 1) base = (gp % nx) + (gp / nx) * (long)nx * ny;
+    else base = gp;
+  }
+  const int c0 = blockIdx.y * a.seg_len;
+  const int c1 = min(c0 + a.seg_len, n);
+
+  // ---- load one cell (position g along AX, may be a ghost) into the ring ----
+  auto load_cell = [&](int g) {
+    double pr[NV];
+    int flip = -1;
+    bool dir = false;
+    int src = g;
+    if (g < 0 || g >= n) {
+      const bool lo = g < 0;
+      const int bc = lo ? a.bc_lo : a.bc_hi;
+      switch (bc) {
+      case BC_PERIODIC: src = ((g % n) + n) % n; break;
+      case BC_OUTFLOW: src = lo ? 0 : n - 1; break;
+      case BC_DIRICHLET: dir = true; break;
+      case BC_SYMMETRY1D:
+      case BC_SYMMETRYX: src = lo ? -1 - g : 2 * n - 1 - g; flip = 1; break;
+      case BC_SYMMETRYY: src = lo ? -1 - g : 2 * n - 1 - g; flip = 2; break;
+      case BC_SYMMETRYZ: src = lo ? -1 - g : 2 * n - 1 - g; flip = 3; break;
+      default: src = lo ? -1 - g : 2 * n - 1 - g; break;  // mirror copy
+      }
+      if (dir)
+        for (int v = 0; v < NV; v++) pr[v] = lo ? a.prim_lo[v] : a.prim_hi[v];
+    }
+    if (!dir) {
+      double cs[NV];
+      const long idx = base + (long)src * sa;
+      for (int v = 0; v < NV; v++) cs[v] = a.U[v * a.N + idx];
+      cons_to_prim<D>(cs, pr);
+      if (flip > 0 && flip < NV - 1) pr[flip] = -pr[flip];
+    }
+    const int slot = (g + 8 * RC) % RC;
+    for (int v = 0; v < NV; v++) cell[cidx(v, slot, lp)] = pr[v];
+    if (CHARW) {
+      double cc[NV];
+      prim_to_cons<D>(pr, cc);
+      for (int v = 0; v < NV; v++) cell[cidx(NV + v, slot, lp)] = cc[v];
+    }
+  };
+
+  // preload cells c0-4 .. c0
+  if (la < 5 && pvalid) load_cell(c0 - 4 + la);
+
+  const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
+  const int m_end = c1 + 3;  // interfaces [c0, c1+3)
+  for (int m0 = c0; m0 < m_end; m0 += CA) {
+    // 1. load cells m0+1 .. m0+CA (up to c1+3)
+    {
+      const int g = m0 + 1 + la;
+      if (pvalid && g <= c1 + 3) load_cell(g);
+    }
+    __syncthreads();
+    // 2. interface m = m0 + la: between cells m-1 and m... (cells m-4 .. m+1)
+    {
+      const int m = m0 + la;
+      if (pvalid && m < m_end) {
+        // window cells m-4+j, j = 0..5; the interface lies between m-2 and m-1
+        int sl[6];
+        for (int j = 0; j < 6; j++) sl[j] = (m - 4 + j + 8 * RC) % RC;
+        double outL[NV], outR[NV];
+        if (!CHARW) {
+          for (int v = 0; v < NV; v++) {
+            double q0 = cell[cidx(v, sl[0], lp)], q1 = cell[cidx(v, sl[1], lp)], q2 = cell[cidx(v, sl[2], lp)];
+            double q3 = cell[cidx(v, sl[3], lp)], q4 = cell[cidx(v, sl[4], lp)], q5 = cell[cidx(v, sl[5], lp)];
+            outL[v] = inter5<SCH>(q0, q1, q2, q3, q4, a.ct);
+            outR[v] = inter5<SCH>(q5, q4, q3, q2, q1, a.ct);
+          }
+        } else {
+          double pL[NV], pR[NV];
+          for (int v = 0; v < NV; v++) {
+            pL[v] = cell[cidx(v, sl[2], lp)];
+            pR[v] = cell[cidx(v, sl[3], lp)];
+          }
+          if (D == 1) {
+            RoeSys1 es;
+            es.build(pL, pR);
+            double QL[3], QR[3];
+            for (int k = 0; k < 3; k++) {
+              double ch[6];
+              for (int j = 0; j < 6; j++) {
+                double c0v = cell[cidx(NV + 0, sl[j], lp)], c1v = cell[cidx(NV + 1, sl[j], lp)],
+                       c2v = cell[cidx(NV + 2, sl[j], lp)];
+                ch[j] = c0v * es.L[3 * k] + c1v * es.L[3 * k + 1] + c2v * es.L[3 * k + 2];
+              }
+              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], a.ct);
+              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], a.ct);
+            }
+            double cL[3], cR[3];
+            for (int i = 0; i < 3; i++) {
+              cL[i] = QL[0] * es.R[3 * i] + QL[1] * es.R[3 * i + 1] + QL[2] * es.R[3 * i + 2];
+              cR[i] = QR[0] * es.R[3 * i] + QR[1] * es.R[3 * i + 1] + QR[2] * es.R[3 * i + 2];
+            }
+            double consL[3], consR[3];
+            prim_to_cons<1>(pL, consL);
+            prim_to_cons<1>(pR, consR);
+            positive_limit(consL, cL, 0);
+            positive_limit(consR, cR, 0);
+            positive_limit(consL, cL, 1);
+            positive_limit(consR, cR, 1);
+            eig_cons_to_prim<1>(cL, outL);
+            eig_cons_to_prim<1>(cR, outR);
+          } else {
+            RoeSys<D, AX> es;
+            es.build(pL, pR);
+            double cw[6][NV];
+            for (int j = 0; j < 6; j++)
+              for (int v = 0; v < NV; v++) cw[j][v] = cell[cidx(NV + v, sl[j], lp)];
+            double QL[NV], QR[NV];
+            for (int k = 0; k < NV; k++) {
+              double l[NV];
+              es.lrow(k, l);
+              double ch[6];
+              for (int j = 0; j < 6; j++) ch[j] = project<D>(cw[j], l);
+              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], a.ct);
+              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], a.ct);
+            }
+            // charToCons in the summation order (ch3, ch4, ch2, ch1[, ch5])
+            double cL[NV], cR[NV];
+            const int order[5] = {2, 3, 1, 0, 4};
+            for (int t = 0; t < NV; t++) {
+              const int k = order[t];
+              double r[NV];
+              es.rcol(k, r);
+              for (int i = 0; i < NV; i++) {
+                if (t == 0) {
+                  cL[i] = QL[k] * r[i];
+                  cR[i] = QR[k] * r[i];
+                } else {
+                  cL[i] = cL[i] + QL[k] * r[i];
+                  cR[i] = cR[i] + QR[k] * r[i];
+                }
+              }
+            }
+            eig_cons_to_prim<D>(cL, outL);
+            eig_cons_to_prim<D>(cR, outR);
+            // positivity fallback (reconstructor5order.hpp:301-302)
+            if (outL[0] < 0 || outL[NV - 1] < 0)
+              for (int v = 0; v < NV; v++) outL[v] = pL[v];
+            if (outR[0] < 0 || outR[NV - 1] < 0) {
+              outR[0] = pR[0];
+              for (int v = 1; v < NV - 1; v++) outR[v] = pL[v];
+              outR[NV - 1] = pR[NV - 1];
+            }
+          }
+        }
+        double F[NV];
+        if (D == 1) {
+          if (FLX == FLX_HLLC) hllc1d(outL, outR, F);
+          else roe1d(outL, outR, F);
+        } else {
+          if (FLX == FLX_HLLC) hllc_md<D, AX>(outL, outR, F);
+          else roe_md<D, AX>(outL, outR, F);
+        }
+        const int fs = m % FR;
+        for (int v = 0; v < NV; v++) flux[fidx(v, fs, lp)] = F[v];
+      }
+    }
+    __syncthreads();
+    // 3. cell i = m0-3+la uses interfaces i .. i+3 and node fluxes at i-1, i+1
+    {
+      const int i = m0 - 3 + la;
+      if (pvalid && i >= c0 && i < c1) {
+        double pm[NV], pp[NV], fm[NV], fp[NV];
+        const int sm = (i - 1 + 8 * RC) % RC, sp = (i + 1 + 8 * RC) % RC;
+        for (int v = 0; v < NV; v++) {
+          pm[v] = cell[cidx(v, sm, lp)];
+          pp[v] = cell[cidx(v, sp, lp)];
+        }
+        node_flux<D, AX>(pm, fm);
+        node_flux<D, AX>(pp, fp);
+        const int f0 = i % FR, f1 = (i + 1) % FR, f2 = (i + 2) % FR, f3 = (i + 3) % FR;
+        const long idx = base + (long)i * sa;
+        double dtv = 0.0;
+        if (a.flags & SW_RK) dtv = *a.dtp;
+        for (int v = 0; v < NV; v++) {
+          const double H0 = flux[fidx(v, f0, lp)], H1 = flux[fidx(v, f1, lp)];
+          const double H2 = flux[fidx(v, f2, lp)], H3 = flux[fidx(v, f3, lp)];
+          const double term = (w1 * (fp[v] - fm[v]) + w0 * (H2 - H1) + w2 * (H3 - H0)) / a.h;
+          const double rold = (a.flags & SW_READ_R) ? a.R[v * a.N + idx] : 0.0;
+          const double r = rold + term;
+          if (a.flags & SW_WRITE_R) a.R[v * a.N + idx] = r;
+          if (a.flags & SW_RK) {
+            const double u0 = a.U0[v * a.N + idx];
+            double un;
+            if (a.stage == 1) {
+              un = u0 - dtv * r;
+            } else {
+              const double uc = a.U[v * a.N + idx];
+              if (a.stage == 2) un = 0.75 * u0 - 0.25 * dtv * r + 0.25 * uc;
+              else un = 1.0 / 3.0 * u0 - 2.0 / 3.0 * dtv * r + 2.0 / 3.0 * uc;
+            }
+            a.Uout[v * a.N + idx] = un;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace cfd

@@ -1,0 +1,227 @@
+"""ctypes binding of ``libcongfd.so`` (C ABI: ``include/congfd.h``).
+
+``Solver`` mirrors the reference's ``BlockSolver`` (zone_FD/src/src/blockSolver.cpp)
+for the right-hand side and SSP-RK3 path: it owns the device state, evaluates
+``rhs = +sum_d dF_d/dx_d`` (``SpDistributor::rhsSolve``) and advances with
+``RK3_SSP``.  Enum values follow the reference (zone_FD/src/include/macro.hpp).
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class InterMethod(enum.IntEnum):
+    """InterMethod (zone_FD/src/include/macro.hpp:37-77), implemented subset."""
+    WCNS5 = 2        # weno5_JSchen
+    WCNSZ5 = 3       # weno5_Z
+    WCNS5CONGZ = 8   # Teno5_CongZ (TENO-S)
+    TCNS5 = 11       # Teno5_Z (TENO5)
+    TCNS5CT4 = 12
+    TCNS5CT7 = 13
+
+
+class FluxMethod(enum.IntEnum):
+    """FluxMethod (macro.hpp:84)."""
+    HLLC = 0
+    ROE = 1
+
+
+class Recon(enum.IntEnum):
+    CHAR = 0        # characteristic-wise (the reference's Euler path)
+    COMPONENT = 1   # component-wise on primitive variables
+
+
+class BC(enum.IntEnum):
+    """BndType values (macro.hpp:33-34) plus extensions."""
+    PERIODIC = 1
+    SYMMETRY1D = 2
+    DIRICHLET = 3
+    OUTFLOW = 7       # SUPERSONICOUTLET
+    SYMMETRYX = 10
+    SYMMETRYY = 11
+    SYMMETRYZ = 101
+    MIRROR_COPY = 102
+
+
+DIFF_MND6 = 3
+
+
+class cfd_config(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("n", ctypes.c_int * 3),
+        ("h", ctypes.c_double * 3),
+        ("inter_method", ctypes.c_int),
+        ("flux_method", ctypes.c_int),
+        ("diff_method", ctypes.c_int),
+        ("recon", ctypes.c_int),
+        ("bc", ctypes.c_int * 6),
+        ("bc_prim", (ctypes.c_double * 5) * 6),
+        ("device", ctypes.c_int),
+    ]
+
+
+# Every entry point declared in include/congfd.h.
+EXPORTS = [
+    "cfd_last_error", "cfd_version", "cfd_create", "cfd_destroy", "cfd_set_state", "cfd_get_state",
+    "cfd_eval_rhs", "cfd_rk3_step", "cfd_rk3_steps_cfl", "cfd_cfl_dt", "cfd_conserved_totals",
+    "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps",
+]
+
+
+def library_path():
+    return os.path.join(_HERE, "libcongfd.so")
+
+
+_lib = None
+
+
+def load_library():
+    """Loads the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
+                           "(make -C cfd_inwsl_amd/csrc)")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    pd = ctypes.POINTER(ctypes.c_double)
+    lib.cfd_last_error.restype = ctypes.c_char_p
+    lib.cfd_version.restype = ctypes.c_char_p
+    lib.cfd_create.argtypes = [ctypes.POINTER(cfd_config), ctypes.POINTER(P)]
+    lib.cfd_destroy.argtypes = [P]
+    lib.cfd_set_state.argtypes = [P, pd]
+    lib.cfd_get_state.argtypes = [P, pd]
+    lib.cfd_eval_rhs.argtypes = [P, pd]
+    lib.cfd_rk3_step.argtypes = [P, ctypes.c_double]
+    lib.cfd_rk3_steps_cfl.argtypes = [P, ctypes.c_double, ctypes.c_int, pd]
+    lib.cfd_cfl_dt.argtypes = [P, ctypes.c_double, pd]
+    lib.cfd_conserved_totals.argtypes = [P, pd]
+    lib.cfd_synchronize.argtypes = [P]
+    lib.cfd_device_buffers.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
+    lib.cfd_time_sweeps.argtypes = [P, ctypes.c_int, pd]
+    _lib = lib
+    return lib
+
+
+def _pd(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class CfdError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One structured block of 1D/2D/3D Euler on one GPU.
+
+    Parameters mirror the reference ``Info`` fields used on this path:
+    ``n`` = cells per direction (``Info::icMax``), ``h`` = spacing
+    (``Info::geth``), ``inter_method`` / ``flux_method`` (``SolverType``
+    selection), boundary codes per face (x-, x+, y-, y+, z-, z+).
+    """
+
+    def __init__(self, dim, n, h, inter_method=InterMethod.WCNS5, flux_method=FluxMethod.ROE,
+                 recon=Recon.CHAR, bc=None, bc_prim=None, device=0):
+        self.lib = load_library()
+        self.dim = dim
+        self.nv = dim + 2
+        nn = list(n) + [1] * (3 - len(n))
+        hh = list(h) if hasattr(h, "__len__") else [h] * dim
+        hh = hh + [1.0] * (3 - len(hh))
+        self.n = tuple(int(x) for x in nn)
+        self.ncell = self.n[0] * self.n[1] * self.n[2]
+        cfg = cfd_config()
+        cfg.dim = dim
+        for i in range(3):
+            cfg.n[i] = self.n[i]
+            cfg.h[i] = float(hh[i])
+        cfg.inter_method = int(inter_method)
+        cfg.flux_method = int(flux_method)
+        cfg.diff_method = DIFF_MND6
+        cfg.recon = int(recon)
+        bc = list(bc) if bc is not None else [BC.OUTFLOW] * 6
+        bc = bc + [BC.OUTFLOW] * (6 - len(bc))
+        for i in range(6):
+            cfg.bc[i] = int(bc[i])
+            for v in range(5):
+                cfg.bc_prim[i][v] = float(bc_prim[i][v]) if bc_prim is not None else 0.0
+        cfg.device = device
+        self.cfg = cfg
+        handle = ctypes.c_void_p()
+        self._check(self.lib.cfd_create(ctypes.byref(cfg), ctypes.byref(handle)))
+        self.h = handle
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CfdError(f"libcongfd error {rc}: {self.lib.cfd_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state in the reference layout: AoS [cell][nv], x fastest
+    def set_state(self, cons):
+        c = np.ascontiguousarray(cons, dtype=np.float64).reshape(self.ncell, self.nv)
+        self._check(self.lib.cfd_set_state(self.h, _pd(c)))
+
+    def get_state(self):
+        out = np.empty((self.ncell, self.nv))
+        self._check(self.lib.cfd_get_state(self.h, _pd(out)))
+        return out
+
+    def rhs(self, download=True):
+        if not download:
+            self._check(self.lib.cfd_eval_rhs(self.h, None))
+            return None
+        out = np.empty((self.ncell, self.nv))
+        self._check(self.lib.cfd_eval_rhs(self.h, _pd(out)))
+        return out
+
+    def rk3_step(self, dt):
+        self._check(self.lib.cfd_rk3_step(self.h, float(dt)))
+
+    def rk3_steps_cfl(self, cfl, nsteps, want_time=True):
+        if want_time:
+            t = ctypes.c_double(0)
+            self._check(self.lib.cfd_rk3_steps_cfl(self.h, float(cfl), int(nsteps), ctypes.byref(t)))
+            return t.value
+        self._check(self.lib.cfd_rk3_steps_cfl(self.h, float(cfl), int(nsteps), None))
+        return None
+
+    def cfl_dt(self, cfl):
+        d = ctypes.c_double(0)
+        self._check(self.lib.cfd_cfl_dt(self.h, float(cfl), ctypes.byref(d)))
+        return d.value
+
+    def totals(self):
+        out = np.empty(self.nv)
+        self._check(self.lib.cfd_conserved_totals(self.h, _pd(out)))
+        return out
+
+    def synchronize(self):
+        self._check(self.lib.cfd_synchronize(self.h))
+
+    def time_sweeps(self, reps=5):
+        """Mean duration (ms) of each direction sweep of one RK stage (HIP events)."""
+        out = np.empty(self.dim)
+        self._check(self.lib.cfd_time_sweeps(self.h, int(reps), _pd(out)))
+        return out
+
+    def device_buffers(self):
+        s, r = ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(self.lib.cfd_device_buffers(self.h, ctypes.byref(s), ctypes.byref(r)))
+        return s.value, r.value

@@ -1,0 +1,131 @@
+/* C ABI of the MI355X CongFD right-hand side and SSP-RK3 time advance.
+ *
+ * The reference solver (wanghongfei23/CFD_inWSL, zone_FD) has no FFI: its
+ * seam is SpDistributor::rhsSolve() (zone_FD/src/src/sp_distributor.cpp:3),
+ * called from BlockSolver::RK3_SSP (zone_FD/src/src/blockSolver.cpp:43-87)
+ * after Equation::consToPrim and Bnds::update.  Each entry point below names
+ * the reference routine it stands in for.
+ *
+ * Conventions
+ *   - All functions return 0 on success and a negative CFD_E_* code on error;
+ *     cfd_last_error() returns a message for the calling thread.
+ *   - Host arrays are in the reference's layout: array-of-structures
+ *     [cell][nv] in double precision with x fastest, cell index
+ *     i + j*nx + k*nx*ny (zone_FD/src/include/macro.hpp:93-96).
+ *     nv = dim + 2: (rho, rho u[, rho v[, rho w]], rho E).
+ *   - A handle owns one HIP stream and its device buffers (structure of
+ *     arrays, one plane per variable).  Handles are not thread-safe; calls on
+ *     one handle are ordered on its stream.
+ *   - The right-hand side follows the reference's sign: rhs = +sum_d dF_d/dx_d,
+ *     so dU/dt = -rhs.
+ */
+#ifndef CONGFD_H
+#define CONGFD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_OK 0
+#define CFD_E_ARG (-1)     /* invalid argument or configuration */
+#define CFD_E_HIP (-2)     /* HIP runtime error */
+#define CFD_E_UNSUPP (-3)  /* scheme / option not implemented */
+
+/* Reference enum values (zone_FD/src/include/macro.hpp:33-90). */
+#define CFD_FLUX_HLLC 0     /* FluxMethod::HLLC */
+#define CFD_FLUX_ROE 1      /* FluxMethod::ROE */
+#define CFD_DIFF_MND6 3     /* DiffMethod::MND6 */
+#define CFD_INTER_WCNS5 2   /* InterMethod::WCNS5 -> weno5_JSchen */
+#define CFD_INTER_WCNSZ5 3  /* InterMethod::WCNSZ5 -> weno5_Z */
+#define CFD_INTER_WCNS5CONGZ 8 /* -> Teno5_CongZ (TENO-S) */
+#define CFD_INTER_TCNS5 11  /* -> Teno5_Z (TENO5) */
+#define CFD_INTER_TCNS5CT4 12
+#define CFD_INTER_TCNS5CT7 13
+
+/* Reconstruction variables.  The reference's Euler path is always
+ * characteristic-wise (SolverType::initReconer, zone_FD/src/src/solverType.cpp:217-594);
+ * component-wise applies Recon5OrderFaceCenter (reconstructor5order.hpp:128-151)
+ * to the primitive variables. */
+#define CFD_RECON_CHAR 0
+#define CFD_RECON_COMPONENT 1
+
+/* Boundary codes: reference BndType values (macro.hpp:33-34) plus extensions. */
+#define CFD_BC_PERIODIC 1        /* true periodic halo */
+#define CFD_BC_SYMMETRY1D 2      /* mirror, u negated */
+#define CFD_BC_DIRICHLET 3       /* fixed primitive ghost state (bc_prim) */
+#define CFD_BC_OUTFLOW 7         /* SUPERSONICOUTLET: edge cell copied */
+#define CFD_BC_SYMMETRYX 10      /* mirror, u negated */
+#define CFD_BC_SYMMETRYY 11      /* mirror, v negated */
+#define CFD_BC_SYMMETRYZ 101     /* extension: mirror, w negated */
+#define CFD_BC_MIRROR_COPY 102   /* the reference's 2D Euler PERIODIC1D wiring
+                                    (initializer.cpp:806-822): mirror, no flip */
+
+typedef struct cfd_config {
+  int dim;              /* 1, 2 or 3 (Info::dim) */
+  int n[3];             /* cells per direction (Info::icMax(); unused = 1) */
+  double h[3];          /* spacing per direction (Info::geth; the reference uses
+                           one spacing for every direction) */
+  int inter_method;     /* Info::interMethod */
+  int flux_method;      /* Info::fluxMethod */
+  int diff_method;      /* Info::diffMethod (MND6 only) */
+  int recon;            /* CFD_RECON_* */
+  int bc[6];            /* per face: x-, x+, y-, y+, z-, z+ */
+  double bc_prim[6][5]; /* Dirichlet ghost state per face, primitive variables */
+  int device;           /* HIP device ordinal */
+} cfd_config;
+
+typedef struct cfd_handle cfd_handle;
+
+const char* cfd_last_error(void);
+const char* cfd_version(void);
+
+/* Builds the solver: stands in for BlockSolver(Info*) with
+ * SolverType(info) scheme selection (zone_FD/src/src/blockSolver.cpp:24-41,
+ * solverType.cpp:20-27) and Initializer::initBnds (initializer.cpp:743-952). */
+int cfd_create(const cfd_config* cfg, cfd_handle** out);
+int cfd_destroy(cfd_handle* h);
+
+/* Conservative state in / out (Equation::cons, equation.hpp). */
+int cfd_set_state(cfd_handle* h, const double* cons_aos);
+int cfd_get_state(cfd_handle* h, double* cons_aos);
+
+/* rhs of the current state: consToPrim + Bnds::update + SpDistributor::rhsSolve
+ * (blockSolver.cpp:51-54).  rhs_aos may be NULL (result stays on the device). */
+int cfd_eval_rhs(cfd_handle* h, double* rhs_aos);
+
+/* One SSP-RK3 step with a given dt: BlockSolver::RK3_SSP (blockSolver.cpp:43-87). */
+int cfd_rk3_step(cfd_handle* h, double dt);
+
+/* nsteps RK3 steps, each with dt = cfl / sum_d max (c + |u_d|)/h_d computed on
+ * the device (BlockSolver::getTimeIntervalExplicit, blockSolver.cpp:318-348,
+ * without the outputDt clamp of stepsLoopCFL).  t_out (may be NULL) receives
+ * the elapsed simulated time; the call is asynchronous when t_out is NULL. */
+int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out);
+
+/* CFL time step of the current state (getTimeIntervalExplicit). */
+int cfd_cfl_dt(cfd_handle* h, double cfl, double* dt);
+
+/* Kahan-compensated domain totals of the conservative variables (sum over
+ * cells, no cell volume).  No counterpart in the reference (it has no
+ * conserved-total diagnostic); used to check conservation. */
+int cfd_conserved_totals(cfd_handle* h, double* out_nv);
+
+/* Waits for all work queued on the handle's stream. */
+int cfd_synchronize(cfd_handle* h);
+
+/* Measures the direction sweeps of one RK stage with HIP events on the
+ * handle's stream: runs stage 1 from the current state into a scratch buffer
+ * (the state is left unchanged) `reps` times; ms_out[d] receives the mean
+ * duration of the direction-d sweep launch.  Diagnostic; no reference
+ * counterpart (the reference times whole steps with std::chrono,
+ * zone_FD/src/src/blockSolver.cpp:194-200). */
+int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out);
+
+/* Device pointer of the current state (structure of arrays, nv planes of
+ * n[0]*n[1]*n[2] doubles) and of the rhs buffer. */
+int cfd_device_buffers(cfd_handle* h, double** state, double** rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CONGFD_H */

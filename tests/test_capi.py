@@ -1,0 +1,63 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol declared in
+include/congfd.h, and rejects bad configurations without touching a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from cfd_inwsl_amd import solver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "congfd.h")).read()
+    return sorted(set(re.findall(r"\b(cfd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_lists_match_binding():
+    assert _declared() == sorted(solver.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = solver.load_library()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_config_struct_matches_header_size():
+    # dim + n[3] | h[3] | 4 ints | bc[6] | bc_prim[6][5] | device, padded to 8
+    c = solver.cfd_config()
+    assert ctypes.sizeof(c) == 4 + 12 + 24 + 16 + 24 + 240 + 4 + 4
+
+
+def test_version_string():
+    lib = solver.load_library()
+    assert lib.cfd_version().decode().startswith("congfd-mi355x")
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("dim", 4, "dim"),
+    ("diff_method", 1, "MND6"),
+    ("flux_method", 5, "flux_method"),
+    ("inter_method", 17, "not implemented"),
+])
+def test_bad_config_is_rejected_before_any_gpu_call(field, value, msg):
+    lib = solver.load_library()
+    cfg = solver.cfd_config()
+    cfg.dim = 2
+    cfg.n[0], cfg.n[1], cfg.n[2] = 8, 8, 1
+    cfg.h[0] = cfg.h[1] = cfg.h[2] = 0.1
+    cfg.inter_method = 2
+    cfg.flux_method = 0
+    cfg.diff_method = 3
+    cfg.recon = 0
+    for i in range(6):
+        cfg.bc[i] = 7
+    setattr(cfg, field, value)
+    h = ctypes.c_void_p()
+    rc = lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h))
+    assert rc < 0
+    assert msg in lib.cfd_last_error().decode()
+    assert not h.value

@@ -1,0 +1,171 @@
+"""GPU parity against the CPU oracle, through the C ABI.
+
+Bar: FP64 results equal to the oracle.  The kernels keep the reference's
+operation order and are compiled without FMA contraction, so the check is
+bit-for-bit (np.array_equal); the stated tolerance of the hot path is
+||gpu - oracle||_inf / ||oracle||_inf <= 1e-12 per variable, asserted as well
+so that a failure reports its size.
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from cfd_inwsl_amd import BC, FluxMethod, InterMethod, Recon, Solver, cases
+from oracle import oracle as O
+from tests.helpers import oracle_grid, random_state, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _check(gpu, ref, what):
+    err = rel_err(gpu, ref)
+    _say(f"{what}: rel err {err:.3e} bitwise={np.array_equal(gpu, ref)}")
+    assert err <= TOL, what
+    assert np.array_equal(gpu, ref), what
+
+
+SIZES = {1: (61,), 2: (37, 29), 3: (13, 11, 9)}
+COMBOS = list(itertools.product(
+    [1, 2, 3],
+    [InterMethod.WCNS5, InterMethod.WCNSZ5, InterMethod.TCNS5, InterMethod.WCNS5CONGZ],
+    [FluxMethod.HLLC, FluxMethod.ROE],
+    [Recon.CHAR, Recon.COMPONENT],
+))
+
+
+@pytest.mark.parametrize("dim,inter,flux,recon", COMBOS)
+def test_rhs_parity(dim, inter, flux, recon):
+    n = SIZES[dim]
+    for bcname, bcv in (("periodic", BC.PERIODIC), ("outflow", BC.OUTFLOW)):
+        bc = [bcv] * 6
+        cons = random_state(dim, n, seed=dim * 100 + int(inter), jumps=True)
+        s = Solver(dim, n, 0.07, inter, flux, recon, bc)
+        s.set_state(cons)
+        g = s.rhs()
+        s.close()
+        ref, _ = O.rhs(oracle_grid(dim, n, 0.07, inter, flux, recon, bc), cons)
+        _check(g, ref, f"rhs dim={dim} {inter.name} {flux.name} {recon.name} {bcname}")
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_rhs_parity_mixed_boundaries(dim):
+    n = {1: (40,), 2: (33, 21), 3: (12, 10, 14)}[dim]
+    bc = [BC.SYMMETRYX, BC.DIRICHLET, BC.SYMMETRYY, BC.OUTFLOW, BC.SYMMETRYZ, BC.MIRROR_COPY]
+    if dim == 1:
+        bc = [BC.SYMMETRY1D, BC.DIRICHLET] + bc[2:]
+    bcp = [[0.0] * 5 for _ in range(6)]
+    bcp[1] = [0.8, 0.1, -0.2, 0.05, 0.9][: dim + 2] + [0.0] * (3 - dim)
+    bcp[1][dim + 1] = 0.9
+    cons = random_state(dim, n, seed=7 + dim)
+    for inter, flux in ((InterMethod.TCNS5, FluxMethod.HLLC), (InterMethod.WCNS5, FluxMethod.ROE)):
+        s = Solver(dim, n, 0.05, inter, flux, Recon.CHAR, bc, bcp)
+        s.set_state(cons)
+        g = s.rhs()
+        s.close()
+        ref, _ = O.rhs(oracle_grid(dim, n, 0.05, inter, flux, Recon.CHAR, bc, bcp), cons)
+        _check(g, ref, f"rhs mixed-bc dim={dim} {inter.name} {flux.name}")
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_rhs_parity_ct_variants(dim):
+    n = SIZES[dim]
+    cons = random_state(dim, n, seed=31)
+    for inter in (InterMethod.TCNS5CT4, InterMethod.TCNS5CT7):
+        s = Solver(dim, n, 0.07, inter, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 6)
+        s.set_state(cons)
+        g = s.rhs()
+        s.close()
+        ref, _ = O.rhs(oracle_grid(dim, n, 0.07, inter, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 6), cons)
+        _check(g, ref, f"rhs dim={dim} {inter.name}")
+
+
+@pytest.mark.parametrize("dim,inter,flux", [(1, InterMethod.WCNS5, FluxMethod.HLLC),
+                                            (2, InterMethod.WCNSZ5, FluxMethod.ROE),
+                                            (3, InterMethod.TCNS5, FluxMethod.HLLC)])
+def test_rk3_parity(dim, inter, flux):
+    n = {1: (80,), 2: (40, 36), 3: (16, 12, 10)}[dim]
+    bc = [BC.PERIODIC] * 6
+    cons = random_state(dim, n, seed=3, jumps=False)
+    s = Solver(dim, n, 0.05, inter, flux, Recon.CHAR, bc)
+    s.set_state(cons)
+    grid = oracle_grid(dim, n, 0.05, inter, flux, Recon.CHAR, bc)
+    dt = 0.2 * O.cfl_dt(grid, cons, 1.0)
+    for _ in range(3):
+        s.rk3_step(dt)
+    g = s.get_state()
+    ref, _ = O.rk3(grid, cons, dt=dt, nsteps=3)
+    _check(g, ref, f"rk3 dim={dim}")
+    # CFL-driven steps with the time step computed on the device
+    t = s.rk3_steps_cfl(0.5, 2)
+    ref2, tref = O.rk3(grid, ref, cfl=0.5, nsteps=2)
+    _check(s.get_state(), ref2, f"rk3 cfl dim={dim}")
+    assert t == tref
+    s.close()
+
+
+def test_cfl_dt_parity():
+    for dim in (1, 2, 3):
+        n = SIZES[dim]
+        cons = random_state(dim, n, seed=5)
+        s = Solver(dim, n, [0.07, 0.05, 0.03][:dim], InterMethod.WCNS5, FluxMethod.ROE, Recon.CHAR)
+        s.set_state(cons)
+        g = s.cfl_dt(0.5)
+        s.close()
+        ref = O.cfl_dt(O.make_grid(dim, n, [0.07, 0.05, 0.03][:dim], O.S_JS, 1, 1, [7] * 6), cons, 0.5)
+        assert g == ref, (dim, g, ref)
+
+
+def test_vortex_256_rhs_and_steps_parity():
+    cons, h = cases.vortex(256)
+    bc = [BC.PERIODIC] * 4
+    s = Solver(2, (256, 256), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, bc)
+    s.set_state(cons)
+    grid = oracle_grid(2, (256, 256), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, bc)
+    ref, _ = O.rhs(grid, cons)
+    _check(s.rhs(), ref, "vortex 256^2 rhs")
+    s.rk3_steps_cfl(0.5, 2)
+    ref2, _ = O.rk3(grid, cons, cfl=0.5, nsteps=2)
+    _check(s.get_state(), ref2, "vortex 256^2 two CFL steps")
+    s.close()
+
+
+def test_full_size_conservation_2048():
+    """Periodic vortex at 2048^2: totals of rho, rho u, rho v, rho E are
+    conserved to round-off over 5 CFL steps (size-independent invariant)."""
+    cons, h = cases.vortex(2048)
+    s = Solver(2, (2048, 2048), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, [BC.PERIODIC] * 4)
+    s.set_state(cons)
+    t0 = s.totals()
+    s.rk3_steps_cfl(0.5, 5)
+    t1 = s.totals()
+    s.close()
+    scale = np.abs(cons).sum(0)
+    _say("2048^2 totals drift", (t1 - t0) / scale)
+    assert np.all(np.abs(t1 - t0) / scale < 1e-13)
+
+
+def test_full_size_3d_freestream_and_conservation():
+    n = 256
+    cons, h = cases.periodic3d(n)
+    s = Solver(3, (n, n, n), h, InterMethod.TCNS5, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 6)
+    s.set_state(cons)
+    t0 = s.totals()
+    s.rk3_steps_cfl(0.5, 2)
+    t1 = s.totals()
+    scale = np.abs(cons).sum(0)
+    _say("256^3 totals drift", (t1 - t0) / scale)
+    assert np.all(np.abs(t1 - t0) / scale < 1e-13)
+    # uniform state: rhs identically zero
+    u = np.tile(cons[:1], (n ** 3, 1))
+    s.set_state(u)
+    r = s.rhs()
+    s.close()
+    assert np.all(r == 0.0)
