@@ -43,6 +43,7 @@ struct cfd_handle {
   double* scal;          // [0] dt, [1..3] max wave speed per direction (bits), [4] t
   double* partial;       // block partial sums for totals
   int npartial;
+  int keep_rhs;
   hipStream_t stream;
 };
 
@@ -88,6 +89,10 @@ __global__ void k_soa_to_aos(const double* __restrict__ in, double* __restrict__
     for (int v = 0; v < NV; v++) out[i * NV + v] = in[v * N + i];
 }
 
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ in, double2* __restrict__ out, long n2) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) out[i] = in[i];
+}
+
 __global__ void k_zero_lambda(double* scal) {
   unsigned long long* b = reinterpret_cast<unsigned long long*>(scal);
   b[1] = 0ull;
@@ -128,6 +133,24 @@ __global__ __launch_bounds__(256) void k_wave_speed(const double* __restrict__ U
     unsigned long long* b = reinterpret_cast<unsigned long long*>(scal);
     for (int d = 0; d < D; d++) atomicMax(&b[1 + d], (unsigned long long)__double_as_longlong(red[d][0]));
   }
+}
+
+// max |R[var]| into scal[5] (bits of a non-negative double)
+__global__ __launch_bounds__(256) void k_abs_max(const double* __restrict__ x, long N, double* scal) {
+  double m = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    double a = fabs(x[i]);
+    m = m < a ? a : m;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s && red[threadIdx.x] < red[threadIdx.x + s]) red[threadIdx.x] = red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicMax(reinterpret_cast<unsigned long long*>(scal) + 5, (unsigned long long)__double_as_longlong(red[0]));
 }
 
 // dt = cfl / ((l0 + l1) + l2); t += dt
@@ -224,8 +247,8 @@ static int rk_stage(cfd_handle* h, int stage, const double* Ucur, double* Uout) 
   for (int ax = 0; ax < h->D; ax++) {
     int flags = 0;
     if (ax > 0) flags |= SW_READ_R;
-    if (ax < h->D - 1) flags |= SW_WRITE_R;
-    else flags |= SW_RK;
+    if (ax < h->D - 1 || h->keep_rhs) flags |= SW_WRITE_R;
+    if (ax == h->D - 1) flags |= SW_RK;
     HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, h->U0, Uout));
   }
   return 0;
@@ -401,6 +424,23 @@ int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out) {
   return CFD_OK;
 }
 
+int cfd_set_keep_rhs(cfd_handle* h, int keep) {
+  if (!h) return fail(CFD_E_ARG, "null handle");
+  h->keep_rhs = keep != 0;
+  return CFD_OK;
+}
+
+int cfd_rhs_linf(cfd_handle* h, int var, double* out) {
+  if (!h || !out || var < 0 || var >= h->NV) return fail(CFD_E_ARG, "bad argument");
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 5, 0.0);
+  int blocks = (int)std::min<long>((h->N + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_abs_max, dim3(blocks), dim3(256), 0, h->stream, h->R + (long)var * h->N, h->N, h->scal);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, h->scal + 5, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
 int cfd_cfl_dt(cfd_handle* h, double cfl, double* dt) {
   if (!h || !dt) return fail(CFD_E_ARG, "null argument");
   int rc = wave_speed(h, cfl);
@@ -452,6 +492,17 @@ int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out) {
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   for (int ax = 0; ax < h->D; ax++) ms_out[ax] = acc[ax] / reps;
+  return CFD_OK;
+}
+
+int cfd_copy_probe(cfd_handle* h, int reps) {
+  if (!h || reps < 1) return fail(CFD_E_ARG, "bad argument");
+  long n2 = h->N * h->NV / 2;
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, h->stream, reinterpret_cast<const double2*>(h->U0),
+                       reinterpret_cast<double2*>(h->Ua), n2);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
   return CFD_OK;
 }
 

@@ -69,12 +69,13 @@ class cfd_config(ctypes.Structure):
 EXPORTS = [
     "cfd_last_error", "cfd_version", "cfd_create", "cfd_destroy", "cfd_set_state", "cfd_get_state",
     "cfd_eval_rhs", "cfd_rk3_step", "cfd_rk3_steps_cfl", "cfd_cfl_dt", "cfd_conserved_totals",
-    "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps",
+    "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf",
 ]
 
 
 def library_path():
-    return os.path.join(_HERE, "libcongfd.so")
+    """In-tree library; CONGFD_LIB may name another build of the same sources."""
+    return os.environ.get("CONGFD_LIB") or os.path.join(_HERE, "libcongfd.so")
 
 
 _lib = None
@@ -106,6 +107,9 @@ def load_library():
     lib.cfd_synchronize.argtypes = [P]
     lib.cfd_device_buffers.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
     lib.cfd_time_sweeps.argtypes = [P, ctypes.c_int, pd]
+    lib.cfd_copy_probe.argtypes = [P, ctypes.c_int]
+    lib.cfd_set_keep_rhs.argtypes = [P, ctypes.c_int]
+    lib.cfd_rhs_linf.argtypes = [P, ctypes.c_int, pd]
     _lib = lib
     return lib
 
@@ -202,6 +206,14 @@ class Solver:
         self._check(self.lib.cfd_rk3_steps_cfl(self.h, float(cfl), int(nsteps), None))
         return None
 
+    def set_keep_rhs(self, keep=True):
+        self._check(self.lib.cfd_set_keep_rhs(self.h, int(bool(keep))))
+
+    def rhs_linf(self, var=0):
+        d = ctypes.c_double(0)
+        self._check(self.lib.cfd_rhs_linf(self.h, int(var), ctypes.byref(d)))
+        return d.value
+
     def cfl_dt(self, cfl):
         d = ctypes.c_double(0)
         self._check(self.lib.cfd_cfl_dt(self.h, float(cfl), ctypes.byref(d)))
@@ -220,6 +232,10 @@ class Solver:
         out = np.empty(self.dim)
         self._check(self.lib.cfd_time_sweeps(self.h, int(reps), _pd(out)))
         return out
+
+    def copy_probe(self, reps=1):
+        """Plain device copy of the state planes (calibration of byte counters)."""
+        self._check(self.lib.cfd_copy_probe(self.h, int(reps)))
 
     def device_buffers(self):
         s, r = ctypes.c_void_p(), ctypes.c_void_p()

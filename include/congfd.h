@@ -102,6 +102,16 @@ int cfd_rk3_step(cfd_handle* h, double dt);
  * the elapsed simulated time; the call is asynchronous when t_out is NULL. */
 int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out);
 
+/* When keep != 0 the RK3 calls also store the rhs of each stage (the last
+ * stage's rhs stays readable, as the reference's Equation::rhs does after
+ * RK3_SSP); off by default, which saves one rhs write per step. */
+int cfd_set_keep_rhs(cfd_handle* h, int keep);
+
+/* max_i |rhs[i][var]| of the stored rhs: Data::getLinf
+ * (zone_FD/src/src/data.cpp:230), the per-step "rhoLinf" log value of
+ * BlockSolver::stepsLoopCFL (blockSolver.cpp:204-206) for var = 0. */
+int cfd_rhs_linf(cfd_handle* h, int var, double* out);
+
 /* CFL time step of the current state (getTimeIntervalExplicit). */
 int cfd_cfl_dt(cfd_handle* h, double cfl, double* dt);
 
@@ -120,6 +130,11 @@ int cfd_synchronize(cfd_handle* h);
  * counterpart (the reference times whole steps with std::chrono,
  * zone_FD/src/src/blockSolver.cpp:194-200). */
 int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out);
+
+/* Plain device copy of the state planes (nv * cells doubles) into a scratch
+ * buffer, `reps` times: a reference kernel of known traffic for calibrating
+ * hardware byte counters.  Diagnostic; no reference counterpart. */
+int cfd_copy_probe(cfd_handle* h, int reps);
 
 /* Device pointer of the current state (structure of arrays, nv planes of
  * n[0]*n[1]*n[2] doubles) and of the rhs buffer. */

@@ -320,3 +320,24 @@ def test_mnd6_exact_for_quartic_flux():
     gr = np.polyval(co, 0.5 + h / 2 + h * np.arange(10))
     r = O.accuracy_rhs(O.S_LINEAR, np.polyval(co, xc), gl, gr, h)
     np.testing.assert_allclose(r, np.polyval(np.polyder(co), xc), rtol=0, atol=1e-11)
+
+
+def test_sod_against_exact_riemann_solution():
+    """Config S through the oracle: 400 cells, WCNS5-JS component-wise + HLLC,
+    CFL 0.5, t = 0.2; density L1 error against Toro's exact solution."""
+    from cfd_inwsl_amd import cases
+    from tests.test_driver import _oracle_steps_cfl
+    n, h = 400, 1.0 / 400
+    x = -0.5 + (np.arange(n) + 0.5) * h
+    cons = np.stack([np.where(x < 0, 1.0, 0.125), np.zeros(n),
+                     np.where(x < 0, 1.0 / (1.4 - 1) * 1, 1.0 / (1.4 - 1) * 0.1)], -1)
+    g = O.make_grid(1, (n,), h, O.S_JS, O.F_HLLC, 0, [7, 7])
+    ref, t = _oracle_steps_cfl(g, cons, 0.5, 0.2, 1)
+    exact = cases.sod_exact(x + 0.5, 0.2)
+    l1 = np.abs(ref[:, 0] - exact[:, 0]).mean()
+    assert abs(t - 0.2) < 1e-14
+    assert 5e-4 < l1 < 3e-3
+    # characteristic-wise (the reference's own 1D path, with its positivity limiter)
+    g2 = O.make_grid(1, (n,), h, O.S_JS, O.F_HLLC, 1, [7, 7])
+    ref2, _ = _oracle_steps_cfl(g2, cons, 0.5, 0.2, 1)
+    assert np.abs(ref2[:, 0] - exact[:, 0]).mean() < 3e-3
