@@ -22,7 +22,8 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-from cfd_inwsl_amd import BC, FluxMethod, InterMethod, Recon, Solver, cases  # noqa: E402
+from cfd_inwsl_amd import (BATCH_GRAPH, BATCH_PER_ZONE, BATCH_PERSISTENT, BC, Batch, FluxMethod,  # noqa: E402
+                           InterMethod, Recon, Solver, cases)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 
@@ -32,6 +33,8 @@ def log(*a):
 
 
 def measure(sol, nv, dim, ncell, steps, warmup, cfl):
+    tiles, segs = sol.autotune(reps=3)
+    log(f"[bench] launch plan: tiles {tiles} segments {segs}")
     sol.rk3_steps_cfl(cfl, warmup, want_time=False)
     sol.synchronize()
     t0 = time.perf_counter()
@@ -50,8 +53,44 @@ def measure(sol, nv, dim, ncell, steps, warmup, cfl):
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "sweep_kernel", "sweep_ms": [round(float(x), 4) for x in ms],
-            "bytes_per_launch": bytes_per_launch}
+            "bytes_per_launch": bytes_per_launch, "launch_plan": {"tiles": tiles, "segments": segs}}
     return value, sec * 1e3 / steps, roof
+
+
+def bench_batch(nzones=96, seed=20261203, steps=3):
+    """Config B: independent 2D four-quadrant Riemann zones, sides uniform in
+    [128, 1024], WCNS5 char-wise + HLLC, outflow; one persistent launch per
+    sweep with the RK3 step replayed from a hipGraph vs per-zone launches."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(128, 1025, size=(nzones, 2))
+    zones, states = [], []
+    for i, (nx, ny) in enumerate(sizes):
+        c, h = cases.riemann2d_quadrants(int(nx), int(ny), seed=seed + i)
+        zones.append({"n": (int(nx), int(ny)), "h": h, "bc": [BC.OUTFLOW] * 6})
+        states.append(c)
+    cells = int(sum(int(a) * int(b) for a, b in sizes))
+    b = Batch(2, zones, InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR)
+    for i, c in enumerate(states):
+        b.set_state(i, c)
+    b.autotune(reps=1)
+    out = {}
+    for name, mode in (("graph", BATCH_GRAPH), ("persistent", BATCH_PERSISTENT), ("per_zone", BATCH_PER_ZONE)):
+        for i, c in enumerate(states):
+            b.set_state(i, c)
+        b.steps_cfl(0.5, 1, mode)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.steps_cfl(0.5, steps, mode)
+        b.synchronize()
+        sec = time.perf_counter() - t0
+        out[name] = {"ms_per_step": round(sec * 1e3 / steps, 3),
+                     "mcell_stage_s": round(cells * 3 * steps / sec / 1e6, 1)}
+        log(f"[bench] batch {name}: {out[name]}")
+    b.close()
+    return {"workload": f"{nzones} independent 2D four-quadrant Riemann zones, sides U[128,1024], "
+                        "WCNS5 char-wise + HLLC, FP64, per-zone CFL 0.5",
+            "zones": nzones, "cells": cells, "steps": steps, "modes": out,
+            "speedup_graph_vs_per_zone": round(out["per_zone"]["ms_per_step"] / out["graph"]["ms_per_step"], 2)}
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -82,6 +121,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D 256^3 secondary figure")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-zones figure")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -114,6 +154,11 @@ def main():
                      "value": round(v3, 2), "unit": "Mcell-stage/s", "ms_per_step": round(ms3, 4),
                      "steps": st3, "roofline": roof3}
 
+    batch = None
+    if not args.no_batch:
+        log("[bench] batched zones")
+        batch = bench_batch()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -139,6 +184,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "secondary": secondary,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

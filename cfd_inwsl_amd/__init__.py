@@ -6,6 +6,7 @@ host-side pieces a driver needs (``Info``-style configuration, scheme
 selection, test cases).
 """
 from .solver import (  # noqa: F401
-    BC, FluxMethod, InterMethod, Recon, Solver, library_path, load_library, cfd_config,
+    BC, Batch, FluxMethod, InterMethod, Recon, Solver, library_path, load_library, cfd_config,
+    BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_ZONE,
 )
 from . import cases  # noqa: F401

@@ -8,13 +8,8 @@
 #include <vector>
 
 #include "../../include/congfd.h"
-#include "sweep.hpp"
-
-namespace cfd {
-hipError_t launch_sweep_d1(int ax, int sch, int flx, bool cw, const SweepArgs& a, dim3 grid, hipStream_t s);
-hipError_t launch_sweep_d2(int ax, int sch, int flx, bool cw, const SweepArgs& a, dim3 grid, hipStream_t s);
-hipError_t launch_sweep_d3(int ax, int sch, int flx, bool cw, const SweepArgs& a, dim3 grid, hipStream_t s);
-}  // namespace cfd
+#include "launch_plan.hpp"
+#include "sweep_inst.hpp"
 
 using namespace cfd;
 
@@ -24,6 +19,9 @@ static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+// error slot shared with the batch entry points (batch.hip)
+int cfd_set_error_from_batch(int code, const char* msg) { return fail(code, msg); }
 
 #define HIPCHK(expr)                                                                   \
   do {                                                                                 \
@@ -44,6 +42,8 @@ struct cfd_handle {
   double* partial;       // block partial sums for totals
   int npartial;
   int keep_rhs;
+  int tile[3];     // tile shape per direction
+  int seg[3];      // segment length per direction
   hipStream_t stream;
 };
 
@@ -51,7 +51,7 @@ struct cfd_handle {
 // SolverType::initReconer for EULER (zone_FD/src/src/solverType.cpp:217-594).
 // Values the reference routes to its default branch map to weno5_JSchen as
 // there; research variants it lists explicitly are not implemented here.
-static int select_scheme(int inter, int* sch, double* ct) {
+int cfd_select_scheme(int inter, int* sch, double* ct) {
   *ct = 1e-5;
   switch (inter) {
   case 2: *sch = SCH_JS; return 0;             // WCNS5
@@ -192,26 +192,8 @@ __global__ __launch_bounds__(256) void k_totals(const double* __restrict__ U, lo
 }
 
 // ---- helpers -----------------------------------------------------------------------
-static int seg_len_for(const cfd_handle* h, int ax, long* nblk_out) {
-  const int n = h->cfg.n[ax];
-  const int* nn = h->cfg.n;
-  long npencil = ax == 0 ? (long)nn[1] * nn[2] : (ax == 1 ? (long)nn[0] * nn[2] : (long)nn[0] * nn[1]);
-  int CP = ax == 0 ? Tile<0>::CP : Tile<1>::CP;
-  int CA = ax == 0 ? Tile<0>::CA : Tile<1>::CA;
-  long nb = (npencil + CP - 1) / CP;
-  long target = 4096;  // enough workgroups to fill 256 CUs several times
-  long nseg = (target + nb - 1) / nb;
-  long maxseg = n / (4 * CA);
-  if (maxseg < 1) maxseg = 1;
-  if (nseg > maxseg) nseg = maxseg;
-  if (nseg < 1) nseg = 1;
-  int seg = (int)((n + nseg - 1) / nseg);
-  *nblk_out = nb;
-  return seg;
-}
-
 static hipError_t launch_sweep(cfd_handle* h, int ax, const double* U, int flags, int stage, const double* U0,
-                               double* Uout) {
+                               double* Uout, int tile = -1, int seg = -1) {
   SweepArgs a;
   memset(&a, 0, sizeof(a));
   a.U = U;
@@ -223,8 +205,9 @@ static hipError_t launch_sweep(cfd_handle* h, int ax, const double* U, int flags
   for (int i = 0; i < 3; i++) a.n[i] = h->cfg.n[i];
   a.flags = flags;
   a.stage = stage;
-  long nb;
-  a.seg_len = seg_len_for(h, ax, &nb);
+  if (tile < 0) tile = h->tile[ax];
+  if (seg < 0) seg = h->seg[ax];
+  a.seg_len = seg;
   a.h = h->cfg.h[ax];
   a.ct = h->ct;
   a.bc_lo = h->cfg.bc[2 * ax];
@@ -233,13 +216,10 @@ static hipError_t launch_sweep(cfd_handle* h, int ax, const double* U, int flags
     a.prim_lo[v] = h->cfg.bc_prim[2 * ax][v];
     a.prim_hi[v] = h->cfg.bc_prim[2 * ax + 1][v];
   }
-  int nseg = (h->cfg.n[ax] + a.seg_len - 1) / a.seg_len;
-  dim3 grid((unsigned)nb, (unsigned)nseg);
-  switch (h->D) {
-  case 1: return launch_sweep_d1(ax, h->sch, h->flx, h->cw, a, grid, h->stream);
-  case 2: return launch_sweep_d2(ax, h->sch, h->flx, h->cw, a, grid, h->stream);
-  default: return launch_sweep_d3(ax, h->sch, h->flx, h->cw, a, grid, h->stream);
-  }
+  const long nb = plan_pencil_blocks(h->cfg.n, ax, tile);
+  const int nseg = (h->cfg.n[ax] + seg - 1) / seg;
+  return launch_sweep_dispatch(h->D, ax, h->sch, h->flx, h->cw, tile, &a, nullptr, dim3((unsigned)nb, (unsigned)nseg),
+                               h->stream);
 }
 
 // One RK stage: sweeps over every direction; the last one applies the stage update.
@@ -301,7 +281,7 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
     return fail(CFD_E_ARG, "recon must be CFD_RECON_CHAR or CFD_RECON_COMPONENT");
   int sch;
   double ct;
-  if (select_scheme(cfg->inter_method, &sch, &ct))
+  if (cfd_select_scheme(cfg->inter_method, &sch, &ct))
     return fail(CFD_E_UNSUPP, "inter_method " + std::to_string(cfg->inter_method) + " is not implemented");
   for (int d = 0; d < 3; d++) {
     int want_active = d < cfg->dim;
@@ -326,6 +306,11 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   h->ct = ct;
   h->flx = cfg->flux_method;
   h->cw = cfg->recon == CFD_RECON_CHAR;
+  for (int ax = 0; ax < 3; ax++) {
+    h->tile[ax] = ax == 0 ? 0 : 1;
+    h->seg[ax] = cfg->n[ax];
+    if (ax < cfg->dim) plan_default(cfg->n, ax, &h->tile[ax], &h->seg[ax]);
+  }
   hipError_t e = hipSetDevice(cfg->device);
   if (e != hipSuccess) { delete h; return fail(CFD_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
   size_t bytes = sizeof(double) * h->N * h->NV;
@@ -503,6 +488,66 @@ int cfd_copy_probe(cfd_handle* h, int reps) {
                        reinterpret_cast<double2*>(h->Ua), n2);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
+int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out) {
+  if (!h || reps < 1) return fail(CFD_E_ARG, "bad argument");
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal, 0.0);
+  HIPCHK(hipGetLastError());
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int ax = 0; ax < h->D; ax++) {
+    PlanCandidate cand[16];
+    int nc = plan_candidates(h->cfg.n, ax, cand, 16);
+    double best = 1e30;
+    int bt = h->tile[ax], bs = h->seg[ax];
+    const int flags = SW_WRITE_R | (ax > 0 ? SW_READ_R : 0);
+    for (int c = 0; c < nc; c++) {
+      // warm-up launch, then timed launches
+      HIPCHK(launch_sweep(h, ax, h->U0, flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
+      HIPCHK(hipEventRecord(e0, h->stream));
+      for (int r = 0; r < reps; r++)
+        HIPCHK(launch_sweep(h, ax, h->U0, flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
+      HIPCHK(hipEventRecord(e1, h->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; bt = cand[c].tile; bs = cand[c].seg; }
+    }
+    h->tile[ax] = bt;
+    h->seg[ax] = bs;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int ax = 0; ax < 3; ax++) {
+    if (tiles_out) tiles_out[ax] = h->tile[ax];
+    if (segs_out) segs_out[ax] = h->seg[ax];
+  }
+  return CFD_OK;
+}
+
+int cfd_launch_plan(cfd_handle* h, int* tiles_out, int* segs_out) {
+  if (!h) return fail(CFD_E_ARG, "null handle");
+  for (int ax = 0; ax < 3; ax++) {
+    if (tiles_out) tiles_out[ax] = h->tile[ax];
+    if (segs_out) segs_out[ax] = h->seg[ax];
+  }
+  return CFD_OK;
+}
+
+int cfd_set_launch_plan(cfd_handle* h, const int* tiles, const int* segs) {
+  if (!h || !tiles || !segs) return fail(CFD_E_ARG, "null argument");
+  for (int ax = 0; ax < h->D; ax++) {
+    bool ok = ax == 0 ? (tiles[ax] == 0 || tiles[ax] == 1) : (tiles[ax] == 1 || tiles[ax] == 2);
+    if (!ok) return fail(CFD_E_ARG, "tile code not compiled for this direction");
+    if (segs[ax] < 1 || segs[ax] > h->cfg.n[ax]) return fail(CFD_E_ARG, "segment length out of range");
+  }
+  for (int ax = 0; ax < h->D; ax++) {
+    h->tile[ax] = tiles[ax];
+    h->seg[ax] = segs[ax];
+  }
   return CFD_OK;
 }
 

@@ -55,15 +55,25 @@ struct SweepArgs {
   double prim_lo[5], prim_hi[5];
 };
 
-template <int AX> struct Tile;
+// Tile shapes: CA interfaces per chunk along the sweep axis x CP pencils.
+// For AX = 0 lanes run along the pencil (CA >= 16 keeps loads coalesced);
+// for AX > 0 lanes run across x-adjacent pencils (CP >= 16).
+template <int T> struct Tile;
 template <> struct Tile<0> { static constexpr int CA = 32, CP = 8; };
 template <> struct Tile<1> { static constexpr int CA = 16, CP = 16; };
-template <> struct Tile<2> { static constexpr int CA = 16, CP = 16; };
+template <> struct Tile<2> { static constexpr int CA = 8, CP = 32; };
+__host__ __device__ constexpr int tile_ca(int t) { return t == 0 ? 32 : (t == 1 ? 16 : 8); }
+__host__ __device__ constexpr int tile_cp(int t) { return t == 0 ? 8 : (t == 1 ? 16 : 32); }
 
-template <int D, int AX, int SCH, int FLX, bool CHARW>
-__global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
+#ifndef CFD_SWEEP_WAVES_PER_EU
+#define CFD_SWEEP_WAVES_PER_EU 1
+#endif
+
+// One workgroup's share of a sweep: pencil group bx, segment by.
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE>
+__device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, const int by) {
   constexpr int NV = D + 2;
-  constexpr int CA = Tile<AX>::CA, CP = Tile<AX>::CP;
+  constexpr int CA = Tile<TILE>::CA, CP = Tile<TILE>::CP;
   constexpr int RC = CA + 5;  // cell ring slots
   constexpr int FR = CA + 3;  // flux ring slots
   constexpr int NCV = CHARW ? 2 * NV : NV;  // per cell: prim [+ rebuilt cons]
@@ -86,7 +96,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
   const int n = a.n[AX];
   const long sa = AX == 0 ? 1 : (AX == 1 ? (long)nx : (long)nx * ny);
   const long npencil = AX == 0 ? (long)ny * nz : (AX == 1 ? (long)nx * nz : (long)nx * ny);
-  const long gp = (long)blockIdx.x * CP + lp;
+  const long gp = bx * CP + lp;
   const bool pvalid = gp < npencil;
   long base = 0;
   if (pvalid) {
@@ -94,7 +104,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
     else if (AX == 1) base = (gp % nx) + (gp / nx) * (long)nx * ny;
     else base = gp;
   }
-  const int c0 = blockIdx.y * a.seg_len;
+  const int c0 = by * a.seg_len;
   const int c1 = min(c0 + a.seg_len, n);
 
   // ---- load one cell (position g along AX, may be a ghost) into the ring ----
@@ -124,7 +134,8 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
       const long idx = base + (long)src * sa;
       for (int v = 0; v < NV; v++) cs[v] = a.U[v * a.N + idx];
       cons_to_prim<D>(cs, pr);
-      if (flip > 0 && flip < NV - 1) pr[flip] = -pr[flip];
+      for (int v = 1; v < NV - 1; v++)
+        if (v == flip) pr[v] = -pr[v];
     }
     const int slot = (g + 8 * RC) % RC;
     for (int v = 0; v < NV; v++) cell[cidx(v, slot, lp)] = pr[v];
@@ -292,6 +303,86 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
       }
     }
     __syncthreads();
+  }
+}
+
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE>
+__global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_kernel(SweepArgs a) {
+  sweep_block<D, AX, SCH, FLX, CHARW, TILE>(a, blockIdx.x, blockIdx.y);
+}
+
+// ---- batched zones ------------------------------------------------------------
+// Independent structured zones of uneven sizes share pooled buffers; every
+// zone keeps the single-block layout (nv planes of its own cell count) at
+// element offset `off` of each pool.
+struct ZoneDesc {
+  long off;          // element offset of the zone in every pool
+  long N;            // cells
+  int n[3];
+  int seg_len[3];    // segment length per direction
+  double h[3];
+  int bc[6];
+  double prim[6][5];
+};
+
+struct WorkItem {
+  int zone;
+  int bx;  // pencil group
+  int by;  // segment
+};
+
+struct BatchArgs {
+  const double* U;
+  double* R;
+  const double* U0;
+  double* Uout;
+  const double* dt;        // per-zone time step
+  const ZoneDesc* zones;
+  const WorkItem* items;   // work list of this direction, largest zones first
+  int nitems;
+  unsigned* counter;       // work-queue head of this launch (zeroed before the launch)
+  int flags;
+  int stage;
+  double ct;
+};
+
+// Persistent kernel: each workgroup pulls work items from the queue until it
+// is empty, so one launch covers every zone of the batch.
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE>
+__global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kernel(BatchArgs b) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(b.counter, 1u);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= b.nitems) break;
+    const WorkItem w = b.items[item];
+    const int z = __builtin_amdgcn_readfirstlane(w.zone);
+    const ZoneDesc& zd = b.zones[z];
+    SweepArgs a;
+    a.U = b.U + zd.off;
+    a.R = b.R + zd.off;
+    a.U0 = b.U0 + zd.off;
+    a.Uout = b.Uout + zd.off;
+    a.dtp = b.dt + z;
+    a.N = zd.N;
+    a.n[0] = zd.n[0];
+    a.n[1] = zd.n[1];
+    a.n[2] = zd.n[2];
+    a.flags = b.flags;
+    a.stage = b.stage;
+    a.seg_len = zd.seg_len[AX];
+    a.h = zd.h[AX];
+    a.ct = b.ct;
+    a.bc_lo = zd.bc[2 * AX];
+    a.bc_hi = zd.bc[2 * AX + 1];
+    for (int v = 0; v < 5; v++) {
+      a.prim_lo[v] = zd.prim[2 * AX][v];
+      a.prim_hi[v] = zd.prim[2 * AX + 1][v];
+    }
+    sweep_block<D, AX, SCH, FLX, CHARW, TILE>(a, __builtin_amdgcn_readfirstlane(w.bx),
+                                              __builtin_amdgcn_readfirstlane(w.by));
   }
 }
 

@@ -1,35 +1,58 @@
-// Launch table of the sweep kernel for one dimension D: one instantiation per
-// (axis, interpolation scheme, Riemann flux, projection).
+// Launch tables of the sweep kernels for one (D, AX): one instantiation per
+// (interpolation scheme, Riemann flux, projection, tile shape), for the
+// single-zone kernel and the persistent batched kernel.
 #pragma once
 #include "sweep.hpp"
 
 namespace cfd {
 
-template <int D, int AX, int SCH, int FLX, bool CW>
-static hipError_t launch_one(const SweepArgs& a, dim3 grid, hipStream_t s) {
-  hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW>), grid, dim3(256), 0, s, a);
+// Tile shapes compiled per axis: x sweeps {0: 32x8, 1: 16x16},
+// y/z sweeps {1: 16x16, 2: 8x32} (see Tile<>).
+template <int AX> struct TilePair;
+template <> struct TilePair<0> { static constexpr int A = 0, B = 1; };
+template <> struct TilePair<1> { static constexpr int A = 1, B = 2; };
+template <> struct TilePair<2> { static constexpr int A = 1, B = 2; };
+
+template <int D, int AX, int SCH, int FLX, bool CW, int T>
+static hipError_t launch_one(const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
+  if (a) hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW, T>), grid, dim3(256), 0, s, *a);
+  else hipLaunchKernelGGL((sweep_batch_kernel<D, AX, SCH, FLX, CW, T>), grid, dim3(256), 0, s, *b);
   return hipGetLastError();
 }
 
+template <int D, int AX, int SCH, int FLX, bool CW>
+static hipError_t launch_tile(int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
+  if (tile == TilePair<AX>::A) return launch_one<D, AX, SCH, FLX, CW, TilePair<AX>::A>(a, b, grid, s);
+  if (tile == TilePair<AX>::B) return launch_one<D, AX, SCH, FLX, CW, TilePair<AX>::B>(a, b, grid, s);
+  return hipErrorInvalidValue;
+}
+
 template <int D, int AX, int SCH, int FLX>
-static hipError_t launch_cw(bool cw, const SweepArgs& a, dim3 grid, hipStream_t s) {
-  return cw ? launch_one<D, AX, SCH, FLX, true>(a, grid, s) : launch_one<D, AX, SCH, FLX, false>(a, grid, s);
+static hipError_t launch_cw(bool cw, int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
+  return cw ? launch_tile<D, AX, SCH, FLX, true>(tile, a, b, grid, s)
+            : launch_tile<D, AX, SCH, FLX, false>(tile, a, b, grid, s);
 }
 
 template <int D, int AX, int SCH>
-static hipError_t launch_flx(int flx, bool cw, const SweepArgs& a, dim3 grid, hipStream_t s) {
-  return flx == FLX_HLLC ? launch_cw<D, AX, SCH, FLX_HLLC>(cw, a, grid, s)
-                         : launch_cw<D, AX, SCH, FLX_ROE>(cw, a, grid, s);
+static hipError_t launch_flx(int flx, bool cw, int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid,
+                             hipStream_t s) {
+  return flx == FLX_HLLC ? launch_cw<D, AX, SCH, FLX_HLLC>(cw, tile, a, b, grid, s)
+                         : launch_cw<D, AX, SCH, FLX_ROE>(cw, tile, a, b, grid, s);
 }
 
 template <int D, int AX>
-static hipError_t launch_sch(int sch, int flx, bool cw, const SweepArgs& a, dim3 grid, hipStream_t s) {
+static hipError_t launch_sch(int sch, int flx, bool cw, int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid,
+                             hipStream_t s) {
   switch (sch) {
-  case SCH_JS: return launch_flx<D, AX, SCH_JS>(flx, cw, a, grid, s);
-  case SCH_Z: return launch_flx<D, AX, SCH_Z>(flx, cw, a, grid, s);
-  case SCH_TENO: return launch_flx<D, AX, SCH_TENO>(flx, cw, a, grid, s);
-  default: return launch_flx<D, AX, SCH_TENOS>(flx, cw, a, grid, s);
+  case SCH_JS: return launch_flx<D, AX, SCH_JS>(flx, cw, tile, a, b, grid, s);
+  case SCH_Z: return launch_flx<D, AX, SCH_Z>(flx, cw, tile, a, b, grid, s);
+  case SCH_TENO: return launch_flx<D, AX, SCH_TENO>(flx, cw, tile, a, b, grid, s);
+  default: return launch_flx<D, AX, SCH_TENOS>(flx, cw, tile, a, b, grid, s);
   }
 }
+
+// Entry points defined in sweep_d<D>_<axis>.hip
+hipError_t launch_sweep_dispatch(int D, int ax, int sch, int flx, bool cw, int tile, const SweepArgs* a,
+                                 const BatchArgs* b, dim3 grid, hipStream_t s);
 
 }  // namespace cfd

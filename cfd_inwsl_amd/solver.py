@@ -65,11 +65,25 @@ class cfd_config(ctypes.Structure):
     ]
 
 
+class cfd_zone(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("h", ctypes.c_double * 3),
+        ("bc", ctypes.c_int * 6),
+        ("bc_prim", (ctypes.c_double * 5) * 6),
+    ]
+
+
+BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_ZONE = 0, 1, 2
+
+
 # Every entry point declared in include/congfd.h.
 EXPORTS = [
     "cfd_last_error", "cfd_version", "cfd_create", "cfd_destroy", "cfd_set_state", "cfd_get_state",
     "cfd_eval_rhs", "cfd_rk3_step", "cfd_rk3_steps_cfl", "cfd_cfl_dt", "cfd_conserved_totals",
-    "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf",
+    "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf", "cfd_autotune", "cfd_launch_plan", "cfd_set_launch_plan",
+    "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_state", "cfd_batch_get_state", "cfd_batch_steps_cfl",
+    "cfd_batch_times", "cfd_batch_autotune", "cfd_batch_synchronize",
 ]
 
 
@@ -110,6 +124,18 @@ def load_library():
     lib.cfd_copy_probe.argtypes = [P, ctypes.c_int]
     lib.cfd_set_keep_rhs.argtypes = [P, ctypes.c_int]
     lib.cfd_rhs_linf.argtypes = [P, ctypes.c_int, pd]
+    pi = ctypes.POINTER(ctypes.c_int)
+    lib.cfd_autotune.argtypes = [P, ctypes.c_int, pi, pi]
+    lib.cfd_launch_plan.argtypes = [P, pi, pi]
+    lib.cfd_set_launch_plan.argtypes = [P, pi, pi]
+    lib.cfd_batch_create.argtypes = [ctypes.POINTER(cfd_config), ctypes.c_int, ctypes.POINTER(cfd_zone), ctypes.POINTER(P)]
+    lib.cfd_batch_destroy.argtypes = [P]
+    lib.cfd_batch_set_state.argtypes = [P, ctypes.c_int, pd]
+    lib.cfd_batch_get_state.argtypes = [P, ctypes.c_int, pd]
+    lib.cfd_batch_steps_cfl.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    lib.cfd_batch_times.argtypes = [P, pd]
+    lib.cfd_batch_autotune.argtypes = [P, ctypes.c_int]
+    lib.cfd_batch_synchronize.argtypes = [P]
     _lib = lib
     return lib
 
@@ -237,7 +263,101 @@ class Solver:
         """Plain device copy of the state planes (calibration of byte counters)."""
         self._check(self.lib.cfd_copy_probe(self.h, int(reps)))
 
+    def autotune(self, reps=3):
+        """Times the launch candidates of every direction; returns (tiles, segments)."""
+        t = (ctypes.c_int * 3)()
+        g = (ctypes.c_int * 3)()
+        self._check(self.lib.cfd_autotune(self.h, int(reps), t, g))
+        return list(t)[: self.dim], list(g)[: self.dim]
+
+    def launch_plan(self):
+        t = (ctypes.c_int * 3)()
+        g = (ctypes.c_int * 3)()
+        self._check(self.lib.cfd_launch_plan(self.h, t, g))
+        return list(t)[: self.dim], list(g)[: self.dim]
+
+    def set_launch_plan(self, tiles, segs):
+        t = (ctypes.c_int * 3)(*(list(tiles) + [1] * (3 - len(tiles))))
+        g = (ctypes.c_int * 3)(*(list(segs) + [1] * (3 - len(segs))))
+        self._check(self.lib.cfd_set_launch_plan(self.h, t, g))
+
     def device_buffers(self):
         s, r = ctypes.c_void_p(), ctypes.c_void_p()
         self._check(self.lib.cfd_device_buffers(self.h, ctypes.byref(s), ctypes.byref(r)))
         return s.value, r.value
+
+
+class Batch:
+    """Independent zones of uneven sizes advanced together (cfd_batch_*)."""
+
+    def __init__(self, dim, zones, inter_method=InterMethod.WCNS5, flux_method=FluxMethod.HLLC,
+                 recon=Recon.CHAR, device=0):
+        """zones: list of dicts with keys n (tuple), h (float or tuple), bc (6 codes),
+        optional bc_prim."""
+        self.lib = load_library()
+        self.dim = dim
+        self.nv = dim + 2
+        cfg = cfd_config()
+        cfg.dim = dim
+        cfg.inter_method = int(inter_method)
+        cfg.flux_method = int(flux_method)
+        cfg.diff_method = DIFF_MND6
+        cfg.recon = int(recon)
+        cfg.device = device
+        arr = (cfd_zone * len(zones))()
+        self.ncell = []
+        for i, z in enumerate(zones):
+            n = list(z["n"]) + [1] * (3 - len(z["n"]))
+            h = z["h"] if hasattr(z["h"], "__len__") else [z["h"]] * dim
+            h = list(h) + [1.0] * (3 - len(h))
+            for d in range(3):
+                arr[i].n[d] = int(n[d])
+                arr[i].h[d] = float(h[d])
+            bc = list(z.get("bc", [BC.OUTFLOW] * 6)) + [BC.OUTFLOW] * 6
+            for f in range(6):
+                arr[i].bc[f] = int(bc[f])
+                for v in range(5):
+                    arr[i].bc_prim[f][v] = float(z["bc_prim"][f][v]) if "bc_prim" in z else 0.0
+            self.ncell.append(n[0] * n[1] * n[2])
+        handle = ctypes.c_void_p()
+        self._check(self.lib.cfd_batch_create(ctypes.byref(cfg), len(zones), arr, ctypes.byref(handle)))
+        self.h = handle
+        self.nz = len(zones)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CfdError(f"libcongfd error {rc}: {self.lib.cfd_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cfd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, zone, cons):
+        c = np.ascontiguousarray(cons, dtype=np.float64).reshape(self.ncell[zone], self.nv)
+        self._check(self.lib.cfd_batch_set_state(self.h, int(zone), _pd(c)))
+
+    def get_state(self, zone):
+        out = np.empty((self.ncell[zone], self.nv))
+        self._check(self.lib.cfd_batch_get_state(self.h, int(zone), _pd(out)))
+        return out
+
+    def steps_cfl(self, cfl, nsteps, mode=BATCH_GRAPH):
+        self._check(self.lib.cfd_batch_steps_cfl(self.h, float(cfl), int(nsteps), int(mode)))
+
+    def times(self):
+        out = np.empty(self.nz)
+        self._check(self.lib.cfd_batch_times(self.h, _pd(out)))
+        return out
+
+    def autotune(self, reps=2):
+        self._check(self.lib.cfd_batch_autotune(self.h, int(reps)))
+
+    def synchronize(self):
+        self._check(self.lib.cfd_batch_synchronize(self.h))

@@ -136,6 +136,53 @@ int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out);
  * hardware byte counters.  Diagnostic; no reference counterpart. */
 int cfd_copy_probe(cfd_handle* h, int reps);
 
+/* Chooses the tile shape and segment length of each direction's sweep by
+ * timing the candidates of the launch model on the current state (reps
+ * launches each).  tiles_out / segs_out (3 ints each, may be NULL) receive the
+ * choice.  No reference counterpart. */
+int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out);
+
+/* Current launch plan: tile shape and segment length per direction. */
+int cfd_launch_plan(cfd_handle* h, int* tiles_out, int* segs_out);
+
+/* Sets the launch plan (3 ints each; entries beyond dim ignored).  Tile codes:
+ * x sweeps 0 (32x8) or 1 (16x16); y/z sweeps 1 (16x16) or 2 (8x32); the
+ * segment length is 1..n[d].  Results do not depend on the plan. */
+int cfd_set_launch_plan(cfd_handle* h, const int* tiles, const int* segs);
+
+/* ---- batches of independent zones ---------------------------------------
+ * Many independent structured zones (separate cases or blocks of uneven
+ * sizes) advanced together.  The reference solves one block per process
+ * (BlockSolver); this has no reference counterpart beyond running
+ * BlockSolver::stepsLoopCFL on each zone.  Every zone takes its own CFL time
+ * step.  Scheme, flux, recon and dim come from `common`; n, h and boundaries
+ * from each zone. */
+typedef struct cfd_zone {
+  int n[3];
+  double h[3];
+  int bc[6];
+  double bc_prim[6][5];
+} cfd_zone;
+
+typedef struct cfd_batch cfd_batch;
+
+#define CFD_BATCH_GRAPH 0        /* persistent work-queue launch per sweep, RK3 step replayed as a hipGraph */
+#define CFD_BATCH_PERSISTENT 1   /* persistent work-queue launches, no graph */
+#define CFD_BATCH_PER_ZONE 2     /* the single-zone kernels launched zone after zone */
+
+int cfd_batch_create(const cfd_config* common, int nzones, const cfd_zone* zones, cfd_batch** out);
+int cfd_batch_destroy(cfd_batch* b);
+int cfd_batch_set_state(cfd_batch* b, int zone, const double* cons_aos);
+int cfd_batch_get_state(cfd_batch* b, int zone, double* cons_aos);
+/* nsteps RK3 steps of every zone with its own device CFL dt (see
+ * cfd_rk3_steps_cfl); asynchronous. */
+int cfd_batch_steps_cfl(cfd_batch* b, double cfl, int nsteps, int mode);
+/* Simulated time reached by each zone (nzones doubles). */
+int cfd_batch_times(cfd_batch* b, double* t_out);
+/* Picks the tile shape per direction and the persistent grid size by timing. */
+int cfd_batch_autotune(cfd_batch* b, int reps);
+int cfd_batch_synchronize(cfd_batch* b);
+
 /* Device pointer of the current state (structure of arrays, nv planes of
  * n[0]*n[1]*n[2] doubles) and of the rhs buffer. */
 int cfd_device_buffers(cfd_handle* h, double** state, double** rhs);
