@@ -97,6 +97,7 @@ def test_launch_plan_does_not_change_results(dim):
              ([0, 1, 2][:dim], [61, 45, 8][:dim]),
              ([1, 1, 1][:dim], [5, 7, 3][:dim])]
     for tiles, segs in plans:
+        segs = [min(g, n[d]) for d, g in enumerate(segs)]
         s.set_launch_plan(tiles, segs)
         assert np.array_equal(s.rhs(), r0), (tiles, segs)
     tuned = s.autotune(2)
