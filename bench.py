@@ -22,8 +22,8 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-from cfd_inwsl_amd import (BATCH_GRAPH, BATCH_PER_ZONE, BATCH_PERSISTENT, BC, Batch, FluxMethod,  # noqa: E402
-                           InterMethod, Recon, Solver, cases)
+from cfd_inwsl_amd import (BATCH_GRAPH, BATCH_PER_ZONE, BATCH_PERSISTENT, BC, PRECISION_FP32,  # noqa: E402
+                           PRECISION_FP64, Batch, FluxMethod, InterMethod, Recon, Solver, cases)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 
@@ -93,6 +93,44 @@ def bench_batch(nzones=96, seed=20261203, steps=3):
             "speedup_graph_vs_per_zone": round(out["per_zone"]["ms_per_step"] / out["graph"]["ms_per_step"], 2)}
 
 
+def bench_fp32(steps_err=100):
+    """FP32 storage + arithmetic in the sweeps, FP64 U^n and stage combination:
+    throughput on both headline grids, and the L1 / Linf difference to the
+    FP64 path (equal to the oracle bit for bit) after `steps_err` RK3 steps of
+    a fixed dt (config T: 3D 256^3, 100 steps)."""
+    out = {}
+    for name, dim, n, mk, inter, flux in (
+            ("2d_vortex_2048", 2, (2048, 2048), lambda: cases.vortex(2048), InterMethod.WCNSZ5, FluxMethod.ROE),
+            ("3d_periodic_256", 3, (256, 256, 256), lambda: cases.periodic3d(256), InterMethod.TCNS5, FluxMethod.HLLC)):
+        cons, h = mk()
+        ncell = int(np.prod(n))
+        res = {}
+        finals = {}
+        for prec in (PRECISION_FP64, PRECISION_FP32):
+            s = Solver(dim, n, h, inter, flux, Recon.CHAR, [BC.PERIODIC] * 6, precision=prec)
+            s.set_state(cons)
+            s.autotune(reps=2)
+            dt = s.cfl_dt(0.5)
+            s.rk3_step(dt)
+            s.synchronize()
+            s.set_state(cons)
+            t0 = time.perf_counter()
+            for _ in range(steps_err):
+                s.rk3_step(dt)
+            s.synchronize()
+            sec = time.perf_counter() - t0
+            res["fp32" if prec else "fp64"] = round(ncell * 3 * steps_err / sec / 1e6, 1)
+            finals[prec] = s.get_state()
+            s.close()
+        a, b = finals[PRECISION_FP32], finals[PRECISION_FP64]
+        scale = np.abs(b).max(axis=0)
+        out[name] = {"mcell_stage_s": res, "steps": steps_err,
+                     "L1_rel": [float(x) for x in np.abs(a - b).mean(axis=0) / scale],
+                     "Linf_rel": [float(x) for x in np.abs(a - b).max(axis=0) / scale]}
+        log(f"[bench] fp32 {name}: {out[name]}")
+    return out
+
+
 def cpu_baseline(seconds_target=15.0):
     """In-repo oracle (OpenMP over pencils, like SpDistributor::rhsSolve) timed
     on the host cores on a bounded sample of the same workload."""
@@ -122,6 +160,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D 256^3 secondary figure")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-zones figure")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -159,6 +198,11 @@ def main():
         log("[bench] batched zones")
         batch = bench_batch()
 
+    fp32 = None
+    if not args.no_fp32:
+        log("[bench] FP32 storage variant")
+        fp32 = bench_fp32()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -185,6 +229,7 @@ def main():
         "cpu_baseline": cpu,
         "secondary": secondary,
         "batch": batch,
+        "fp32": fp32,
     }
     print(json.dumps(line), flush=True)
 

@@ -219,7 +219,7 @@ static int batch_step_persistent(cfd_batch* b, double cfl) {
       a.stage = st + 1;
       a.ct = b->ct;
       int grid = std::min(b->persist_blocks, b->nitems[ax]);
-      BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], nullptr, &a, dim3(grid), b->stream));
+      BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], false, nullptr, &a, dim3(grid), b->stream));
     }
   return 0;
 }
@@ -282,7 +282,7 @@ static int batch_step_per_zone(cfd_batch* b, double cfl) {
         }
         long nb = plan_pencil_blocks(zd.n, ax, b->tile[ax]);
         int nseg = (zd.n[ax] + zd.seg_len[ax] - 1) / zd.seg_len[ax];
-        BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], &a, nullptr,
+        BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], false, &a, nullptr,
                                    dim3((unsigned)nb, (unsigned)nseg), b->stream));
       }
   }
@@ -320,6 +320,7 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
     return cfd_set_error_from_batch(CFD_E_ARG, "flux_method must be HLLC (0) or ROE (1)");
   int sch;
   double ct;
+  if (c->precision != CFD_PRECISION_FP64) return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches run in FP64");
   if (cfd_select_scheme(c->inter_method, &sch, &ct))
     return cfd_set_error_from_batch(CFD_E_UNSUPP, "inter_method is not implemented");
   cfd_batch* b = new cfd_batch();

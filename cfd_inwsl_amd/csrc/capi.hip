@@ -37,7 +37,11 @@ struct cfd_handle {
   int sch, flx;
   bool cw;
   double ct;
-  double *U0, *Ua, *Ub, *R;
+  double* U0;            // FP64 state at the start of the step
+  void *Ua, *Ub, *R;     // stage states and rhs: element type of the sweeps
+  float* U0f;            // FP32 shadow of U0 (FP32 precision only)
+  bool fp32;
+  size_t esz;            // bytes per element of Ua / Ub / R
   double* scal;          // [0] dt, [1..3] max wave speed per direction (bits), [4] t
   double* partial;       // block partial sums for totals
   int npartial;
@@ -93,6 +97,16 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ in, do
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) out[i] = in[i];
 }
 
+__global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+__global__ void k_soa_to_aos_f(const float* __restrict__ in, double* __restrict__ out, long N, int NV) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N)
+    for (int v = 0; v < NV; v++) out[i * NV + v] = (double)in[v * N + i];
+}
+
 __global__ void k_zero_lambda(double* scal) {
   unsigned long long* b = reinterpret_cast<unsigned long long*>(scal);
   b[1] = 0ull;
@@ -136,10 +150,11 @@ __global__ __launch_bounds__(256) void k_wave_speed(const double* __restrict__ U
 }
 
 // max |R[var]| into scal[5] (bits of a non-negative double)
-__global__ __launch_bounds__(256) void k_abs_max(const double* __restrict__ x, long N, double* scal) {
+template <class E>
+__global__ __launch_bounds__(256) void k_abs_max(const E* __restrict__ x, long N, double* scal) {
   double m = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-    double a = fabs(x[i]);
+    double a = fabs((double)x[i]);
     m = m < a ? a : m;
   }
   __shared__ double red[256];
@@ -192,14 +207,15 @@ __global__ __launch_bounds__(256) void k_totals(const double* __restrict__ U, lo
 }
 
 // ---- helpers -----------------------------------------------------------------------
-static hipError_t launch_sweep(cfd_handle* h, int ax, const double* U, int flags, int stage, const double* U0,
-                               double* Uout, int tile = -1, int seg = -1) {
+static hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int stage, const double* U0,
+                               void* Uout, int tile = -1, int seg = -1, void* Ushadow = nullptr) {
   SweepArgs a;
   memset(&a, 0, sizeof(a));
   a.U = U;
   a.R = h->R;
   a.U0 = U0;
   a.Uout = Uout;
+  a.Ushadow = Ushadow;
   a.dtp = h->scal;
   a.N = h->N;
   for (int i = 0; i < 3; i++) a.n[i] = h->cfg.n[i];
@@ -218,27 +234,30 @@ static hipError_t launch_sweep(cfd_handle* h, int ax, const double* U, int flags
   }
   const long nb = plan_pencil_blocks(h->cfg.n, ax, tile);
   const int nseg = (h->cfg.n[ax] + seg - 1) / seg;
-  return launch_sweep_dispatch(h->D, ax, h->sch, h->flx, h->cw, tile, &a, nullptr, dim3((unsigned)nb, (unsigned)nseg),
-                               h->stream);
+  return launch_sweep_dispatch(h->D, ax, h->sch, h->flx, h->cw, tile, h->fp32, &a, nullptr,
+                               dim3((unsigned)nb, (unsigned)nseg), h->stream);
 }
 
 // One RK stage: sweeps over every direction; the last one applies the stage update.
-static int rk_stage(cfd_handle* h, int stage, const double* Ucur, double* Uout) {
+static int rk_stage(cfd_handle* h, int stage, const void* Ucur, void* Uout, void* Ushadow) {
   for (int ax = 0; ax < h->D; ax++) {
     int flags = 0;
     if (ax > 0) flags |= SW_READ_R;
     if (ax < h->D - 1 || h->keep_rhs) flags |= SW_WRITE_R;
     if (ax == h->D - 1) flags |= SW_RK;
-    HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, h->U0, Uout));
+    HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, h->U0, Uout, -1, -1, Ushadow));
   }
   return 0;
 }
 
+// state read by the sweeps at the start of a step
+static const void* sweep_state(cfd_handle* h) { return h->fp32 ? (const void*)h->U0f : (const void*)h->U0; }
+
 static int rk3(cfd_handle* h) {
   int rc;
-  if ((rc = rk_stage(h, 1, h->U0, h->Ua))) return rc;
-  if ((rc = rk_stage(h, 2, h->Ua, h->Ub))) return rc;
-  if ((rc = rk_stage(h, 3, h->Ub, h->U0))) return rc;
+  if ((rc = rk_stage(h, 1, sweep_state(h), h->Ua, nullptr))) return rc;
+  if ((rc = rk_stage(h, 2, h->Ua, h->Ub, nullptr))) return rc;
+  if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->fp32 ? (void*)h->U0f : nullptr))) return rc;
   return 0;
 }
 
@@ -306,6 +325,16 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   h->ct = ct;
   h->flx = cfg->flux_method;
   h->cw = cfg->recon == CFD_RECON_CHAR;
+  if (cfg->precision != CFD_PRECISION_FP64 && cfg->precision != CFD_PRECISION_FP32) {
+    delete h;
+    return fail(CFD_E_ARG, "precision must be CFD_PRECISION_FP64 or CFD_PRECISION_FP32");
+  }
+  h->fp32 = cfg->precision == CFD_PRECISION_FP32;
+  if (h->fp32 && cfg->dim == 1) {
+    delete h;
+    return fail(CFD_E_UNSUPP, "FP32 precision is available for dim 2 and 3");
+  }
+  h->esz = h->fp32 ? sizeof(float) : sizeof(double);
   for (int ax = 0; ax < 3; ax++) {
     h->tile[ax] = ax == 0 ? 0 : 1;
     h->seg[ax] = cfg->n[ax];
@@ -314,11 +343,14 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   hipError_t e = hipSetDevice(cfg->device);
   if (e != hipSuccess) { delete h; return fail(CFD_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
   size_t bytes = sizeof(double) * h->N * h->NV;
+  size_t ebytes = h->esz * h->N * h->NV;
   h->npartial = 1024;
   e = hipMalloc(&h->U0, bytes);
+  // Ua doubles as the host-transfer staging buffer, so it is FP64-sized
   if (e == hipSuccess) e = hipMalloc(&h->Ua, bytes);
   if (e == hipSuccess) e = hipMalloc(&h->Ub, bytes);
-  if (e == hipSuccess) e = hipMalloc(&h->R, bytes);
+  if (e == hipSuccess) e = hipMalloc(&h->R, ebytes);
+  if (e == hipSuccess && h->fp32) e = hipMalloc(&h->U0f, sizeof(float) * h->N * h->NV);
   if (e == hipSuccess) e = hipMalloc(&h->scal, 8 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->partial, sizeof(double) * 2 * 5 * h->npartial);
   if (e == hipSuccess) e = hipStreamCreate(&h->stream);
@@ -340,6 +372,7 @@ int cfd_destroy(cfd_handle* h) {
   (void)hipFree(h->Ua);
   (void)hipFree(h->Ub);
   (void)hipFree(h->R);
+  if (h->U0f) (void)hipFree(h->U0f);
   (void)hipFree(h->scal);
   (void)hipFree(h->partial);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -351,8 +384,11 @@ int cfd_set_state(cfd_handle* h, const double* cons) {
   if (!h || !cons) return fail(CFD_E_ARG, "null argument");
   size_t bytes = sizeof(double) * h->N * h->NV;
   HIPCHK(hipMemcpyAsync(h->Ub, cons, bytes, hipMemcpyHostToDevice, h->stream));
-  hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream, h->Ub, h->U0, h->N,
-                     h->NV);
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream,
+                     static_cast<const double*>(h->Ub), h->U0, h->N, h->NV);
+  if (h->fp32)
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((h->N * h->NV + 255) / 256)), dim3(256), 0, h->stream, h->U0, h->U0f,
+                       h->N * h->NV);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   return CFD_OK;
@@ -361,8 +397,8 @@ int cfd_set_state(cfd_handle* h, const double* cons) {
 int cfd_get_state(cfd_handle* h, double* cons) {
   if (!h || !cons) return fail(CFD_E_ARG, "null argument");
   size_t bytes = sizeof(double) * h->N * h->NV;
-  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream, h->U0, h->Ub, h->N,
-                     h->NV);
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream, h->U0,
+                     static_cast<double*>(h->Ub), h->N, h->NV);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(cons, h->Ub, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -373,12 +409,16 @@ int cfd_eval_rhs(cfd_handle* h, double* rhs) {
   if (!h) return fail(CFD_E_ARG, "null handle");
   for (int ax = 0; ax < h->D; ax++) {
     int flags = SW_WRITE_R | (ax > 0 ? SW_READ_R : 0);
-    HIPCHK(launch_sweep(h, ax, h->U0, flags, 0, nullptr, nullptr));
+    HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 0, nullptr, nullptr));
   }
   if (rhs) {
     size_t bytes = sizeof(double) * h->N * h->NV;
-    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream, h->R, h->Ub, h->N,
-                       h->NV);
+    if (h->fp32)
+      hipLaunchKernelGGL(k_soa_to_aos_f, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream,
+                         static_cast<const float*>(h->R), static_cast<double*>(h->Ub), h->N, h->NV);
+    else
+      hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream,
+                         static_cast<const double*>(h->R), static_cast<double*>(h->Ub), h->N, h->NV);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(rhs, h->Ub, bytes, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -419,7 +459,12 @@ int cfd_rhs_linf(cfd_handle* h, int var, double* out) {
   if (!h || !out || var < 0 || var >= h->NV) return fail(CFD_E_ARG, "bad argument");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 5, 0.0);
   int blocks = (int)std::min<long>((h->N + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_abs_max, dim3(blocks), dim3(256), 0, h->stream, h->R + (long)var * h->N, h->N, h->scal);
+  if (h->fp32)
+    hipLaunchKernelGGL(k_abs_max<float>, dim3(blocks), dim3(256), 0, h->stream,
+                       static_cast<const float*>(h->R) + (long)var * h->N, h->N, h->scal);
+  else
+    hipLaunchKernelGGL(k_abs_max<double>, dim3(blocks), dim3(256), 0, h->stream,
+                       static_cast<const double*>(h->R) + (long)var * h->N, h->N, h->scal);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, h->scal + 5, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -465,7 +510,7 @@ int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out) {
       if (ax < h->D - 1) flags |= SW_WRITE_R;
       else flags |= SW_RK;
       HIPCHK(hipEventRecord(ev[2 * ax], h->stream));
-      HIPCHK(launch_sweep(h, ax, h->U0, flags, 1, h->U0, h->Ua));
+      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua));
       HIPCHK(hipEventRecord(ev[2 * ax + 1], h->stream));
     }
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -485,7 +530,7 @@ int cfd_copy_probe(cfd_handle* h, int reps) {
   long n2 = h->N * h->NV / 2;
   for (int r = 0; r < reps; r++)
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, h->stream, reinterpret_cast<const double2*>(h->U0),
-                       reinterpret_cast<double2*>(h->Ua), n2);
+                       static_cast<double2*>(h->Ua), n2);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   return CFD_OK;
@@ -506,10 +551,10 @@ int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out) {
     const int flags = SW_WRITE_R | (ax > 0 ? SW_READ_R : 0);
     for (int c = 0; c < nc; c++) {
       // warm-up launch, then timed launches
-      HIPCHK(launch_sweep(h, ax, h->U0, flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
+      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
       HIPCHK(hipEventRecord(e0, h->stream));
       for (int r = 0; r < reps; r++)
-        HIPCHK(launch_sweep(h, ax, h->U0, flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
+        HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
       HIPCHK(hipEventRecord(e1, h->stream));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0;
@@ -560,7 +605,7 @@ int cfd_synchronize(cfd_handle* h) {
 int cfd_device_buffers(cfd_handle* h, double** state, double** rhs) {
   if (!h) return fail(CFD_E_ARG, "null handle");
   if (state) *state = h->U0;
-  if (rhs) *rhs = h->R;
+  if (rhs) *rhs = static_cast<double*>(h->R);
   return CFD_OK;
 }
 

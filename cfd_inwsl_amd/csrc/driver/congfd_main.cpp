@@ -29,7 +29,7 @@ static void usage() {
                "usage: congfd [info.txt] [--flux hllc|roe] [--recon char|component] [--inter N]\n"
                "              [--case N] [--dim N] [--imax I J K] [--zone x0 x1 y0 y1 z0 z1]\n"
                "              [--endstep N] [--outputdt T] [--cfl C] [--true-periodic] [--device N]\n"
-               "              [--outdir DIR] [--no-output]\n");
+               "              [--outdir DIR] [--no-output] [--fp32]\n");
 }
 
 #define CHECK(call)                                                                      \
@@ -104,6 +104,7 @@ int main(int argc, char** argv) {
     else if (s == "--device") info.device = std::atoi(next());
     else if (s == "--outdir") outdir = next();
     else if (s == "--no-output") output = false;
+    else if (s == "--fp32") info.precision = CFD_PRECISION_FP32;
     else { usage(); return 2; }
   }
 
@@ -115,6 +116,7 @@ int main(int argc, char** argv) {
   cfg.diff_method = info.diffMethod;
   cfg.recon = info.recon;
   cfg.device = info.device;
+  cfg.precision = info.precision;
   std::vector<double> cons;
   std::string why = setupCase(info, cons, cfg);
   if (!why.empty()) {

@@ -30,6 +30,7 @@ struct Info {
   int threads = 1;                                    // info.txt line 16 (OpenMP threads: unused on GPU)
   bool truePeriodic = false;                          // use a true periodic halo where the reference mirrors
   int device = 0;
+  int precision = CFD_PRECISION_FP64;
 
   std::array<int, 3> icMax() const;   // cells per direction (info.cpp:115-128)
   double geth() const;                // one spacing for every direction (info.cpp:310-314, initializer.cpp:594-597)

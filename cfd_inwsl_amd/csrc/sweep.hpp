@@ -38,11 +38,16 @@ enum BcCode {
 // flags of a sweep launch
 enum SweepFlags { SW_READ_R = 1, SW_WRITE_R = 2, SW_RK = 4 };
 
+// Buffers are typed by the kernel's compute type T (double or float): U, R
+// and the stage outputs hold T; U0 (the state at the start of the step) is
+// always FP64.  With T = float the stage-3 output is the FP64 state U0 plus
+// its FP32 shadow `Ushadow` read by the next step's sweeps.
 struct SweepArgs {
-  const double* U;   // state whose RHS is evaluated (SoA planes of N)
-  double* R;         // RHS accumulator (SoA)
-  const double* U0;  // RK: state at the start of the step
-  double* Uout;      // RK: stage output
+  const void* U;     // state whose RHS is evaluated (SoA planes of N), T
+  void* R;           // RHS accumulator (SoA), T
+  const double* U0;  // RK: state at the start of the step (FP64)
+  void* Uout;        // RK: stage output (T; FP64 at stage 3 when T = float)
+  void* Ushadow;     // RK, T = float, stage 3: FP32 copy of the new state
   const double* dtp; // RK: device time step
   long N;            // cells per variable plane
   int n[3];          // cells per direction
@@ -70,17 +75,20 @@ __host__ __device__ constexpr int tile_cp(int t) { return t == 0 ? 8 : (t == 1 ?
 #endif
 
 // One workgroup's share of a sweep: pencil group bx, segment by.
-template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE>
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
 __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, const int by) {
   constexpr int NV = D + 2;
+  const T* __restrict__ Ut = static_cast<const T*>(a.U);
+  T* Rt = static_cast<T*>(a.R);
+  const T ct = T(a.ct);
   constexpr int CA = Tile<TILE>::CA, CP = Tile<TILE>::CP;
   constexpr int RC = CA + 5;  // cell ring slots
   constexpr int FR = CA + 3;  // flux ring slots
   constexpr int NCV = CHARW ? 2 * NV : NV;  // per cell: prim [+ rebuilt cons]
   // LDS layout: [var][slot][pencil] for AX>0 (pencil fastest = lane fastest),
   //             [var][pencil][slot] for AX=0 (slot fastest = lane fastest).
-  __shared__ double cell[NCV * RC * CP];
-  __shared__ double flux[NV * FR * CP];
+  __shared__ T cell[NCV * RC * CP];
+  __shared__ T flux[NV * FR * CP];
   auto cidx = [](int v, int slot, int p) -> int {
     return AX == 0 ? (v * CP + p) * RC + slot : (v * RC + slot) * CP + p;
   };
@@ -109,7 +117,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 
   // ---- load one cell (position g along AX, may be a ghost) into the ring ----
   auto load_cell = [&](int g) {
-    double pr[NV];
+    T pr[NV];
     int flip = -1;
     bool dir = false;
     int src = g;
@@ -127,12 +135,12 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       default: src = lo ? -1 - g : 2 * n - 1 - g; break;  // mirror copy
       }
       if (dir)
-        for (int v = 0; v < NV; v++) pr[v] = lo ? a.prim_lo[v] : a.prim_hi[v];
+        for (int v = 0; v < NV; v++) pr[v] = T(lo ? a.prim_lo[v] : a.prim_hi[v]);
     }
     if (!dir) {
-      double cs[NV];
+      T cs[NV];
       const long idx = base + (long)src * sa;
-      for (int v = 0; v < NV; v++) cs[v] = a.U[v * a.N + idx];
+      for (int v = 0; v < NV; v++) cs[v] = Ut[v * a.N + idx];
       cons_to_prim<D>(cs, pr);
       for (int v = 1; v < NV - 1; v++)
         if (v == flip) pr[v] = -pr[v];
@@ -140,7 +148,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
     const int slot = (g + 8 * RC) % RC;
     for (int v = 0; v < NV; v++) cell[cidx(v, slot, lp)] = pr[v];
     if (CHARW) {
-      double cc[NV];
+      T cc[NV];
       prim_to_cons<D>(pr, cc);
       for (int v = 0; v < NV; v++) cell[cidx(NV + v, slot, lp)] = cc[v];
     }
@@ -165,40 +173,40 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
         // window cells m-4+j, j = 0..5; the interface lies between m-2 and m-1
         int sl[6];
         for (int j = 0; j < 6; j++) sl[j] = (m - 4 + j + 8 * RC) % RC;
-        double outL[NV], outR[NV];
+        T outL[NV], outR[NV];
         if (!CHARW) {
           for (int v = 0; v < NV; v++) {
-            double q0 = cell[cidx(v, sl[0], lp)], q1 = cell[cidx(v, sl[1], lp)], q2 = cell[cidx(v, sl[2], lp)];
-            double q3 = cell[cidx(v, sl[3], lp)], q4 = cell[cidx(v, sl[4], lp)], q5 = cell[cidx(v, sl[5], lp)];
-            outL[v] = inter5<SCH>(q0, q1, q2, q3, q4, a.ct);
-            outR[v] = inter5<SCH>(q5, q4, q3, q2, q1, a.ct);
+            T q0 = cell[cidx(v, sl[0], lp)], q1 = cell[cidx(v, sl[1], lp)], q2 = cell[cidx(v, sl[2], lp)];
+            T q3 = cell[cidx(v, sl[3], lp)], q4 = cell[cidx(v, sl[4], lp)], q5 = cell[cidx(v, sl[5], lp)];
+            outL[v] = inter5<SCH>(q0, q1, q2, q3, q4, ct);
+            outR[v] = inter5<SCH>(q5, q4, q3, q2, q1, ct);
           }
         } else {
-          double pL[NV], pR[NV];
+          T pL[NV], pR[NV];
           for (int v = 0; v < NV; v++) {
             pL[v] = cell[cidx(v, sl[2], lp)];
             pR[v] = cell[cidx(v, sl[3], lp)];
           }
           if (D == 1) {
-            RoeSys1 es;
+            RoeSys1<T> es;
             es.build(pL, pR);
-            double QL[3], QR[3];
+            T QL[3], QR[3];
             for (int k = 0; k < 3; k++) {
-              double ch[6];
+              T ch[6];
               for (int j = 0; j < 6; j++) {
-                double c0v = cell[cidx(NV + 0, sl[j], lp)], c1v = cell[cidx(NV + 1, sl[j], lp)],
-                       c2v = cell[cidx(NV + 2, sl[j], lp)];
+                T c0v = cell[cidx(NV + 0, sl[j], lp)], c1v = cell[cidx(NV + 1, sl[j], lp)],
+                  c2v = cell[cidx(NV + 2, sl[j], lp)];
                 ch[j] = c0v * es.L[3 * k] + c1v * es.L[3 * k + 1] + c2v * es.L[3 * k + 2];
               }
-              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], a.ct);
-              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], a.ct);
+              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct);
+              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct);
             }
-            double cL[3], cR[3];
+            T cL[3], cR[3];
             for (int i = 0; i < 3; i++) {
               cL[i] = QL[0] * es.R[3 * i] + QL[1] * es.R[3 * i + 1] + QL[2] * es.R[3 * i + 2];
               cR[i] = QR[0] * es.R[3 * i] + QR[1] * es.R[3 * i + 1] + QR[2] * es.R[3 * i + 2];
             }
-            double consL[3], consR[3];
+            T consL[3], consR[3];
             prim_to_cons<1>(pL, consL);
             prim_to_cons<1>(pR, consR);
             positive_limit(consL, cL, 0);
@@ -208,26 +216,26 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
             eig_cons_to_prim<1>(cL, outL);
             eig_cons_to_prim<1>(cR, outR);
           } else {
-            RoeSys<D, AX> es;
+            RoeSys<D, AX, T> es;
             es.build(pL, pR);
-            double cw[6][NV];
+            T cw[6][NV];
             for (int j = 0; j < 6; j++)
               for (int v = 0; v < NV; v++) cw[j][v] = cell[cidx(NV + v, sl[j], lp)];
-            double QL[NV], QR[NV];
+            T QL[NV], QR[NV];
             for (int k = 0; k < NV; k++) {
-              double l[NV];
+              T l[NV];
               es.lrow(k, l);
-              double ch[6];
+              T ch[6];
               for (int j = 0; j < 6; j++) ch[j] = project<D>(cw[j], l);
-              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], a.ct);
-              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], a.ct);
+              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct);
+              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct);
             }
             // charToCons in the summation order (ch3, ch4, ch2, ch1[, ch5])
-            double cL[NV], cR[NV];
+            T cL[NV], cR[NV];
             const int order[5] = {2, 3, 1, 0, 4};
             for (int t = 0; t < NV; t++) {
               const int k = order[t];
-              double r[NV];
+              T r[NV];
               es.rcol(k, r);
               for (int i = 0; i < NV; i++) {
                 if (t == 0) {
@@ -251,7 +259,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
             }
           }
         }
-        double F[NV];
+        T F[NV];
         if (D == 1) {
           if (FLX == FLX_HLLC) hllc1d(outL, outR, F);
           else roe1d(outL, outR, F);
@@ -268,7 +276,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
     {
       const int i = m0 - 3 + la;
       if (pvalid && i >= c0 && i < c1) {
-        double pm[NV], pp[NV], fm[NV], fp[NV];
+        T pm[NV], pp[NV], fm[NV], fp[NV];
         const int sm = (i - 1 + 8 * RC) % RC, sp = (i + 1 + 8 * RC) % RC;
         for (int v = 0; v < NV; v++) {
           pm[v] = cell[cidx(v, sm, lp)];
@@ -280,24 +288,32 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
         const long idx = base + (long)i * sa;
         double dtv = 0.0;
         if (a.flags & SW_RK) dtv = *a.dtp;
+        const T hh = T(a.h);
         for (int v = 0; v < NV; v++) {
-          const double H0 = flux[fidx(v, f0, lp)], H1 = flux[fidx(v, f1, lp)];
-          const double H2 = flux[fidx(v, f2, lp)], H3 = flux[fidx(v, f3, lp)];
-          const double term = (w1 * (fp[v] - fm[v]) + w0 * (H2 - H1) + w2 * (H3 - H0)) / a.h;
-          const double rold = (a.flags & SW_READ_R) ? a.R[v * a.N + idx] : 0.0;
-          const double r = rold + term;
-          if (a.flags & SW_WRITE_R) a.R[v * a.N + idx] = r;
+          const T H0 = flux[fidx(v, f0, lp)], H1 = flux[fidx(v, f1, lp)];
+          const T H2 = flux[fidx(v, f2, lp)], H3 = flux[fidx(v, f3, lp)];
+          const T term = (T(w1) * (fp[v] - fm[v]) + T(w0) * (H2 - H1) + T(w2) * (H3 - H0)) / hh;
+          const T rold = (a.flags & SW_READ_R) ? Rt[v * a.N + idx] : T(0.0);
+          const T r = rold + term;
+          if (a.flags & SW_WRITE_R) Rt[v * a.N + idx] = r;
           if (a.flags & SW_RK) {
+            // stage combination in FP64 (BlockSolver::RK3_SSP, blockSolver.cpp:60,72,84)
             const double u0 = a.U0[v * a.N + idx];
+            const double rd = (double)r;
             double un;
             if (a.stage == 1) {
-              un = u0 - dtv * r;
+              un = u0 - dtv * rd;
             } else {
-              const double uc = a.U[v * a.N + idx];
-              if (a.stage == 2) un = 0.75 * u0 - 0.25 * dtv * r + 0.25 * uc;
-              else un = 1.0 / 3.0 * u0 - 2.0 / 3.0 * dtv * r + 2.0 / 3.0 * uc;
+              const double uc = (double)Ut[v * a.N + idx];
+              if (a.stage == 2) un = 0.75 * u0 - 0.25 * dtv * rd + 0.25 * uc;
+              else un = 1.0 / 3.0 * u0 - 2.0 / 3.0 * dtv * rd + 2.0 / 3.0 * uc;
             }
-            a.Uout[v * a.N + idx] = un;
+            if (sizeof(T) == sizeof(double) || a.stage != 3) {
+              static_cast<T*>(a.Uout)[v * a.N + idx] = (T)un;
+            } else {
+              static_cast<double*>(a.Uout)[v * a.N + idx] = un;
+              static_cast<T*>(a.Ushadow)[v * a.N + idx] = (T)un;
+            }
           }
         }
       }
@@ -306,9 +322,9 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
   }
 }
 
-template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE>
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
 __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_kernel(SweepArgs a) {
-  sweep_block<D, AX, SCH, FLX, CHARW, TILE>(a, blockIdx.x, blockIdx.y);
+  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T>(a, blockIdx.x, blockIdx.y);
 }
 
 // ---- batched zones ------------------------------------------------------------
@@ -365,6 +381,7 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     a.R = b.R + zd.off;
     a.U0 = b.U0 + zd.off;
     a.Uout = b.Uout + zd.off;
+    a.Ushadow = nullptr;
     a.dtp = b.dt + z;
     a.N = zd.N;
     a.n[0] = zd.n[0];
@@ -381,7 +398,7 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
       a.prim_lo[v] = zd.prim[2 * AX][v];
       a.prim_hi[v] = zd.prim[2 * AX + 1][v];
     }
-    sweep_block<D, AX, SCH, FLX, CHARW, TILE>(a, __builtin_amdgcn_readfirstlane(w.bx),
+    sweep_block<D, AX, SCH, FLX, CHARW, TILE, double>(a, __builtin_amdgcn_readfirstlane(w.bx),
                                               __builtin_amdgcn_readfirstlane(w.by));
   }
 }

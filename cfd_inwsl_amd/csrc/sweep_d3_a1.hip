@@ -1,9 +1,10 @@
 // Sweep kernels for 3D Euler, direction 1.
+#define CFD_WITH_FP32 1
 #include "sweep_inst.hpp"
 
 namespace cfd {
-hipError_t launch_d3_a1(int sch, int flx, bool cw, int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid,
-                          hipStream_t s) {
-  return launch_sch<3, 1>(sch, flx, cw, tile, a, b, grid, s);
+hipError_t launch_d3_a1(int sch, int flx, bool cw, int tile, bool fp32, const SweepArgs* a, const BatchArgs* b,
+                          dim3 grid, hipStream_t s) {
+  return launch_sch<3, 1>(sch, flx, cw, tile, fp32, a, b, grid, s);
 }
 }  // namespace cfd

@@ -13,46 +13,57 @@ template <> struct TilePair<0> { static constexpr int A = 0, B = 1; };
 template <> struct TilePair<1> { static constexpr int A = 1, B = 2; };
 template <> struct TilePair<2> { static constexpr int A = 1, B = 2; };
 
-template <int D, int AX, int SCH, int FLX, bool CW, int T>
-static hipError_t launch_one(const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
-  if (a) hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW, T>), grid, dim3(256), 0, s, *a);
-  else hipLaunchKernelGGL((sweep_batch_kernel<D, AX, SCH, FLX, CW, T>), grid, dim3(256), 0, s, *b);
+template <int D, int AX, int SCH, int FLX, bool CW, int TL>
+static hipError_t launch_one(bool fp32, const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
+  if (b) {
+    hipLaunchKernelGGL((sweep_batch_kernel<D, AX, SCH, FLX, CW, TL>), grid, dim3(256), 0, s, *b);
+  } else if (fp32) {
+#if CFD_WITH_FP32
+    hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW, TL, float>), grid, dim3(256), 0, s, *a);
+#else
+    return hipErrorInvalidValue;
+#endif
+  } else {
+    hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW, TL, double>), grid, dim3(256), 0, s, *a);
+  }
   return hipGetLastError();
 }
 
 template <int D, int AX, int SCH, int FLX, bool CW>
-static hipError_t launch_tile(int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
-  if (tile == TilePair<AX>::A) return launch_one<D, AX, SCH, FLX, CW, TilePair<AX>::A>(a, b, grid, s);
-  if (tile == TilePair<AX>::B) return launch_one<D, AX, SCH, FLX, CW, TilePair<AX>::B>(a, b, grid, s);
+static hipError_t launch_tile(int tile, bool fp32, const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
+  if (tile == TilePair<AX>::A) return launch_one<D, AX, SCH, FLX, CW, TilePair<AX>::A>(fp32, a, b, grid, s);
+  if (tile == TilePair<AX>::B) return launch_one<D, AX, SCH, FLX, CW, TilePair<AX>::B>(fp32, a, b, grid, s);
   return hipErrorInvalidValue;
 }
 
 template <int D, int AX, int SCH, int FLX>
-static hipError_t launch_cw(bool cw, int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
-  return cw ? launch_tile<D, AX, SCH, FLX, true>(tile, a, b, grid, s)
-            : launch_tile<D, AX, SCH, FLX, false>(tile, a, b, grid, s);
+static hipError_t launch_cw(bool cw, int tile, bool fp32, const SweepArgs* a, const BatchArgs* b, dim3 grid,
+                            hipStream_t s) {
+  return cw ? launch_tile<D, AX, SCH, FLX, true>(tile, fp32, a, b, grid, s)
+            : launch_tile<D, AX, SCH, FLX, false>(tile, fp32, a, b, grid, s);
 }
 
 template <int D, int AX, int SCH>
-static hipError_t launch_flx(int flx, bool cw, int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid,
+static hipError_t launch_flx(int flx, bool cw, int tile, bool fp32, const SweepArgs* a, const BatchArgs* b, dim3 grid,
                              hipStream_t s) {
-  return flx == FLX_HLLC ? launch_cw<D, AX, SCH, FLX_HLLC>(cw, tile, a, b, grid, s)
-                         : launch_cw<D, AX, SCH, FLX_ROE>(cw, tile, a, b, grid, s);
+  return flx == FLX_HLLC ? launch_cw<D, AX, SCH, FLX_HLLC>(cw, tile, fp32, a, b, grid, s)
+                         : launch_cw<D, AX, SCH, FLX_ROE>(cw, tile, fp32, a, b, grid, s);
 }
 
 template <int D, int AX>
-static hipError_t launch_sch(int sch, int flx, bool cw, int tile, const SweepArgs* a, const BatchArgs* b, dim3 grid,
-                             hipStream_t s) {
+static hipError_t launch_sch(int sch, int flx, bool cw, int tile, bool fp32, const SweepArgs* a, const BatchArgs* b,
+                             dim3 grid, hipStream_t s) {
   switch (sch) {
-  case SCH_JS: return launch_flx<D, AX, SCH_JS>(flx, cw, tile, a, b, grid, s);
-  case SCH_Z: return launch_flx<D, AX, SCH_Z>(flx, cw, tile, a, b, grid, s);
-  case SCH_TENO: return launch_flx<D, AX, SCH_TENO>(flx, cw, tile, a, b, grid, s);
-  default: return launch_flx<D, AX, SCH_TENOS>(flx, cw, tile, a, b, grid, s);
+  case SCH_JS: return launch_flx<D, AX, SCH_JS>(flx, cw, tile, fp32, a, b, grid, s);
+  case SCH_Z: return launch_flx<D, AX, SCH_Z>(flx, cw, tile, fp32, a, b, grid, s);
+  case SCH_TENO: return launch_flx<D, AX, SCH_TENO>(flx, cw, tile, fp32, a, b, grid, s);
+  default: return launch_flx<D, AX, SCH_TENOS>(flx, cw, tile, fp32, a, b, grid, s);
   }
 }
 
 // Entry points defined in sweep_d<D>_<axis>.hip
-hipError_t launch_sweep_dispatch(int D, int ax, int sch, int flx, bool cw, int tile, const SweepArgs* a,
+// fp32: FP32 storage / arithmetic kernels (not compiled for 1D).
+hipError_t launch_sweep_dispatch(int D, int ax, int sch, int flx, bool cw, int tile, bool fp32, const SweepArgs* a,
                                  const BatchArgs* b, dim3 grid, hipStream_t s);
 
 }  // namespace cfd

@@ -62,7 +62,11 @@ class cfd_config(ctypes.Structure):
         ("bc", ctypes.c_int * 6),
         ("bc_prim", (ctypes.c_double * 5) * 6),
         ("device", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
+
+
+PRECISION_FP64, PRECISION_FP32 = 0, 1
 
 
 class cfd_zone(ctypes.Structure):
@@ -158,7 +162,7 @@ class Solver:
     """
 
     def __init__(self, dim, n, h, inter_method=InterMethod.WCNS5, flux_method=FluxMethod.ROE,
-                 recon=Recon.CHAR, bc=None, bc_prim=None, device=0):
+                 recon=Recon.CHAR, bc=None, bc_prim=None, device=0, precision=PRECISION_FP64):
         self.lib = load_library()
         self.dim = dim
         self.nv = dim + 2
@@ -183,6 +187,7 @@ class Solver:
             for v in range(5):
                 cfg.bc_prim[i][v] = float(bc_prim[i][v]) if bc_prim is not None else 0.0
         cfg.device = device
+        cfg.precision = int(precision)
         self.cfg = cfg
         handle = ctypes.c_void_p()
         self._check(self.lib.cfd_create(ctypes.byref(cfg), ctypes.byref(handle)))

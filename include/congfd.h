@@ -72,7 +72,14 @@ typedef struct cfd_config {
   int bc[6];            /* per face: x-, x+, y-, y+, z-, z+ */
   double bc_prim[6][5]; /* Dirichlet ghost state per face, primitive variables */
   int device;           /* HIP device ordinal */
+  int precision;        /* CFD_PRECISION_FP64 (reference arithmetic, bitwise) or
+                           CFD_PRECISION_FP32 (FP32 storage and arithmetic of the
+                           sweeps; U^n and the RK3 stage combination stay FP64);
+                           FP32 is available for dim 2 and 3, single zones */
 } cfd_config;
+
+#define CFD_PRECISION_FP64 0
+#define CFD_PRECISION_FP32 1
 
 typedef struct cfd_handle cfd_handle;
 

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_matches_header_size():
-    # dim + n[3] | h[3] | 4 ints | bc[6] | bc_prim[6][5] | device, padded to 8
+    # dim + n[3] | h[3] | 4 ints | bc[6] | bc_prim[6][5] | device, precision
     c = solver.cfd_config()
     assert ctypes.sizeof(c) == 4 + 12 + 24 + 16 + 24 + 240 + 4 + 4
 
@@ -61,3 +61,20 @@ def test_bad_config_is_rejected_before_any_gpu_call(field, value, msg):
     assert rc < 0
     assert msg in lib.cfd_last_error().decode()
     assert not h.value
+
+
+def test_fp32_rejected_for_1d_and_batches():
+    lib = solver.load_library()
+    cfg = solver.cfd_config()
+    cfg.dim = 1
+    cfg.n[0], cfg.n[1], cfg.n[2] = 16, 1, 1
+    cfg.h[0] = cfg.h[1] = cfg.h[2] = 0.1
+    cfg.inter_method, cfg.flux_method, cfg.diff_method, cfg.recon = 2, 0, 3, 0
+    for i in range(6):
+        cfg.bc[i] = 7
+    cfg.precision = 1
+    h = ctypes.c_void_p()
+    assert lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h)) < 0
+    assert "FP32" in lib.cfd_last_error().decode()
+    cfg.precision = 7
+    assert lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h)) < 0

@@ -1,0 +1,72 @@
+"""FP32-storage / FP32-arithmetic variant: sweeps in FP32, U^n and the RK3
+stage combination in FP64.  Its error against the FP64 path (which equals the
+oracle bit for bit, see test_gpu_parity.py) is measured and bounded."""
+import sys
+
+import numpy as np
+import pytest
+
+from cfd_inwsl_amd import BC, PRECISION_FP32, PRECISION_FP64, FluxMethod, InterMethod, Recon, Solver, cases
+from oracle import oracle as O
+from tests.helpers import oracle_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _errs(a, b):
+    scale = np.abs(b).max(axis=0)
+    l1 = np.abs(a - b).mean(axis=0) / scale
+    linf = np.abs(a - b).max(axis=0) / scale
+    return l1, linf
+
+
+def test_fp32_rhs_against_oracle():
+    cons, h = cases.vortex(128)
+    bc = [BC.PERIODIC] * 4
+    s = Solver(2, (128, 128), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, bc, precision=PRECISION_FP32)
+    s.set_state(cons)
+    g = s.rhs()
+    s.close()
+    ref, _ = O.rhs(oracle_grid(2, (128, 128), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, bc), cons)
+    l1, linf = _errs(g, ref)
+    _say("fp32 rhs vortex 128^2: L1", l1, "Linf", linf)
+    assert np.all(linf < 1e-4)
+    assert np.all(l1 < 1e-5)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fp32_steps_against_fp64(dim):
+    if dim == 2:
+        n = (256, 256)
+        cons, h = cases.vortex(256)
+        inter, flux = InterMethod.WCNSZ5, FluxMethod.ROE
+        steps = 40
+    else:
+        n = (48, 48, 48)
+        cons, h = cases.periodic3d(48)
+        inter, flux = InterMethod.TCNS5, FluxMethod.HLLC
+        steps = 20
+    bc = [BC.PERIODIC] * 6
+    out = {}
+    for prec in (PRECISION_FP64, PRECISION_FP32):
+        s = Solver(dim, n, h, inter, flux, Recon.CHAR, bc, precision=prec)
+        s.set_state(cons)
+        t0 = s.totals()
+        dt = s.cfl_dt(0.5)
+        for _ in range(steps):
+            s.rk3_step(dt)
+        out[prec] = (s.get_state(), s.totals(), t0)
+        s.close()
+    l1, linf = _errs(out[PRECISION_FP32][0], out[PRECISION_FP64][0])
+    _say(f"fp32 vs fp64 dim={dim} after {steps} steps: L1 {l1} Linf {linf}")
+    assert np.all(linf < 1e-4)
+    # conserved totals: FP64 U^n, FP32 fluxes
+    t1, t0 = out[PRECISION_FP32][1], out[PRECISION_FP32][2]
+    drift = np.abs(t1 - t0) / np.abs(cons).sum(0)
+    _say("fp32 totals drift", drift)
+    assert np.all(drift < 1e-6)
