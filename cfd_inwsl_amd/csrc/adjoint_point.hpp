@@ -1,0 +1,578 @@
+// Hand-written reverse-mode (vector-Jacobian) counterparts of the 2D
+// characteristic-wise WENO-JS + HLLC interface flux, the node flux and the
+// conservative->primitive map.  Each *_vjp function recomputes its forward
+// intermediates and walks them backwards; branch decisions (HLLC wave
+// pattern, ps clipping, shock factors, positivity fallback) are those of the
+// forward evaluation, i.e. the derivative of the active branch.
+//
+// Forward formulas: weno5_JSchen (zone_FD/src/include/interScheme.hpp:17-43),
+// eigensystemEuler2D (zone_FD/src/src/eigenSystem.cpp:13-168),
+// Recon5Order2DEulerEig::reconI (zone_FD/src/include/reconstructor5order.hpp:247-356),
+// HLLCFlux2D2 (zone_FD/src/include/fluxSchemes.hpp:202-327),
+// fluxSolveEuler2D (fluxSchemes.hpp:583-599), Equation::consToPrimEuler2D
+// (zone_FD/src/src/equation.cpp:67-99).
+//
+// The adjoint needs derivatives, not the reference's rounding, so it is free
+// to reassociate; it is compiled for host (unit tests against finite
+// differences) and device.
+#pragma once
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define CFD_HD __host__ __device__
+#else
+#define CFD_HD
+#endif
+
+namespace cfdadj {
+
+constexpr double G = 1.4;
+constexpr double G1 = G - 1.0;          // gamma - 1
+constexpr double GG1 = G / (G - 1.0);   // gamma / (gamma - 1)
+
+// ---- WENO-JS half-node value and its gradient ---------------------------------
+struct JS {
+  double b[3], a[3], u[3], e[3], S, out;
+  CFD_HD void fwd(const double* q) {
+    double A0 = q[0] - 2.0 * q[1] + q[2], B0 = q[0] - 4.0 * q[1] + 3.0 * q[2];
+    double A1 = q[1] - 2.0 * q[2] + q[3], B1 = q[1] - q[3];
+    double A2 = q[2] - 2.0 * q[3] + q[4], B2 = 3.0 * q[2] - 4.0 * q[3] + q[4];
+    b[0] = A0 * A0 + 0.25 * B0 * B0;
+    b[1] = A1 * A1 + 0.25 * B1 * B1;
+    b[2] = A2 * A2 + 0.25 * B2 * B2;
+    u[0] = 0.375 * q[0] - 1.25 * q[1] + 1.875 * q[2];
+    u[1] = -0.125 * q[1] + 0.75 * q[2] + 0.375 * q[3];
+    u[2] = 0.375 * q[2] + 0.75 * q[3] - 0.125 * q[4];
+    const double d[3] = {0.0625, 0.625, 0.3125};
+    for (int k = 0; k < 3; k++) {
+      e[k] = 1.0 / (1e-6 + b[k]);
+      a[k] = d[k] * e[k] * e[k];
+    }
+    S = a[0] + a[1] + a[2];
+    out = (a[0] * u[0] + a[1] * u[1] + a[2] * u[2]) / S;
+  }
+  // adds lam * d(out)/dq to g
+  CFD_HD void bwd(const double* q, double lam, double* g) const {
+    const double iS = lam / S;
+    double lu[3], lb[3];
+    for (int k = 0; k < 3; k++) {
+      lu[k] = a[k] * iS;
+      lb[k] = (u[k] - out) * iS * (-2.0 * a[k] * e[k]);
+    }
+    // u
+    g[0] += 0.375 * lu[0];
+    g[1] += -1.25 * lu[0] - 0.125 * lu[1];
+    g[2] += 1.875 * lu[0] + 0.75 * lu[1] + 0.375 * lu[2];
+    g[3] += 0.375 * lu[1] + 0.75 * lu[2];
+    g[4] += -0.125 * lu[2];
+    // betas
+    double A0 = q[0] - 2.0 * q[1] + q[2], B0 = q[0] - 4.0 * q[1] + 3.0 * q[2];
+    double A1 = q[1] - 2.0 * q[2] + q[3], B1 = q[1] - q[3];
+    double A2 = q[2] - 2.0 * q[3] + q[4], B2 = 3.0 * q[2] - 4.0 * q[3] + q[4];
+    double s0A = 2.0 * A0 * lb[0], s0B = 0.5 * B0 * lb[0];
+    double s1A = 2.0 * A1 * lb[1], s1B = 0.5 * B1 * lb[1];
+    double s2A = 2.0 * A2 * lb[2], s2B = 0.5 * B2 * lb[2];
+    g[0] += s0A + s0B;
+    g[1] += -2.0 * s0A - 4.0 * s0B + s1A + s1B;
+    g[2] += s0A + 3.0 * s0B - 2.0 * s1A + s2A + 3.0 * s2B;
+    g[3] += s1A - s1B - 2.0 * s2A - 4.0 * s2B;
+    g[4] += s2A + s2B;
+  }
+};
+
+// ---- primitive <-> conservative ---------------------------------------------------
+CFD_HD inline void p2c(const double* p, double* c) {
+  c[0] = p[0];
+  c[1] = p[0] * p[1];
+  c[2] = p[0] * p[2];
+  c[3] = p[3] / G1 + 0.5 * p[0] * (p[1] * p[1] + p[2] * p[2]);
+}
+// g_p += (dc/dp)^T lc
+CFD_HD inline void p2c_vjp(const double* p, const double* lc, double* gp) {
+  double ek = 0.5 * (p[1] * p[1] + p[2] * p[2]);
+  gp[0] += lc[0] + lc[1] * p[1] + lc[2] * p[2] + lc[3] * ek;
+  gp[1] += lc[1] * p[0] + lc[3] * p[0] * p[1];
+  gp[2] += lc[2] * p[0] + lc[3] * p[0] * p[2];
+  gp[3] += lc[3] / G1;
+}
+CFD_HD inline void c2p(const double* c, double* p) {
+  double r = c[0], u = c[1] / r, v = c[2] / r;
+  p[0] = r;
+  p[1] = u;
+  p[2] = v;
+  p[3] = G1 * (c[3] - 0.5 * r * (u * u + v * v));
+}
+// g_c += (dp/dc)^T lp
+CFD_HD inline void c2p_vjp(const double* c, const double* lp, double* gc) {
+  double r = c[0], ir = 1.0 / r, u = c[1] * ir, v = c[2] * ir;
+  // p1 = c1/r, p2 = c2/r, p3 = G1 (c3 - (c1^2 + c2^2)/(2r))
+  gc[0] += lp[0] - lp[1] * u * ir - lp[2] * v * ir + lp[3] * G1 * 0.5 * (u * u + v * v);
+  gc[1] += lp[1] * ir - lp[3] * G1 * u;
+  gc[2] += lp[2] * ir - lp[3] * G1 * v;
+  gc[3] += lp[3] * G1;
+}
+
+// ---- node flux (fluxSolveEuler2D) -------------------------------------------------
+template <int AX> CFD_HD inline void node_flux(const double* p, double* F) {
+  double r = p[0], u = p[1], v = p[2], pp = p[3];
+  double Vn = AX == 0 ? u : v;
+  F[0] = r * Vn;
+  F[1] = r * u * Vn + (AX == 0 ? pp : 0.0);
+  F[2] = r * v * Vn + (AX == 1 ? pp : 0.0);
+  F[3] = (0.5 * (u * u + v * v) * r + GG1 * pp) * Vn;
+}
+template <int AX> CFD_HD inline void node_flux_vjp(const double* p, const double* lF, double* gp) {
+  double r = p[0], u = p[1], v = p[2], pp = p[3];
+  double Vn = AX == 0 ? u : v;
+  double E = 0.5 * (u * u + v * v) * r + GG1 * pp;
+  double lVn = lF[0] * r + lF[1] * r * u + lF[2] * r * v + lF[3] * E;
+  gp[0] += lF[0] * Vn + lF[1] * u * Vn + lF[2] * v * Vn + lF[3] * 0.5 * (u * u + v * v) * Vn;
+  gp[1] += lF[1] * r * Vn + lF[3] * r * u * Vn + (AX == 0 ? lVn : 0.0);
+  gp[2] += lF[2] * r * Vn + lF[3] * r * v * Vn + (AX == 1 ? lVn : 0.0);
+  gp[3] += (AX == 0 ? lF[1] : lF[2]) + lF[3] * GG1 * Vn;
+}
+
+// ---- HLLC (HLLCFlux2D2) forward and reverse --------------------------------------
+// One side's flux and star-region flux F + S (U* - U).
+template <int AX> struct HSide {
+  // inputs
+  double r, u, v, p;
+  // intermediates
+  double H, Vn;
+};
+
+template <int AX> CFD_HD inline void hllc(const double* L, const double* R, double* F, double* lL = nullptr,
+                                          double* lR = nullptr, const double* lF = nullptr) {
+  const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+  const double rl = L[0], ul = L[1], vl = L[2], pl = L[3];
+  const double rr = R[0], ur = R[1], vr = R[2], pr = R[3];
+  const double HL = 0.5 * (ul * ul + vl * vl) + GG1 * pl / rl;
+  const double HR = 0.5 * (ur * ur + vr * vr) + GG1 * pr / rr;
+  const double VnL = AX == 0 ? ul : vl, VnR = AX == 0 ? ur : vr;
+  const double cl = sqrt(G * pl / rl), cr = sqrt(G * pr / rr);
+  const double rBar = 0.5 * (rl + rr), cBar = 0.5 * (cl + cr);
+  const double psr = 0.5 * (pl + pr) - 0.5 * rBar * cBar * (VnR - VnL);
+  const bool clip = !(0.0 < psr);
+  const double ps = clip ? 0.0 : psr;
+  const double kap = (G + 1.0) / (2.0 * G);
+  const bool shl = !(ps <= pl), shr = !(ps <= pr);
+  const double ql = shl ? sqrt(1.0 + kap * (ps / pl - 1.0)) : 1.0;
+  const double qr = shr ? sqrt(1.0 + kap * (ps / pr - 1.0)) : 1.0;
+  const double SL = VnL - cl * ql, SR = VnR + cr * qr;
+  const double aL = SL - VnL, aR = SR - VnR;
+  const double num = (pr - pl) + (rl * VnL * aL - rr * VnR * aR);
+  const double den = rl * aL - rr * aR;
+  const double Ss = num / den;
+
+  // forward side / star fluxes; reverse accumulators for (r, u, v, p, S, Ss)
+  auto side = [&](double r, double u, double v, double p, double H, double Vn, double* out) {
+    out[0] = r * Vn;
+    out[1] = r * u * Vn + p * nx;
+    out[2] = r * v * Vn + p * ny;
+    out[3] = r * H * Vn;
+  };
+  // reverse of side(): lo = cotangent of out; adds to (lr, lu, lv, lp)
+  auto side_b = [&](double r, double u, double v, double p, double H, double Vn, const double* lo, double w,
+                    double& lr, double& lu, double& lv, double& lp) {
+    double l0 = w * lo[0], l1 = w * lo[1], l2 = w * lo[2], l3 = w * lo[3];
+    double lVn = l0 * r + l1 * r * u + l2 * r * v + l3 * r * H;
+    double lH = l3 * r * Vn;
+    lr += l0 * Vn + l1 * u * Vn + l2 * v * Vn + l3 * H * Vn;
+    lu += l1 * r * Vn;
+    lv += l2 * r * Vn;
+    lp += l1 * nx + l2 * ny;
+    // H = (u^2+v^2)/2 + GG1 p / r
+    lu += lH * u;
+    lv += lH * v;
+    lp += lH * GG1 / r;
+    lr += -lH * GG1 * p / (r * r);
+    if (AX == 0) lu += lVn;
+    else lv += lVn;
+  };
+  auto star = [&](double r, double u, double v, double p, double H, double Vn, double S, double* out) {
+    double Fs[4];
+    side(r, u, v, p, H, Vn, Fs);
+    double Usf = (S - Vn) / (S - Ss);
+    double U3 = r * H - p;
+    double Us0 = Usf * r;
+    double Us1 = Usf * (Ss * nx + u * ny) * r;
+    double Us2 = Usf * (Ss * ny + v * nx) * r;
+    double Us3 = Usf * (U3 + (Ss - Vn) * (r * Ss + p / (S - Vn)));
+    out[0] = Fs[0] + S * (Us0 - r);
+    out[1] = Fs[1] + S * (Us1 - r * u);
+    out[2] = Fs[2] + S * (Us2 - r * v);
+    out[3] = Fs[3] + S * (Us3 - U3);
+  };
+  // reverse of star(): adds to state cotangents and to lS, lSs
+  auto star_b = [&](double r, double u, double v, double p, double H, double Vn, double S, const double* lo, double w,
+                    double& lr, double& lu, double& lv, double& lp, double& lS, double& lSs) {
+    double l0 = w * lo[0], l1 = w * lo[1], l2 = w * lo[2], l3 = w * lo[3];
+    double D = S - Ss, A = S - Vn;
+    double Usf = A / D;
+    double U3 = r * H - p;
+    double Pt = r * Ss + p / A;
+    double Us0 = Usf * r, Us1 = Usf * (Ss * nx + u * ny) * r, Us2 = Usf * (Ss * ny + v * nx) * r;
+    double Us3 = Usf * (U3 + (Ss - Vn) * Pt);
+    // out = Fs + S (Us - U)
+    double lo4[4] = {l0, l1, l2, l3};
+    side_b(r, u, v, p, H, Vn, lo4, 1.0, lr, lu, lv, lp);
+    lS += l0 * (Us0 - r) + l1 * (Us1 - r * u) + l2 * (Us2 - r * v) + l3 * (Us3 - U3);
+    double lUs0 = l0 * S, lUs1 = l1 * S, lUs2 = l2 * S, lUs3 = l3 * S;
+    // -S U terms
+    lr += -l0 * S - l1 * S * u - l2 * S * v;
+    lu += -l1 * S * r;
+    lv += -l2 * S * r;
+    double lU3 = -l3 * S;
+    // Us0 = Usf r
+    double lUsf = lUs0 * r;
+    lr += lUs0 * Usf;
+    // Us1 = Usf (Ss nx + u ny) r
+    double t1 = Ss * nx + u * ny;
+    lUsf += lUs1 * t1 * r;
+    lr += lUs1 * Usf * t1;
+    lSs += lUs1 * Usf * r * nx;
+    lu += lUs1 * Usf * r * ny;
+    // Us2 = Usf (Ss ny + v nx) r
+    double t2 = Ss * ny + v * nx;
+    lUsf += lUs2 * t2 * r;
+    lr += lUs2 * Usf * t2;
+    lSs += lUs2 * Usf * r * ny;
+    lv += lUs2 * Usf * r * nx;
+    // Us3 = Usf (U3 + (Ss - Vn) Pt)
+    double inner = U3 + (Ss - Vn) * Pt;
+    lUsf += lUs3 * inner;
+    double lin = lUs3 * Usf;
+    lU3 += lin;
+    lSs += lin * Pt;
+    double lVn = -lin * Pt;
+    double lPt = lin * (Ss - Vn);
+    // Pt = r Ss + p / A
+    lr += lPt * Ss;
+    lSs += lPt * r;
+    lp += lPt / A;
+    double lA = -lPt * p / (A * A);
+    // Usf = A / D
+    lA += lUsf / D;
+    double lD = -lUsf * A / (D * D);
+    // A = S - Vn ; D = S - Ss
+    lS += lA + lD;
+    lVn += -lA;
+    lSs += -lD;
+    // U3 = r H - p
+    lr += lU3 * H;
+    double lH = lU3 * r;
+    lp += -lU3;
+    // H and Vn
+    lu += lH * u;
+    lv += lH * v;
+    lp += lH * GG1 / r;
+    lr += -lH * GG1 * p / (r * r);
+    if (AX == 0) lu += lVn;
+    else lv += lVn;
+  };
+
+  int br;
+  if (SL > 0) br = 0;
+  else if (Ss > 0) br = 1;
+  else if (Ss == 0) br = 2;
+  else if (SR > 0) br = 3;
+  else br = 4;
+  if (!lF) {
+    if (br == 0) side(rl, ul, vl, pl, HL, VnL, F);
+    else if (br == 1) star(rl, ul, vl, pl, HL, VnL, SL, F);
+    else if (br == 2) {
+      double a[4], b[4];
+      star(rl, ul, vl, pl, HL, VnL, SL, a);
+      star(rr, ur, vr, pr, HR, VnR, SR, b);
+      for (int i = 0; i < 4; i++) F[i] = 0.5 * (a[i] + b[i]);
+    } else if (br == 3) star(rr, ur, vr, pr, HR, VnR, SR, F);
+    else side(rr, ur, vr, pr, HR, VnR, F);
+    return;
+  }
+  // ---- reverse ----
+  double lrl = 0, lul = 0, lvl = 0, lpl = 0, lrr = 0, lur = 0, lvr = 0, lpr = 0;
+  double lSL = 0, lSR = 0, lSs = 0;
+  if (br == 0) side_b(rl, ul, vl, pl, HL, VnL, lF, 1.0, lrl, lul, lvl, lpl);
+  else if (br == 4) side_b(rr, ur, vr, pr, HR, VnR, lF, 1.0, lrr, lur, lvr, lpr);
+  else {
+    const double wl = br == 2 ? 0.5 : (br == 1 ? 1.0 : 0.0);
+    const double wr = br == 2 ? 0.5 : (br == 3 ? 1.0 : 0.0);
+    if (wl != 0.0) star_b(rl, ul, vl, pl, HL, VnL, SL, lF, wl, lrl, lul, lvl, lpl, lSL, lSs);
+    if (wr != 0.0) star_b(rr, ur, vr, pr, HR, VnR, SR, lF, wr, lrr, lur, lvr, lpr, lSR, lSs);
+  }
+  // Ss = num / den
+  double lnum = lSs / den, lden = -lSs * Ss / den;
+  // num = (pr - pl) + rl VnL aL - rr VnR aR ; den = rl aL - rr aR
+  lpr += lnum;
+  lpl += -lnum;
+  double laL = lnum * rl * VnL + lden * rl;
+  double laR = -lnum * rr * VnR - lden * rr;
+  lrl += lnum * VnL * aL + lden * aL;
+  lrr += -lnum * VnR * aR - lden * aR;
+  double lVnL = lnum * rl * aL, lVnR = -lnum * rr * aR;
+  // aL = SL - VnL ; aR = SR - VnR
+  lSL += laL;
+  lVnL += -laL;
+  lSR += laR;
+  lVnR += -laR;
+  // SL = VnL - cl ql ; SR = VnR + cr qr
+  lVnL += lSL;
+  double lcl = -lSL * ql, lql = -lSL * cl;
+  lVnR += lSR;
+  double lcr = lSR * qr, lqr = lSR * cr;
+  // ql = sqrt(1 + kap (ps/pl - 1)) when shocked
+  double lps = 0;
+  if (shl) {
+    double g = lql * 0.5 / ql * kap;
+    lps += g / pl;
+    lpl += -g * ps / (pl * pl);
+  }
+  if (shr) {
+    double g = lqr * 0.5 / qr * kap;
+    lps += g / pr;
+    lpr += -g * ps / (pr * pr);
+  }
+  // ps = max(0, psr); psr = (pl + pr)/2 - rBar cBar (VnR - VnL)/2
+  if (!clip) {
+    lpl += 0.5 * lps;
+    lpr += 0.5 * lps;
+    double dv = VnR - VnL;
+    double lrBar = -0.5 * lps * cBar * dv, lcBar = -0.5 * lps * rBar * dv;
+    lVnR += -0.5 * lps * rBar * cBar;
+    lVnL += 0.5 * lps * rBar * cBar;
+    lrl += 0.5 * lrBar;
+    lrr += 0.5 * lrBar;
+    lcl += 0.5 * lcBar;
+    lcr += 0.5 * lcBar;
+  }
+  // cl = sqrt(G pl / rl)
+  {
+    double g = lcl * 0.5 / cl * G;
+    lpl += g / rl;
+    lrl += -g * pl / (rl * rl);
+    double h = lcr * 0.5 / cr * G;
+    lpr += h / rr;
+    lrr += -h * pr / (rr * rr);
+  }
+  if (AX == 0) {
+    lul += lVnL;
+    lur += lVnR;
+  } else {
+    lvl += lVnL;
+    lvr += lVnR;
+  }
+  lL[0] += lrl;
+  lL[1] += lul;
+  lL[2] += lvl;
+  lL[3] += lpl;
+  lR[0] += lrr;
+  lR[1] += lur;
+  lR[2] += lvr;
+  lR[3] += lpr;
+}
+
+// ---- Roe-averaged characteristic system of eigensystemEuler2D ------------------
+template <int AX> struct Roe2 {
+  double c1, c2, D, HL, HR;
+  double u, v, ht, Vn, ek, h, c;
+  CFD_HD void fwd(const double* pl, const double* pr) {
+    HL = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2]) + GG1 * pl[3] / pl[0];
+    HR = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]) + GG1 * pr[3] / pr[0];
+    c1 = sqrt(pl[0]);
+    c2 = sqrt(pr[0]);
+    D = c1 + c2;
+    u = (c1 * pl[1] + c2 * pr[1]) / D;
+    v = (c1 * pl[2] + c2 * pr[2]) / D;
+    ht = (c1 * HL + c2 * HR) / D;
+    Vn = AX == 0 ? u : v;
+    ek = 0.5 * (u * u + v * v);
+    h = ht - ek;
+    c = sqrt(G1 * h);
+  }
+  // left eigenvector matrix (rows = waves) and right (R[i][k])
+  CFD_HD void mats(double L[4][4], double R[4][4]) const {
+    const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+    L[0][0] = -ny * u + nx * v; L[0][1] = ny; L[0][2] = -nx; L[0][3] = 0.0;
+    L[1][0] = h - ek; L[1][1] = u; L[1][2] = v; L[1][3] = -1.0;
+    L[2][0] = 0.5 * (Vn / c + ek / h); L[2][1] = 0.5 * (-nx / c - u / h); L[2][2] = 0.5 * (-ny / c - v / h); L[2][3] = 0.5 / h;
+    L[3][0] = 0.5 * (-Vn / c + ek / h); L[3][1] = 0.5 * (nx / c - u / h); L[3][2] = 0.5 * (ny / c - v / h); L[3][3] = 0.5 / h;
+    R[0][0] = 0.0; R[0][1] = 1.0 / h; R[0][2] = 1.0; R[0][3] = 1.0;
+    R[1][0] = ny; R[1][1] = u / h; R[1][2] = u - nx * c; R[1][3] = u + nx * c;
+    R[2][0] = -nx; R[2][1] = v / h; R[2][2] = v - ny * c; R[2][3] = v + ny * c;
+    R[3][0] = ny * u - nx * v; R[3][1] = ek / h; R[3][2] = h + ek - Vn * c; R[3][3] = h + ek + Vn * c;
+  }
+  // cotangents of L and R -> cotangents of pl, pr (added)
+  CFD_HD void bwd(const double* pl, const double* pr, const double gL[4][4], const double gR[4][4], double* lpl,
+                  double* lpr) const {
+    const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+    double lu = 0, lv = 0, lh = 0, lc = 0, lek = 0, lVn = 0;
+    const double ic = 1.0 / c, ih = 1.0 / h, ic2 = ic * ic, ih2 = ih * ih;
+    // L rows
+    lu += -ny * gL[0][0];
+    lv += nx * gL[0][0];
+    lh += gL[1][0];
+    lek += -gL[1][0];
+    lu += gL[1][1];
+    lv += gL[1][2];
+    // row 2
+    lVn += gL[2][0] * 0.5 * ic;
+    lc += -gL[2][0] * 0.5 * Vn * ic2;
+    lek += gL[2][0] * 0.5 * ih;
+    lh += -gL[2][0] * 0.5 * ek * ih2;
+    lc += gL[2][1] * 0.5 * nx * ic2;
+    lu += -gL[2][1] * 0.5 * ih;
+    lh += gL[2][1] * 0.5 * u * ih2;
+    lc += gL[2][2] * 0.5 * ny * ic2;
+    lv += -gL[2][2] * 0.5 * ih;
+    lh += gL[2][2] * 0.5 * v * ih2;
+    lh += -gL[2][3] * 0.5 * ih2;
+    // row 3
+    lVn += -gL[3][0] * 0.5 * ic;
+    lc += gL[3][0] * 0.5 * Vn * ic2;
+    lek += gL[3][0] * 0.5 * ih;
+    lh += -gL[3][0] * 0.5 * ek * ih2;
+    lc += -gL[3][1] * 0.5 * nx * ic2;
+    lu += -gL[3][1] * 0.5 * ih;
+    lh += gL[3][1] * 0.5 * u * ih2;
+    lc += -gL[3][2] * 0.5 * ny * ic2;
+    lv += -gL[3][2] * 0.5 * ih;
+    lh += gL[3][2] * 0.5 * v * ih2;
+    lh += -gL[3][3] * 0.5 * ih2;
+    // R
+    lu += ny * gR[3][0];
+    lv += -nx * gR[3][0];
+    lh += -(gR[0][1] + gR[1][1] * u + gR[2][1] * v + gR[3][1] * ek) * ih2;
+    lu += gR[1][1] * ih;
+    lv += gR[2][1] * ih;
+    lek += gR[3][1] * ih;
+    lu += gR[1][2] + gR[1][3];
+    lv += gR[2][2] + gR[2][3];
+    lc += -nx * gR[1][2] + nx * gR[1][3] - ny * gR[2][2] + ny * gR[2][3];
+    lh += gR[3][2] + gR[3][3];
+    lek += gR[3][2] + gR[3][3];
+    lVn += -c * gR[3][2] + c * gR[3][3];
+    lc += -Vn * gR[3][2] + Vn * gR[3][3];
+    // c = sqrt(G1 h); h = ht - ek; ek = (u^2+v^2)/2; Vn = u | v
+    lh += lc * 0.5 * G1 / c;
+    double lht = lh;
+    lek += -lh;
+    lu += lek * u;
+    lv += lek * v;
+    if (AX == 0) lu += lVn;
+    else lv += lVn;
+    // u = (c1 ul + c2 ur)/D etc.
+    const double iD = 1.0 / D;
+    double lc1 = (lu * (pl[1] - u) + lv * (pl[2] - v) + lht * (HL - ht)) * iD;
+    double lc2 = (lu * (pr[1] - u) + lv * (pr[2] - v) + lht * (HR - ht)) * iD;
+    lpl[1] += lu * c1 * iD;
+    lpr[1] += lu * c2 * iD;
+    lpl[2] += lv * c1 * iD;
+    lpr[2] += lv * c2 * iD;
+    double lHL = lht * c1 * iD, lHR = lht * c2 * iD;
+    lpl[0] += lc1 * 0.5 / c1;
+    lpr[0] += lc2 * 0.5 / c2;
+    // HL = (ul^2+vl^2)/2 + GG1 pl/rl
+    lpl[1] += lHL * pl[1];
+    lpl[2] += lHL * pl[2];
+    lpl[3] += lHL * GG1 / pl[0];
+    lpl[0] += -lHL * GG1 * pl[3] / (pl[0] * pl[0]);
+    lpr[1] += lHR * pr[1];
+    lpr[2] += lHR * pr[2];
+    lpr[3] += lHR * GG1 / pr[0];
+    lpr[0] += -lHR * GG1 * pr[3] / (pr[0] * pr[0]);
+  }
+};
+
+// ---- interface flux: cells w[0..5] (primitive), interface between w[2], w[3] ------
+// Forward (for checks) and reverse: lw[j][v] += (dF/dw[j][v])^T lF.
+template <int AX> CFD_HD inline void interface(const double w[6][4], double* F, double lw[6][4] = nullptr,
+                                               const double* lF = nullptr) {
+  Roe2<AX> es;
+  es.fwd(w[2], w[3]);
+  double L[4][4], R[4][4];
+  es.mats(L, R);
+  double c[6][4];
+  for (int j = 0; j < 6; j++) p2c(w[j], c[j]);
+  double ch[4][6];
+  for (int k = 0; k < 4; k++)
+    for (int j = 0; j < 6; j++) ch[k][j] = L[k][0] * c[j][0] + L[k][1] * c[j][1] + L[k][2] * c[j][2] + L[k][3] * c[j][3];
+  JS jl[4], jr[4];
+  double QL[4], QR[4];
+  for (int k = 0; k < 4; k++) {
+    double ql[5] = {ch[k][0], ch[k][1], ch[k][2], ch[k][3], ch[k][4]};
+    double qr[5] = {ch[k][5], ch[k][4], ch[k][3], ch[k][2], ch[k][1]};
+    jl[k].fwd(ql);
+    jr[k].fwd(qr);
+    QL[k] = jl[k].out;
+    QR[k] = jr[k].out;
+  }
+  double cL[4], cR[4];
+  for (int i = 0; i < 4; i++) {
+    cL[i] = R[i][0] * QL[0] + R[i][1] * QL[1] + R[i][2] * QL[2] + R[i][3] * QL[3];
+    cR[i] = R[i][0] * QR[0] + R[i][1] * QR[1] + R[i][2] * QR[2] + R[i][3] * QR[3];
+  }
+  double oL[4], oR[4];
+  c2p(cL, oL);
+  c2p(cR, oR);
+  const bool fbL = oL[0] < 0 || oL[3] < 0, fbR = oR[0] < 0 || oR[3] < 0;
+  if (fbL)
+    for (int v = 0; v < 4; v++) oL[v] = w[2][v];
+  if (fbR) {
+    oR[0] = w[3][0];
+    oR[1] = w[2][1];
+    oR[2] = w[2][2];
+    oR[3] = w[3][3];
+  }
+  if (!lF) {
+    hllc<AX>(oL, oR, F);
+    return;
+  }
+  double loL[4] = {0, 0, 0, 0}, loR[4] = {0, 0, 0, 0};
+  double Fd[4];
+  hllc<AX>(oL, oR, Fd, loL, loR, lF);
+  double lcL[4] = {0, 0, 0, 0}, lcR[4] = {0, 0, 0, 0};
+  if (fbL) {
+    for (int v = 0; v < 4; v++) lw[2][v] += loL[v];
+  } else {
+    c2p_vjp(cL, loL, lcL);
+  }
+  if (fbR) {
+    lw[3][0] += loR[0];
+    lw[2][1] += loR[1];
+    lw[2][2] += loR[2];
+    lw[3][3] += loR[3];
+  } else {
+    c2p_vjp(cR, loR, lcR);
+  }
+  double gR[4][4], gL[4][4];
+  double lQL[4], lQR[4];
+  for (int k = 0; k < 4; k++) {
+    lQL[k] = R[0][k] * lcL[0] + R[1][k] * lcL[1] + R[2][k] * lcL[2] + R[3][k] * lcL[3];
+    lQR[k] = R[0][k] * lcR[0] + R[1][k] * lcR[1] + R[2][k] * lcR[2] + R[3][k] * lcR[3];
+    for (int i = 0; i < 4; i++) gR[i][k] = QL[k] * lcL[i] + QR[k] * lcR[i];
+  }
+  double lch[4][6];
+  for (int k = 0; k < 4; k++) {
+    for (int j = 0; j < 6; j++) lch[k][j] = 0.0;
+    double ql[5] = {ch[k][0], ch[k][1], ch[k][2], ch[k][3], ch[k][4]};
+    double qr[5] = {ch[k][5], ch[k][4], ch[k][3], ch[k][2], ch[k][1]};
+    double gl[5] = {0, 0, 0, 0, 0}, gr[5] = {0, 0, 0, 0, 0};
+    jl[k].bwd(ql, lQL[k], gl);
+    jr[k].bwd(qr, lQR[k], gr);
+    for (int j = 0; j < 5; j++) lch[k][j] += gl[j];
+    for (int j = 0; j < 5; j++) lch[k][5 - j] += gr[j];
+  }
+  double lc[6][4];
+  for (int j = 0; j < 6; j++)
+    for (int v = 0; v < 4; v++) lc[j][v] = lch[0][j] * L[0][v] + lch[1][j] * L[1][v] + lch[2][j] * L[2][v] + lch[3][j] * L[3][v];
+  for (int k = 0; k < 4; k++)
+    for (int v = 0; v < 4; v++) {
+      double s = 0;
+      for (int j = 0; j < 6; j++) s += lch[k][j] * c[j][v];
+      gL[k][v] = s;
+    }
+  es.bwd(w[2], w[3], gL, gR, lw[2], lw[3]);
+  for (int j = 0; j < 6; j++) p2c_vjp(w[j], lc[j], lw[j]);
+}
+
+}  // namespace cfdadj

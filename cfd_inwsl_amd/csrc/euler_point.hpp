@@ -21,6 +21,10 @@ enum SchemeId { SCH_JS = 0, SCH_Z = 1, SCH_TENO = 2, SCH_TENOS = 3 };
 enum FluxId { FLX_HLLC = 0, FLX_ROE = 1 };
 
 template <class T> __device__ __forceinline__ T sq(T a) { return a * a; }
+// The reference's 1e-40 regulariser of WENO-Z / TENO is below the FP32
+// normal range; the FP32 variant uses 1e-36 (a normal float) instead.
+template <class T> __device__ __forceinline__ T eps_tiny() { return eps_tiny<T>(); }
+template <> __device__ __forceinline__ float eps_tiny<float>() { return 1e-36f; }
 // std::min / std::max argument semantics
 template <class T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
 template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
@@ -65,9 +69,9 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
       a2 = T(0.3125) / sq(T(1e-6) + b2);
     } else {
       T tau = fabs(b2 - b0);
-      a0 = T(0.0625) * (T(1.0) + sq(tau / (b0 + T(1e-40))));
-      a1 = T(0.625) * (T(1.0) + sq(tau / (b1 + T(1e-40))));
-      a2 = T(0.3125) * (T(1.0) + sq(tau / (b2 + T(1e-40))));
+      a0 = T(0.0625) * (T(1.0) + sq(tau / (b0 + eps_tiny<T>())));
+      a1 = T(0.625) * (T(1.0) + sq(tau / (b1 + eps_tiny<T>())));
+      a2 = T(0.3125) * (T(1.0) + sq(tau / (b2 + eps_tiny<T>())));
     }
     T s = a0 + a1 + a2;
     T r = a0 * u0 + a1 * u1 + a2 * u2;
@@ -75,9 +79,9 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
   } else if (SCH == SCH_TENO) {
     // Teno5_Z (interScheme.hpp:384-452); ct = 1e-5 (CT4: 1e-4, CT7: 1e-7)
     T tau = fabs(b2 - b0);
-    T t0 = T(1.0) + tau / (b0 + T(1e-40));
-    T t1 = T(1.0) + tau / (b1 + T(1e-40));
-    T t2 = T(1.0) + tau / (b2 + T(1e-40));
+    T t0 = T(1.0) + tau / (b0 + eps_tiny<T>());
+    T t1 = T(1.0) + tau / (b1 + eps_tiny<T>());
+    T t2 = T(1.0) + tau / (b2 + eps_tiny<T>());
     t0 = t0 * t0;
     t1 = t1 * t1;
     t2 = t2 * t2;
