@@ -15,41 +15,22 @@ using namespace cfd;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+int cfd_fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
 // error slot shared with the batch entry points (batch.hip)
-int cfd_set_error_from_batch(int code, const char* msg) { return fail(code, msg); }
+int cfd_set_error_from_batch(int code, const char* msg) { return cfd_fail(code, msg); }
 
 #define HIPCHK(expr)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \
     if (e_ != hipSuccess)                                                              \
-      return fail(CFD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+      return cfd_fail(CFD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
   } while (0)
 
-struct cfd_handle {
-  cfd_config cfg;
-  int D, NV;
-  long N;
-  int sch, flx;
-  bool cw;
-  double ct;
-  double* U0;            // FP64 state at the start of the step
-  void *Ua, *Ub, *R;     // stage states and rhs: element type of the sweeps
-  float* U0f;            // FP32 shadow of U0 (FP32 precision only)
-  bool fp32;
-  size_t esz;            // bytes per element of Ua / Ub / R
-  double* scal;          // [0] dt, [1..3] max wave speed per direction (bits), [4] t
-  double* partial;       // block partial sums for totals
-  int npartial;
-  int keep_rhs;
-  int tile[3];     // tile shape per direction
-  int seg[3];      // segment length per direction
-  hipStream_t stream;
-};
+#include "handle.hpp"
 
 // ---- scheme selection ---------------------------------------------------------
 // SolverType::initReconer for EULER (zone_FD/src/src/solverType.cpp:217-594).
@@ -207,8 +188,8 @@ __global__ __launch_bounds__(256) void k_totals(const double* __restrict__ U, lo
 }
 
 // ---- helpers -----------------------------------------------------------------------
-static hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int stage, const double* U0,
-                               void* Uout, int tile = -1, int seg = -1, void* Ushadow = nullptr) {
+hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int stage, const double* U0,
+                               void* Uout, int tile, int seg, void* Ushadow) {
   SweepArgs a;
   memset(&a, 0, sizeof(a));
   a.U = U;
@@ -239,25 +220,25 @@ static hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, 
 }
 
 // One RK stage: sweeps over every direction; the last one applies the stage update.
-static int rk_stage(cfd_handle* h, int stage, const void* Ucur, void* Uout, void* Ushadow) {
+int rk_stage(cfd_handle* h, int stage, const void* Ucur, const double* U0, void* Uout, void* Ushadow) {
   for (int ax = 0; ax < h->D; ax++) {
     int flags = 0;
     if (ax > 0) flags |= SW_READ_R;
     if (ax < h->D - 1 || h->keep_rhs) flags |= SW_WRITE_R;
     if (ax == h->D - 1) flags |= SW_RK;
-    HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, h->U0, Uout, -1, -1, Ushadow));
+    HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, U0, Uout, -1, -1, Ushadow));
   }
   return 0;
 }
 
 // state read by the sweeps at the start of a step
-static const void* sweep_state(cfd_handle* h) { return h->fp32 ? (const void*)h->U0f : (const void*)h->U0; }
+const void* sweep_state(cfd_handle* h) { return h->fp32 ? (const void*)h->U0f : (const void*)h->U0; }
 
 static int rk3(cfd_handle* h) {
   int rc;
-  if ((rc = rk_stage(h, 1, sweep_state(h), h->Ua, nullptr))) return rc;
-  if ((rc = rk_stage(h, 2, h->Ua, h->Ub, nullptr))) return rc;
-  if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->fp32 ? (void*)h->U0f : nullptr))) return rc;
+  if ((rc = rk_stage(h, 1, sweep_state(h), h->U0, h->Ua, nullptr))) return rc;
+  if ((rc = rk_stage(h, 2, h->Ua, h->U0, h->Ub, nullptr))) return rc;
+  if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->U0, h->fp32 ? (void*)h->U0f : nullptr))) return rc;
   return 0;
 }
 
@@ -290,29 +271,29 @@ const char* cfd_last_error(void) { return g_err.c_str(); }
 const char* cfd_version(void) { return "congfd-mi355x 0.1"; }
 
 int cfd_create(const cfd_config* cfg, cfd_handle** out) {
-  if (!cfg || !out) return fail(CFD_E_ARG, "null argument");
+  if (!cfg || !out) return cfd_fail(CFD_E_ARG, "null argument");
   *out = nullptr;
-  if (cfg->dim < 1 || cfg->dim > 3) return fail(CFD_E_ARG, "dim must be 1, 2 or 3");
-  if (cfg->diff_method != CFD_DIFF_MND6) return fail(CFD_E_UNSUPP, "only DiffMethod MND6 (3) is implemented");
+  if (cfg->dim < 1 || cfg->dim > 3) return cfd_fail(CFD_E_ARG, "dim must be 1, 2 or 3");
+  if (cfg->diff_method != CFD_DIFF_MND6) return cfd_fail(CFD_E_UNSUPP, "only DiffMethod MND6 (3) is implemented");
   if (cfg->flux_method != CFD_FLUX_HLLC && cfg->flux_method != CFD_FLUX_ROE)
-    return fail(CFD_E_ARG, "flux_method must be HLLC (0) or ROE (1)");
+    return cfd_fail(CFD_E_ARG, "flux_method must be HLLC (0) or ROE (1)");
   if (cfg->recon != CFD_RECON_CHAR && cfg->recon != CFD_RECON_COMPONENT)
-    return fail(CFD_E_ARG, "recon must be CFD_RECON_CHAR or CFD_RECON_COMPONENT");
+    return cfd_fail(CFD_E_ARG, "recon must be CFD_RECON_CHAR or CFD_RECON_COMPONENT");
   int sch;
   double ct;
   if (cfd_select_scheme(cfg->inter_method, &sch, &ct))
-    return fail(CFD_E_UNSUPP, "inter_method " + std::to_string(cfg->inter_method) + " is not implemented");
+    return cfd_fail(CFD_E_UNSUPP, "inter_method " + std::to_string(cfg->inter_method) + " is not implemented");
   for (int d = 0; d < 3; d++) {
     int want_active = d < cfg->dim;
-    if (cfg->n[d] < 1) return fail(CFD_E_ARG, "n[] must be >= 1");
-    if (!want_active && cfg->n[d] != 1) return fail(CFD_E_ARG, "n[d] must be 1 beyond dim");
+    if (cfg->n[d] < 1) return cfd_fail(CFD_E_ARG, "n[] must be >= 1");
+    if (!want_active && cfg->n[d] != 1) return cfd_fail(CFD_E_ARG, "n[d] must be 1 beyond dim");
     if (want_active) {
-      if (!(cfg->h[d] > 0)) return fail(CFD_E_ARG, "h[d] must be > 0");
+      if (!(cfg->h[d] > 0)) return cfd_fail(CFD_E_ARG, "h[d] must be > 0");
       for (int s = 0; s < 2; s++) {
         int b = cfg->bc[2 * d + s];
-        if (!bc_known(b)) return fail(CFD_E_ARG, "unknown boundary code " + std::to_string(b));
+        if (!bc_known(b)) return cfd_fail(CFD_E_ARG, "unknown boundary code " + std::to_string(b));
         if (b != BC_PERIODIC && b != BC_OUTFLOW && b != BC_DIRICHLET && cfg->n[d] < 4)
-          return fail(CFD_E_ARG, "mirror boundaries need at least 4 cells along the axis");
+          return cfd_fail(CFD_E_ARG, "mirror boundaries need at least 4 cells along the axis");
       }
     }
   }
@@ -327,12 +308,12 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   h->cw = cfg->recon == CFD_RECON_CHAR;
   if (cfg->precision != CFD_PRECISION_FP64 && cfg->precision != CFD_PRECISION_FP32) {
     delete h;
-    return fail(CFD_E_ARG, "precision must be CFD_PRECISION_FP64 or CFD_PRECISION_FP32");
+    return cfd_fail(CFD_E_ARG, "precision must be CFD_PRECISION_FP64 or CFD_PRECISION_FP32");
   }
   h->fp32 = cfg->precision == CFD_PRECISION_FP32;
   if (h->fp32 && cfg->dim == 1) {
     delete h;
-    return fail(CFD_E_UNSUPP, "FP32 precision is available for dim 2 and 3");
+    return cfd_fail(CFD_E_UNSUPP, "FP32 precision is available for dim 2 and 3");
   }
   h->esz = h->fp32 ? sizeof(float) : sizeof(double);
   for (int ax = 0; ax < 3; ax++) {
@@ -341,7 +322,7 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
     if (ax < cfg->dim) plan_default(cfg->n, ax, &h->tile[ax], &h->seg[ax]);
   }
   hipError_t e = hipSetDevice(cfg->device);
-  if (e != hipSuccess) { delete h; return fail(CFD_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
+  if (e != hipSuccess) { delete h; return cfd_fail(CFD_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
   size_t bytes = sizeof(double) * h->N * h->NV;
   size_t ebytes = h->esz * h->N * h->NV;
   h->npartial = 1024;
@@ -359,7 +340,7 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   if (e != hipSuccess) {
     std::string m = std::string("allocation: ") + hipGetErrorString(e);
     cfd_destroy(h);
-    return fail(CFD_E_HIP, m);
+    return cfd_fail(CFD_E_HIP, m);
   }
   *out = h;
   return CFD_OK;
@@ -373,6 +354,7 @@ int cfd_destroy(cfd_handle* h) {
   (void)hipFree(h->Ub);
   (void)hipFree(h->R);
   if (h->U0f) (void)hipFree(h->U0f);
+  cfd_adjoint_release(h);
   (void)hipFree(h->scal);
   (void)hipFree(h->partial);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -381,7 +363,7 @@ int cfd_destroy(cfd_handle* h) {
 }
 
 int cfd_set_state(cfd_handle* h, const double* cons) {
-  if (!h || !cons) return fail(CFD_E_ARG, "null argument");
+  if (!h || !cons) return cfd_fail(CFD_E_ARG, "null argument");
   size_t bytes = sizeof(double) * h->N * h->NV;
   HIPCHK(hipMemcpyAsync(h->Ub, cons, bytes, hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream,
@@ -395,7 +377,7 @@ int cfd_set_state(cfd_handle* h, const double* cons) {
 }
 
 int cfd_get_state(cfd_handle* h, double* cons) {
-  if (!h || !cons) return fail(CFD_E_ARG, "null argument");
+  if (!h || !cons) return cfd_fail(CFD_E_ARG, "null argument");
   size_t bytes = sizeof(double) * h->N * h->NV;
   hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->stream, h->U0,
                      static_cast<double*>(h->Ub), h->N, h->NV);
@@ -406,10 +388,10 @@ int cfd_get_state(cfd_handle* h, double* cons) {
 }
 
 int cfd_eval_rhs(cfd_handle* h, double* rhs) {
-  if (!h) return fail(CFD_E_ARG, "null handle");
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
   for (int ax = 0; ax < h->D; ax++) {
     int flags = SW_WRITE_R | (ax > 0 ? SW_READ_R : 0);
-    HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 0, nullptr, nullptr));
+    HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 0, nullptr, nullptr, -1, -1, nullptr));
   }
   if (rhs) {
     size_t bytes = sizeof(double) * h->N * h->NV;
@@ -427,15 +409,15 @@ int cfd_eval_rhs(cfd_handle* h, double* rhs) {
 }
 
 int cfd_rk3_step(cfd_handle* h, double dt) {
-  if (!h) return fail(CFD_E_ARG, "null handle");
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal, dt);
   HIPCHK(hipGetLastError());
   return rk3(h);
 }
 
 int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out) {
-  if (!h) return fail(CFD_E_ARG, "null handle");
-  if (!(cfl > 0)) return fail(CFD_E_ARG, "cfl must be > 0");
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
+  if (!(cfl > 0)) return cfd_fail(CFD_E_ARG, "cfl must be > 0");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 4, 0.0);
   for (int s = 0; s < nsteps; s++) {
     int rc = wave_speed(h, cfl);
@@ -450,13 +432,13 @@ int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out) {
 }
 
 int cfd_set_keep_rhs(cfd_handle* h, int keep) {
-  if (!h) return fail(CFD_E_ARG, "null handle");
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
   h->keep_rhs = keep != 0;
   return CFD_OK;
 }
 
 int cfd_rhs_linf(cfd_handle* h, int var, double* out) {
-  if (!h || !out || var < 0 || var >= h->NV) return fail(CFD_E_ARG, "bad argument");
+  if (!h || !out || var < 0 || var >= h->NV) return cfd_fail(CFD_E_ARG, "bad argument");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 5, 0.0);
   int blocks = (int)std::min<long>((h->N + 255) / 256, 2048);
   if (h->fp32)
@@ -472,7 +454,7 @@ int cfd_rhs_linf(cfd_handle* h, int var, double* out) {
 }
 
 int cfd_cfl_dt(cfd_handle* h, double cfl, double* dt) {
-  if (!h || !dt) return fail(CFD_E_ARG, "null argument");
+  if (!h || !dt) return cfd_fail(CFD_E_ARG, "null argument");
   int rc = wave_speed(h, cfl);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(dt, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -481,7 +463,7 @@ int cfd_cfl_dt(cfd_handle* h, double cfl, double* dt) {
 }
 
 int cfd_conserved_totals(cfd_handle* h, double* out) {
-  if (!h || !out) return fail(CFD_E_ARG, "null argument");
+  if (!h || !out) return cfd_fail(CFD_E_ARG, "null argument");
   int blocks = (int)std::min<long>((h->N + 255) / 256, h->npartial);
   hipLaunchKernelGGL(k_totals, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, h->NV, h->partial);
   HIPCHK(hipGetLastError());
@@ -497,7 +479,7 @@ int cfd_conserved_totals(cfd_handle* h, double* out) {
 }
 
 int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out) {
-  if (!h || !ms_out || reps < 1) return fail(CFD_E_ARG, "bad argument");
+  if (!h || !ms_out || reps < 1) return cfd_fail(CFD_E_ARG, "bad argument");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal, 0.0);
   HIPCHK(hipGetLastError());
   std::vector<hipEvent_t> ev(2 * h->D);
@@ -510,7 +492,7 @@ int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out) {
       if (ax < h->D - 1) flags |= SW_WRITE_R;
       else flags |= SW_RK;
       HIPCHK(hipEventRecord(ev[2 * ax], h->stream));
-      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua));
+      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, -1, -1, nullptr));
       HIPCHK(hipEventRecord(ev[2 * ax + 1], h->stream));
     }
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -526,7 +508,7 @@ int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out) {
 }
 
 int cfd_copy_probe(cfd_handle* h, int reps) {
-  if (!h || reps < 1) return fail(CFD_E_ARG, "bad argument");
+  if (!h || reps < 1) return cfd_fail(CFD_E_ARG, "bad argument");
   long n2 = h->N * h->NV / 2;
   for (int r = 0; r < reps; r++)
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, h->stream, reinterpret_cast<const double2*>(h->U0),
@@ -537,7 +519,7 @@ int cfd_copy_probe(cfd_handle* h, int reps) {
 }
 
 int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out) {
-  if (!h || reps < 1) return fail(CFD_E_ARG, "bad argument");
+  if (!h || reps < 1) return cfd_fail(CFD_E_ARG, "bad argument");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal, 0.0);
   HIPCHK(hipGetLastError());
   hipEvent_t e0, e1;
@@ -551,10 +533,10 @@ int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out) {
     const int flags = SW_WRITE_R | (ax > 0 ? SW_READ_R : 0);
     for (int c = 0; c < nc; c++) {
       // warm-up launch, then timed launches
-      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
+      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg, nullptr));
       HIPCHK(hipEventRecord(e0, h->stream));
       for (int r = 0; r < reps; r++)
-        HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg));
+        HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg, nullptr));
       HIPCHK(hipEventRecord(e1, h->stream));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0;
@@ -574,7 +556,7 @@ int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out) {
 }
 
 int cfd_launch_plan(cfd_handle* h, int* tiles_out, int* segs_out) {
-  if (!h) return fail(CFD_E_ARG, "null handle");
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
   for (int ax = 0; ax < 3; ax++) {
     if (tiles_out) tiles_out[ax] = h->tile[ax];
     if (segs_out) segs_out[ax] = h->seg[ax];
@@ -583,11 +565,11 @@ int cfd_launch_plan(cfd_handle* h, int* tiles_out, int* segs_out) {
 }
 
 int cfd_set_launch_plan(cfd_handle* h, const int* tiles, const int* segs) {
-  if (!h || !tiles || !segs) return fail(CFD_E_ARG, "null argument");
+  if (!h || !tiles || !segs) return cfd_fail(CFD_E_ARG, "null argument");
   for (int ax = 0; ax < h->D; ax++) {
     bool ok = ax == 0 ? (tiles[ax] == 0 || tiles[ax] == 1) : (tiles[ax] == 1 || tiles[ax] == 2);
-    if (!ok) return fail(CFD_E_ARG, "tile code not compiled for this direction");
-    if (segs[ax] < 1 || segs[ax] > h->cfg.n[ax]) return fail(CFD_E_ARG, "segment length out of range");
+    if (!ok) return cfd_fail(CFD_E_ARG, "tile code not compiled for this direction");
+    if (segs[ax] < 1 || segs[ax] > h->cfg.n[ax]) return cfd_fail(CFD_E_ARG, "segment length out of range");
   }
   for (int ax = 0; ax < h->D; ax++) {
     h->tile[ax] = tiles[ax];
@@ -597,13 +579,13 @@ int cfd_set_launch_plan(cfd_handle* h, const int* tiles, const int* segs) {
 }
 
 int cfd_synchronize(cfd_handle* h) {
-  if (!h) return fail(CFD_E_ARG, "null handle");
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
   HIPCHK(hipStreamSynchronize(h->stream));
   return CFD_OK;
 }
 
 int cfd_device_buffers(cfd_handle* h, double** state, double** rhs) {
-  if (!h) return fail(CFD_E_ARG, "null handle");
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
   if (state) *state = h->U0;
   if (rhs) *rhs = static_cast<double*>(h->R);
   return CFD_OK;

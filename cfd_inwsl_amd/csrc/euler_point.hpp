@@ -23,7 +23,7 @@ enum FluxId { FLX_HLLC = 0, FLX_ROE = 1 };
 template <class T> __device__ __forceinline__ T sq(T a) { return a * a; }
 // The reference's 1e-40 regulariser of WENO-Z / TENO is below the FP32
 // normal range; the FP32 variant uses 1e-36 (a normal float) instead.
-template <class T> __device__ __forceinline__ T eps_tiny() { return eps_tiny<T>(); }
+template <class T> __device__ __forceinline__ T eps_tiny() { return T(1e-40); }
 template <> __device__ __forceinline__ float eps_tiny<float>() { return 1e-36f; }
 // std::min / std::max argument semantics
 template <class T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }

@@ -1,0 +1,38 @@
+// Internal state of a cfd_handle, shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/congfd.h"
+
+struct AdjointStore;
+
+struct cfd_handle {
+  cfd_config cfg;
+  int D, NV;
+  long N;
+  int sch, flx;
+  bool cw;
+  double ct;
+  double* U0;            // FP64 state at the start of the step
+  void *Ua, *Ub, *R;     // stage states and rhs: element type of the sweeps
+  float* U0f;            // FP32 shadow of U0 (FP32 precision only)
+  bool fp32;
+  size_t esz;            // bytes per element of Ua / Ub / R
+  double* scal;          // [0] dt, [1..3] max wave speed per direction (bits), [4] t
+  double* partial;       // block partial sums for totals
+  int npartial;
+  int keep_rhs;
+  int tile[3];     // tile shape per direction
+  int seg[3];      // segment length per direction
+  AdjointStore* adj;     // checkpoints and work buffers of the adjoint (lazily allocated)
+  hipStream_t stream;
+};
+
+int cfd_fail(int code, const std::string& msg);
+hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int stage, const double* U0, void* Uout,
+                        int tile, int seg, void* Ushadow);
+int rk_stage(cfd_handle* h, int stage, const void* Ucur, const double* U0, void* Uout, void* Ushadow);
+const void* sweep_state(cfd_handle* h);
+void cfd_adjoint_release(cfd_handle* h);
