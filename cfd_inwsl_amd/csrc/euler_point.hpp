@@ -67,11 +67,29 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
       a0 = T(0.0625) / sq(T(1e-6) + b0);
       a1 = T(0.625) / sq(T(1e-6) + b1);
       a2 = T(0.3125) / sq(T(1e-6) + b2);
-    } else {
+    } else if (sizeof(T) == sizeof(double)) {
       T tau = fabs(b2 - b0);
       a0 = T(0.0625) * (T(1.0) + sq(tau / (b0 + eps_tiny<T>())));
       a1 = T(0.625) * (T(1.0) + sq(tau / (b1 + eps_tiny<T>())));
       a2 = T(0.3125) * (T(1.0) + sq(tau / (b2 + eps_tiny<T>())));
+    } else {
+      // FP32: (tau / beta)^2 overflows the float range once beta is ~19
+      // orders below tau, and inf / inf would follow.  The weights are
+      // invariant to a common factor, so they are scaled by (min_k beta_k /
+      // tau)^2 whenever tau exceeds the smallest beta: every term stays in
+      // [0, 2] and the normalised weights are the same real numbers.
+      T tau = fabs(b2 - b0);
+      T s0 = b0 + eps_tiny<T>(), s1 = b1 + eps_tiny<T>(), s2 = b2 + eps_tiny<T>();
+      T sm = smin(smin(s0, s1), s2);
+      T base = T(1.0);
+      T num = tau;
+      if (tau > sm) {
+        base = sq(sm / tau);
+        num = sm;
+      }
+      a0 = T(0.0625) * (base + sq(num / s0));
+      a1 = T(0.625) * (base + sq(num / s1));
+      a2 = T(0.3125) * (base + sq(num / s2));
     }
     T s = a0 + a1 + a2;
     T r = a0 * u0 + a1 * u1 + a2 * u2;
@@ -79,9 +97,27 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
   } else if (SCH == SCH_TENO) {
     // Teno5_Z (interScheme.hpp:384-452); ct = 1e-5 (CT4: 1e-4, CT7: 1e-7)
     T tau = fabs(b2 - b0);
-    T t0 = T(1.0) + tau / (b0 + eps_tiny<T>());
-    T t1 = T(1.0) + tau / (b1 + eps_tiny<T>());
-    T t2 = T(1.0) + tau / (b2 + eps_tiny<T>());
+    T t0, t1, t2;
+    if (sizeof(T) == sizeof(double)) {
+      t0 = T(1.0) + tau / (b0 + eps_tiny<T>());
+      t1 = T(1.0) + tau / (b1 + eps_tiny<T>());
+      t2 = T(1.0) + tau / (b2 + eps_tiny<T>());
+    } else {
+      // FP32: (1 + tau / beta)^6 overflows for tau / beta > ~2.6e6.  The
+      // cut-off test g_k < ct * sum(g) is invariant to a common factor, so
+      // the terms are scaled by min_k beta_k / tau when tau exceeds it.
+      T s0 = b0 + eps_tiny<T>(), s1 = b1 + eps_tiny<T>(), s2 = b2 + eps_tiny<T>();
+      T sm = smin(smin(s0, s1), s2);
+      T base = T(1.0);
+      T num = tau;
+      if (tau > sm) {
+        base = sm / tau;
+        num = sm;
+      }
+      t0 = base + num / s0;
+      t1 = base + num / s1;
+      t2 = base + num / s2;
+    }
     t0 = t0 * t0;
     t1 = t1 * t1;
     t2 = t2 * t2;
