@@ -575,4 +575,150 @@ template <int AX> CFD_HD inline void interface(const double w[6][4], double* F, 
   for (int j = 0; j < 6; j++) p2c_vjp(w[j], lc[j], lw[j]);
 }
 
+// ---- ghost cells ------------------------------------------------------------------
+// Source of ghost cell g (g < 0 or g >= n) of a pencil of n cells under the
+// boundary code bc, as the forward sweep fills it: the cell it copies, the
+// primitive component it negates (-1: none) and whether it is a fixed state
+// (Dirichlet: no cotangent flows back).  Codes as in congfd.h.
+struct GhostSrc {
+  int src, flip;
+  bool fixed;
+};
+CFD_HD inline GhostSrc ghost_source(int bc, int g, int n) {
+  const bool lo = g < 0;
+  GhostSrc s{lo ? -1 - g : 2 * n - 1 - g, -1, false};
+  switch (bc) {
+  case 1: s.src = ((g % n) + n) % n; break;       // periodic
+  case 7: s.src = lo ? 0 : n - 1; break;           // outflow
+  case 3: s.fixed = true; break;                   // Dirichlet
+  case 2:
+  case 10: s.flip = 1; break;                      // symmetry, u negated
+  case 11: s.flip = 2; break;                      // symmetry, v negated
+  default: break;                                  // mirror copy
+  }
+  return s;
+}
+
+// ---- register-lean reverse pass of interface<AX> ----------------------------------
+// Same derivative as interface<AX>(w, F, lw, lF), for device use: the six cells
+// are read through W(j, v) (LDS on the device) and the per-wave projections
+// and WENO intermediates are recomputed in the reverse sweep instead of held,
+// so the live state is the eigenvector matrices, the cotangent accumulators
+// and lw.  lw is accumulated (+=).
+template <int AX, class W> CFD_HD inline void interface_vjp_lean(const W& w, const double* lF, double lw[6][4]) {
+  double pl[4], pr[4];
+  for (int v = 0; v < 4; v++) {
+    pl[v] = w(2, v);
+    pr[v] = w(3, v);
+  }
+  Roe2<AX> es;
+  es.fwd(pl, pr);
+  double L[4][4], R[4][4];
+  es.mats(L, R);
+  // characteristic values of wave k on the six cells
+  auto project = [&](int k, double* ch) {
+    for (int j = 0; j < 6; j++) {
+      double p[4], c[4];
+      for (int v = 0; v < 4; v++) p[v] = w(j, v);
+      p2c(p, c);
+      ch[j] = L[k][0] * c[0] + L[k][1] * c[1] + L[k][2] * c[2] + L[k][3] * c[3];
+    }
+  };
+  double QL[4], QR[4];
+  for (int k = 0; k < 4; k++) {
+    double ch[6];
+    project(k, ch);
+    double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
+    double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
+    JS a, b;
+    a.fwd(ql);
+    b.fwd(qr);
+    QL[k] = a.out;
+    QR[k] = b.out;
+  }
+  double cL[4], cR[4];
+  for (int i = 0; i < 4; i++) {
+    cL[i] = R[i][0] * QL[0] + R[i][1] * QL[1] + R[i][2] * QL[2] + R[i][3] * QL[3];
+    cR[i] = R[i][0] * QR[0] + R[i][1] * QR[1] + R[i][2] * QR[2] + R[i][3] * QR[3];
+  }
+  double oL[4], oR[4];
+  c2p(cL, oL);
+  c2p(cR, oR);
+  const bool fbL = oL[0] < 0 || oL[3] < 0, fbR = oR[0] < 0 || oR[3] < 0;
+  if (fbL)
+    for (int v = 0; v < 4; v++) oL[v] = pl[v];
+  if (fbR) {
+    oR[0] = pr[0];
+    oR[1] = pl[1];
+    oR[2] = pl[2];
+    oR[3] = pr[3];
+  }
+  double loL[4] = {0, 0, 0, 0}, loR[4] = {0, 0, 0, 0};
+  {
+    double Fd[4];
+    hllc<AX>(oL, oR, Fd, loL, loR, lF);
+  }
+  double lcL[4] = {0, 0, 0, 0}, lcR[4] = {0, 0, 0, 0};
+  if (fbL) {
+    for (int v = 0; v < 4; v++) lw[2][v] += loL[v];
+  } else {
+    c2p_vjp(cL, loL, lcL);
+  }
+  if (fbR) {
+    lw[3][0] += loR[0];
+    lw[2][1] += loR[1];
+    lw[2][2] += loR[2];
+    lw[3][3] += loR[3];
+  } else {
+    c2p_vjp(cR, loR, lcR);
+  }
+  double gR[4][4], gL[4][4];
+  double lQL[4], lQR[4];
+  for (int k = 0; k < 4; k++) {
+    lQL[k] = R[0][k] * lcL[0] + R[1][k] * lcL[1] + R[2][k] * lcL[2] + R[3][k] * lcL[3];
+    lQR[k] = R[0][k] * lcR[0] + R[1][k] * lcR[1] + R[2][k] * lcR[2] + R[3][k] * lcR[3];
+    for (int i = 0; i < 4; i++) gR[i][k] = QL[k] * lcL[i] + QR[k] * lcR[i];
+  }
+  double lc[6][4];
+  for (int j = 0; j < 6; j++)
+    for (int v = 0; v < 4; v++) lc[j][v] = 0.0;
+  for (int k = 0; k < 4; k++) {
+    double ch[6];
+    project(k, ch);
+    double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
+    double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
+    double gl[5] = {0, 0, 0, 0, 0}, gr[5] = {0, 0, 0, 0, 0};
+    {
+      JS a;
+      a.fwd(ql);
+      a.bwd(ql, lQL[k], gl);
+    }
+    {
+      JS b;
+      b.fwd(qr);
+      b.bwd(qr, lQR[k], gr);
+    }
+    double lch[6];
+    lch[0] = gl[0];
+    for (int j = 1; j < 5; j++) lch[j] = gl[j] + gr[5 - j];
+    lch[5] = gr[0];
+    for (int v = 0; v < 4; v++) gL[k][v] = 0.0;
+    for (int j = 0; j < 6; j++) {
+      double p[4], c[4];
+      for (int v = 0; v < 4; v++) p[v] = w(j, v);
+      p2c(p, c);
+      for (int v = 0; v < 4; v++) {
+        lc[j][v] += lch[j] * L[k][v];
+        gL[k][v] += lch[j] * c[v];
+      }
+    }
+  }
+  es.bwd(pl, pr, gL, gR, lw[2], lw[3]);
+  for (int j = 0; j < 6; j++) {
+    double p[4];
+    for (int v = 0; v < 4; v++) p[v] = w(j, v);
+    p2c_vjp(p, lc[j], lw[j]);
+  }
+}
+
 }  // namespace cfdadj
