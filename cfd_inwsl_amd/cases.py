@@ -85,7 +85,7 @@ def _random_phase_field(n, kmax, rng):
     mask = (kk >= 1) & (kk <= kmax)
     phase = rng.uniform(0, 2 * np.pi, size=kk.shape)
     spec = np.where(mask, np.exp(1j * phase), 0.0)
-    f = np.fft.irfftn(spec, s=(n, n, n))
+    f = np.fft.irfftn(spec, s=(n, n, n), axes=(0, 1, 2))
     return f / np.abs(f).max()
 
 
@@ -100,6 +100,70 @@ def periodic3d(n, seed=20261203, mach=0.3, amp=0.05, kmax=8):
         prim[..., v] = amp * _random_phase_field(n, kmax, rng)
     prim[..., 4] = 1.0 / (GAMMA * mach * mach)
     return prim_to_cons(prim).reshape(-1, 5), h
+
+
+def _random_phase_field_2d(n, kmax, rng):
+    """Real field on an n^2 periodic grid ([y][x]) with unit-amplitude
+    random-phase Fourier modes 1 <= |k| <= kmax, scaled to max |f| = 1."""
+    k = np.fft.fftfreq(n, d=1.0 / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n)
+    KY, KX = np.meshgrid(k, kr, indexing="ij")
+    kk = np.sqrt(KX * KX + KY * KY)
+    mask = (kk >= 1) & (kk <= kmax)
+    phase = rng.uniform(0, 2 * np.pi, size=kk.shape)
+    spec = np.where(mask, np.exp(1j * phase), 0.0)
+    f = np.fft.irfftn(spec, s=(n, n), axes=(0, 1))
+    return f / np.abs(f).max()
+
+
+# Rankine-Hugoniot jumps of a normal shock at upstream Mach 1.5, gamma 1.4
+_M_SHOCK = 1.5
+_RHO_JUMP = (GAMMA + 1) * _M_SHOCK ** 2 / ((GAMMA - 1) * _M_SHOCK ** 2 + 2)
+_P_JUMP = 1 + 2 * GAMMA / (GAMMA + 1) * (_M_SHOCK ** 2 - 1)
+
+
+def adjoint_shock(n, seed=20261204, amp=1e-2, kmax=8):
+    """Adjoint workload on the periodic unit square, n x n cells.
+
+    rho0 = 1 + amp * f (seeded random-phase modes |k| <= kmax) multiplies a
+    density profile that holds a stationary normal shock at x = 1/2: upstream
+    (rho, u, p) = (1, 1.5, 1/gamma) (Mach 1.5), downstream the
+    Rankine-Hugoniot state, which relaxes smoothly back to the upstream state
+    around x = 0.9 so that the periodic seam is not a second shock; v = 0.
+    Returns a dict with rho0 [ny][nx], the profile factors (rho_factor, u,
+    p, all [ny][nx]), h, and the objective window (y slice, x slice) of
+    n/8 x n/8 cells just downstream of the shock.
+    """
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    xc = (np.arange(n) + 0.5) * h
+    phi = np.where(xc > 0.5, 1.0, 0.0) * 0.5 * (1.0 - np.tanh((xc - 0.9) / 0.02))
+    u_up, p_up = _M_SHOCK, 1.0 / GAMMA
+    u_dn = u_up / _RHO_JUMP
+    rho_factor = 1.0 + (_RHO_JUMP - 1.0) * phi
+    u = u_up + (u_dn - u_up) * phi
+    p = p_up * (1.0 + (_P_JUMP - 1.0) * phi)
+    rho0 = 1.0 + amp * _random_phase_field_2d(n, kmax, rng)
+    w = max(n // 8, 1)
+    i0 = n // 2 + max(n // 128, 1)
+    j0 = n // 2 - w // 2
+    ones = np.ones((n, 1))
+    return {"rho0": rho0, "rho_factor": ones * rho_factor, "u": ones * u, "v": np.zeros((n, n)), "p": ones * p,
+            "h": h, "window": (slice(j0, j0 + w), slice(i0, i0 + w))}
+
+
+def adjoint_state(case, rho0=None):
+    """Conservative state [cell][4] of an adjoint_shock case for the density field rho0."""
+    r = (case["rho0"] if rho0 is None else rho0) * case["rho_factor"]
+    prim = np.stack([r, case["u"], case["v"], case["p"]], -1)
+    return prim_to_cons(prim).reshape(-1, 4)
+
+
+def cfl_dt_2d(cons, n, h, cfl=0.5):
+    """dt = CFL / sum_d max (c + |u_d|) / h (BlockSolver::getTimeIntervalExplicit)."""
+    pr = cons_to_prim(np.asarray(cons).reshape(-1, 4))
+    c = np.sqrt(GAMMA * pr[:, 3] / pr[:, 0])
+    return cfl / ((c + np.abs(pr[:, 1])).max() / h + (c + np.abs(pr[:, 2])).max() / h)
 
 
 def riemann2d_quadrants(nx, ny, seed):

@@ -1,15 +1,516 @@
-// Discrete adjoint of the 2D RHS and SSP-RK3 step (characteristic-wise
-// WENO-JS + HLLC + MND6): device buffers and entry points.
+// Discrete adjoint of the 2D right-hand side and of the SSP-RK3 step for the
+// characteristic-wise WCNS5 (WENO-JS) + HLLC + MND6 configuration, FP64.
+//
+// Forward map (per direction, per pencil of n cells, G_k the interface flux
+// between cells k-1 and k built from cells k-3 .. k+2, f_j the node flux):
+//   rhs_i += (w1 (f_{i+1} - f_{i-1}) + w0 (G_{i+1} - G_i) + w2 (G_{i+2} - G_{i-1})) / h
+// (SpDistributor::rhsSolve, zone_FD/src/src/sp_distributor.cpp:9-78, with
+// MidNodeAndNodeDif6, zone_FD/src/src/differ.cpp:54-103).  Its transpose:
+//   lambda_G_k = (w0 (mu_{k-1} - mu_k) + w2 (mu_{k-2} - mu_{k+1})) / h
+//   lambda_f_j = w1 (mu_{j-1} - mu_{j+1}) / h              (mu = 0 off the pencil)
+// then the interface / node-flux vector-Jacobian products give primitive
+// cotangents on cells -4 .. n+3, ghosts fold back onto their source cells,
+// and the conservative cotangent is (d prim / d cons)^T of the sum.
+//
+// Kernel: one workgroup owns CP pencils x one segment of cells and marches
+// along the pencil in chunks of CA interfaces.  Per chunk: load cells
+// (primitive from U, scaled cotangent, zeroed accumulator) into LDS rings;
+// each lane takes one interface through the reverse pass; its six cell
+// cotangents are added to the ring in six phases separated by barriers (lane
+// k adds to cell k-3+j in phase j, so no two lanes meet and the summation
+// order is fixed: results are deterministic); finally each lane finalises one
+// cell whose six interfaces are done.  Ghost-cell cotangents go to a small
+// per-pencil buffer and a second kernel folds them onto their source cells.
+//
+// RK3 (BlockSolver::RK3_SSP, zone_FD/src/src/blockSolver.cpp:43-87):
+//   u1 = u0 - dt R(u0); u2 = 3/4 u0 + 1/4 u1 - 1/4 dt R(u1);
+//   u3 = 1/3 u0 + 2/3 u2 - 2/3 dt R(u2)
+// reversed with l3 = dJ/du3:
+//   l2 = 2/3 l3 - 2/3 dt R'(u2)^T l3;  l1 = 1/4 l2 - 1/4 dt R'(u1)^T l2;
+//   l0 = 1/3 l3 + 3/4 l2 + l1 - dt R'(u0)^T l1.
+// Time steps are fixed (dt is a constant of the map, not a function of u).
+// The forward pass keeps the state every `checkpoint_every` steps; the
+// backward pass recomputes each segment's stage states from its checkpoint
+// with the forward sweep kernels, then walks the segment backwards.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "adjoint_point.hpp"
+#include "euler_point.hpp"
 #include "handle.hpp"
 
-struct AdjointStore {
-  int dummy;
+namespace {
+
+constexpr int ADJ_CA = 16;  // interfaces per chunk
+constexpr int ADJ_CP = 16;  // pencils per workgroup
+
+struct VjpArgs {
+  const double* U;   // linearisation state, SoA planes of N
+  const double* MU;  // cotangent of the rhs, SoA
+  double* OUT;       // += coef * (d rhs_AX / d U)^T MU, SoA
+  double* GH;        // ghost cotangents [pencil][8][4]
+  double coef;
+  long N;
+  int n[2];
+  int seg_len;
+  double h;
+  int bc_lo, bc_hi;
+  double prim_lo[4], prim_hi[4];
 };
 
+// Forward record of one interface in the forward sweep's own arithmetic
+// (the characteristic-wise path of sweep_block, operation for operation), so
+// that every branch the reverse pass follows -- positivity fallback, HLLC
+// pressure clipping, shock factors, wave region -- is the branch the solver
+// took, also where a predicate is tied at rounding level (pressure
+// equilibrium makes p* - p_L vanish up to rounding).
+template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd& f) {
+  double pL[4], pR[4];
+  for (int v = 0; v < 4; v++) {
+    pL[v] = w(2, v);
+    pR[v] = w(3, v);
+  }
+  cfd::RoeSys<2, AX, double> es;
+  es.build(pL, pR);
+  for (int k = 0; k < 4; k++) {
+    double l[4];
+    es.lrow(k, l);
+    double ch[6];
+    for (int j = 0; j < 6; j++) {
+      double p[4], c[4];
+      for (int v = 0; v < 4; v++) p[v] = w(j, v);
+      cfd::prim_to_cons<2>(p, c);
+      ch[j] = cfd::project<2>(c, l);
+    }
+    for (int v = 0; v < 4; v++) f.L[k][v] = l[v];
+    f.QL[k] = cfd::inter5<cfd::SCH_JS>(ch[0], ch[1], ch[2], ch[3], ch[4], 0.0);
+    f.QR[k] = cfd::inter5<cfd::SCH_JS>(ch[5], ch[4], ch[3], ch[2], ch[1], 0.0);
+  }
+  // charToCons in the summation order (ch3, ch4, ch2, ch1)
+  const int order[4] = {2, 3, 1, 0};
+  for (int t = 0; t < 4; t++) {
+    const int k = order[t];
+    double r[4];
+    es.rcol(k, r);
+    for (int i = 0; i < 4; i++) {
+      f.R[i][k] = r[i];
+      if (t == 0) {
+        f.cL[i] = f.QL[k] * r[i];
+        f.cR[i] = f.QR[k] * r[i];
+      } else {
+        f.cL[i] = f.cL[i] + f.QL[k] * r[i];
+        f.cR[i] = f.cR[i] + f.QR[k] * r[i];
+      }
+    }
+  }
+  cfd::eig_cons_to_prim<2>(f.cL, f.oL);
+  cfd::eig_cons_to_prim<2>(f.cR, f.oR);
+  f.fbL = f.oL[0] < 0 || f.oL[3] < 0;
+  f.fbR = f.oR[0] < 0 || f.oR[3] < 0;
+  if (f.fbL)
+    for (int v = 0; v < 4; v++) f.oL[v] = pL[v];
+  if (f.fbR) {
+    f.oR[0] = pR[0];
+    f.oR[1] = pL[1];
+    f.oR[2] = pL[2];
+    f.oR[3] = pR[3];
+  }
+  f.hb = cfdadj::hllc_branch_bits<AX>(f.oL, f.oR);
+}
+
+template <int AX, int CA, int CP>
+__global__ __launch_bounds__(CA* CP) void k_rhs_vjp(VjpArgs a) {
+  constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
+  __shared__ double wc[4 * RC * CP];  // primitive state
+  __shared__ double mc[4 * RC * CP];  // coef * cotangent
+  __shared__ double ac[4 * RC * CP];  // accumulated primitive cotangent
+  auto idx = [](int v, int slot, int p) -> int {
+    return AX == 0 ? (v * CP + p) * RC + slot : (v * RC + slot) * CP + p;
+  };
+  auto slot_of = [](int c) -> int { return (c + 8 * RC) % RC; };
+
+  const int lane = threadIdx.x;
+  const int la = AX == 0 ? (lane % CA) : (lane / CP);
+  const int lp = AX == 0 ? (lane / CA) : (lane % CP);
+  const int nx = a.n[0], ny = a.n[1];
+  const int n = a.n[AX];
+  const long sa = AX == 0 ? 1 : nx;
+  const long npencil = AX == 0 ? ny : nx;
+  const long gp = (long)blockIdx.x * CP + lp;
+  const bool pvalid = gp < npencil;
+  const long base = pvalid ? (AX == 0 ? gp * nx : gp) : 0;
+  const int c0 = blockIdx.y * a.seg_len;
+  const int c1 = min(c0 + a.seg_len, n);
+  const int e0 = c0 == 0 ? -4 : c0;  // cells finalised here (ghosts at the ends)
+  const int e1 = c1 == n ? n + 4 : c1;
+  const int kmin = max(e0 - 2, -1), kmax = min(e1 + 2, n + 1);
+  const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
+
+  auto load = [&](int c) {
+    const int s = slot_of(c);
+    double p[4], m[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c >= 0 && c < n) {
+      const long i = base + (long)c * sa;
+      double u[4];
+      for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
+      cfd::cons_to_prim<2>(u, p);
+      for (int v = 0; v < 4; v++) m[v] = a.coef * a.MU[v * a.N + i];
+    } else {
+      const bool lo = c < 0;
+      const cfdadj::GhostSrc gs = cfdadj::ghost_source(lo ? a.bc_lo : a.bc_hi, c, n);
+      if (gs.fixed) {
+        for (int v = 0; v < 4; v++) p[v] = lo ? a.prim_lo[v] : a.prim_hi[v];
+      } else {
+        const long i = base + (long)gs.src * sa;
+        double u[4];
+        for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
+        cfd::cons_to_prim<2>(u, p);
+        for (int v = 1; v < 3; v++)
+          if (v == gs.flip) p[v] = -p[v];
+      }
+    }
+    for (int v = 0; v < 4; v++) {
+      wc[idx(v, s, lp)] = p[v];
+      mc[idx(v, s, lp)] = m[v];
+      ac[idx(v, s, lp)] = 0.0;
+    }
+  };
+  auto M = [&](int c, int v) -> double { return mc[idx(v, slot_of(c), lp)]; };
+  struct Cells {
+    const double* wc;
+    int k, lp;
+    __device__ double operator()(int j, int v) const {
+      const int c = k - 3 + j;
+      const int s = (c + 8 * RC) % RC;
+      return wc[AX == 0 ? (v * CP + lp) * RC + s : (v * RC + s) * CP + lp];
+    }
+  };
+
+  if (pvalid && la < 5) load(kmin - 3 + la);
+  for (int k0 = kmin; k0 - 3 < e1; k0 += CA) {
+    {
+      const int c = k0 + 2 + la;
+      if (pvalid && c <= kmax + 2) load(c);
+    }
+    __syncthreads();
+    const int k = k0 + la;
+    const bool act = pvalid && k <= kmax;
+    double lw[6][4];
+    for (int j = 0; j < 6; j++)
+      for (int v = 0; v < 4; v++) lw[j][v] = 0.0;
+    if (act) {
+      double lG[4];
+      for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) / a.h;
+      Cells cells{wc, k, lp};
+      cfdadj::IfaceFwd f;
+      iface_forward_exact<AX>(cells, f);
+      cfdadj::iface_backward<AX>(cells, f, lG, lw);
+    }
+    for (int j = 0; j < 6; j++) {
+      if (act) {
+        const int s = slot_of(k - 3 + j);
+        for (int v = 0; v < 4; v++) ac[idx(v, s, lp)] += lw[j][v];
+      }
+      __syncthreads();
+    }
+    {
+      const int c = k0 - 3 + la;
+      if (pvalid && c >= e0 && c < e1) {
+        const int s = slot_of(c);
+        double lwc[4];
+        for (int v = 0; v < 4; v++) lwc[v] = ac[idx(v, s, lp)];
+        if (c >= -1 && c <= n) {  // node fluxes exist on cells -1 .. n
+          double p[4], lf[4];
+          for (int v = 0; v < 4; v++) {
+            p[v] = wc[idx(v, s, lp)];
+            lf[v] = w1 * (M(c - 1, v) - M(c + 1, v)) / a.h;
+          }
+          cfdadj::node_flux_vjp<AX>(p, lf, lwc);
+        }
+        if (c >= 0 && c < n) {
+          const long i = base + (long)c * sa;
+          double u[4], g[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
+          cfdadj::c2p_vjp(u, lwc, g);
+          for (int v = 0; v < 4; v++) a.OUT[v * a.N + i] += g[v];
+        } else {
+          const int gs = c < 0 ? c + 4 : c - n + 4;  // 0..7
+          for (int v = 0; v < 4; v++) a.GH[(gp * 8 + gs) * 4 + v] = lwc[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Ghost cotangents onto their source cells, one thread per pencil, ghosts in
+// a fixed order (deterministic sums).
+template <int AX> __global__ __launch_bounds__(256) void k_ghost_fold(VjpArgs a) {
+  const long gp = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = a.n[0], ny = a.n[1];
+  const int n = a.n[AX];
+  const long npencil = AX == 0 ? ny : nx;
+  if (gp >= npencil) return;
+  const long sa = AX == 0 ? 1 : nx;
+  const long base = AX == 0 ? gp * nx : gp;
+  for (int g = 0; g < 8; g++) {
+    const int c = g < 4 ? g - 4 : n + g - 4;
+    const cfdadj::GhostSrc gs = cfdadj::ghost_source(g < 4 ? a.bc_lo : a.bc_hi, c, n);
+    if (gs.fixed) continue;
+    double l[4];
+    for (int v = 0; v < 4; v++) l[v] = a.GH[(gp * 8 + g) * 4 + v];
+    if (gs.flip >= 0) l[gs.flip] = -l[gs.flip];
+    const long i = base + (long)gs.src * sa;
+    double u[4], gr[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
+    cfdadj::c2p_vjp(u, l, gr);
+    for (int v = 0; v < 4; v++) a.OUT[v * a.N + i] += gr[v];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill(double* x, double val, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = val;
+}
+__global__ __launch_bounds__(256) void k_scale(double* __restrict__ out, const double* __restrict__ in, double s,
+                                               long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = s * in[i];
+}
+// l0 = 1/3 l3 + 3/4 l2 + l1 (in place over l3)
+__global__ __launch_bounds__(256) void k_comb0(double* __restrict__ l3, const double* __restrict__ l2,
+                                               const double* __restrict__ l1, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    l3[i] = 1.0 / 3.0 * l3[i] + 0.75 * l2[i] + l1[i];
+}
+__global__ void k_aos_to_soa4(const double* __restrict__ in, double* __restrict__ out, long N) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N)
+    for (int v = 0; v < 4; v++) out[v * N + i] = in[i * 4 + v];
+}
+__global__ void k_soa_to_aos4(const double* __restrict__ in, double* __restrict__ out, long N) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N)
+    for (int v = 0; v < 4; v++) out[i * 4 + v] = in[v * N + i];
+}
+__global__ void k_put_dt(double* scal, double dt) { scal[0] = dt; }
+
+}  // namespace
+
+struct AdjointStore {
+  size_t state_bytes = 0;
+  double *LA = nullptr, *LB = nullptr, *LC = nullptr;  // cotangent buffers
+  double* GH = nullptr;                                // ghost cotangents
+  double* stage = nullptr;                             // segment stage states (3K+1)
+  int stage_cap = 0;                                   // states the stage buffer holds
+  std::vector<double*> ckpt;                           // checkpointed states
+  int nsteps = 0, every = 0;
+  double dt = 0.0;
+  int vjp_seg[2] = {0, 0};
+};
+
+#define AHIP(expr)                                                                      \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return cfd_fail(CFD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
 void cfd_adjoint_release(cfd_handle* h) {
-  if (!h->adj) return;
-  delete h->adj;
+  AdjointStore* s = h->adj;
+  if (!s) return;
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(s->LA);
+  (void)hipFree(s->LB);
+  (void)hipFree(s->LC);
+  (void)hipFree(s->GH);
+  (void)hipFree(s->stage);
+  for (double* p : s->ckpt) (void)hipFree(p);
+  delete s;
   h->adj = nullptr;
 }
+
+static int adj_supported(cfd_handle* h) {
+  if (h->D != 2 || h->fp32 || !h->cw || h->flx != 0 || h->sch != cfd::SCH_JS)
+    return cfd_fail(CFD_E_UNSUPP,
+                    "the adjoint covers 2D FP64 characteristic-wise WCNS5 (WENO-JS) with HLLC flux");
+  return 0;
+}
+
+static int adj_ensure(cfd_handle* h) {
+  if (int rc = adj_supported(h)) return rc;
+  if (h->cfg.n[0] < 8 || h->cfg.n[1] < 8) return cfd_fail(CFD_E_UNSUPP, "the adjoint needs at least 8 cells per direction");
+  if (h->adj) return 0;
+  AdjointStore* s = new AdjointStore();
+  h->adj = s;
+  s->state_bytes = sizeof(double) * h->N * 4;
+  const long np = std::max(h->cfg.n[0], h->cfg.n[1]);
+  AHIP(hipMalloc(&s->LA, s->state_bytes));
+  AHIP(hipMalloc(&s->LB, s->state_bytes));
+  AHIP(hipMalloc(&s->LC, s->state_bytes));
+  AHIP(hipMalloc(&s->GH, sizeof(double) * np * 8 * 4));
+  // segment length per direction: enough workgroups to fill the chip
+  for (int ax = 0; ax < 2; ax++) {
+    const int n = h->cfg.n[ax];
+    const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
+    const long pb = (pencils + ADJ_CP - 1) / ADJ_CP;
+    long nseg = (2048 + pb - 1) / pb;
+    nseg = std::max<long>(1, std::min<long>(nseg, n / 16));
+    s->vjp_seg[ax] = (int)((n + nseg - 1) / nseg);
+  }
+  return 0;
+}
+
+static const int kFillBlocks = 1024;
+
+// OUT += coef * (d rhs / d U)^T MU at the state U, both directions.
+static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef, double* OUT) {
+  AdjointStore* s = h->adj;
+  for (int ax = 0; ax < 2; ax++) {
+    VjpArgs a;
+    memset(&a, 0, sizeof(a));
+    a.U = U;
+    a.MU = MU;
+    a.OUT = OUT;
+    a.GH = s->GH;
+    a.coef = coef;
+    a.N = h->N;
+    a.n[0] = h->cfg.n[0];
+    a.n[1] = h->cfg.n[1];
+    a.seg_len = s->vjp_seg[ax];
+    a.h = h->cfg.h[ax];
+    a.bc_lo = h->cfg.bc[2 * ax];
+    a.bc_hi = h->cfg.bc[2 * ax + 1];
+    for (int v = 0; v < 4; v++) {
+      a.prim_lo[v] = h->cfg.bc_prim[2 * ax][v];
+      a.prim_hi[v] = h->cfg.bc_prim[2 * ax + 1][v];
+    }
+    const int n = h->cfg.n[ax];
+    const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
+    const dim3 grid((unsigned)((pencils + ADJ_CP - 1) / ADJ_CP), (unsigned)((n + a.seg_len - 1) / a.seg_len));
+    const dim3 fold((unsigned)((pencils + 255) / 256));
+    if (ax == 0) {
+      hipLaunchKernelGGL((k_rhs_vjp<0, ADJ_CA, ADJ_CP>), grid, dim3(ADJ_CA * ADJ_CP), 0, h->stream, a);
+      hipLaunchKernelGGL(k_ghost_fold<0>, fold, dim3(256), 0, h->stream, a);
+    } else {
+      hipLaunchKernelGGL((k_rhs_vjp<1, ADJ_CA, ADJ_CP>), grid, dim3(ADJ_CA * ADJ_CP), 0, h->stream, a);
+      hipLaunchKernelGGL(k_ghost_fold<1>, fold, dim3(256), 0, h->stream, a);
+    }
+    AHIP(hipGetLastError());
+  }
+  return 0;
+}
+
+// One RK3 step backwards: s->LA holds l3 on entry and l0 on exit.
+static int adj_step_back(cfd_handle* h, const double* u0, const double* u1, const double* u2, double dt) {
+  AdjointStore* s = h->adj;
+  const long n = h->N * 4;
+  int rc;
+  hipLaunchKernelGGL(k_scale, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LB, s->LA, 2.0 / 3.0, n);
+  if ((rc = adj_vjp(h, u2, s->LA, -2.0 / 3.0 * dt, s->LB))) return rc;
+  hipLaunchKernelGGL(k_scale, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LC, s->LB, 0.25, n);
+  if ((rc = adj_vjp(h, u1, s->LB, -0.25 * dt, s->LC))) return rc;
+  hipLaunchKernelGGL(k_comb0, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LA, s->LB, s->LC, n);
+  if ((rc = adj_vjp(h, u0, s->LC, -dt, s->LA))) return rc;
+  AHIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" {
+
+int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad) {
+  if (!h || !cot || !grad) return cfd_fail(CFD_E_ARG, "null argument");
+  if (int rc = adj_ensure(h)) return rc;
+  AdjointStore* s = h->adj;
+  const unsigned nb = (unsigned)((h->N + 255) / 256);
+  AHIP(hipMemcpyAsync(s->LC, cot, s->state_bytes, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_aos_to_soa4, dim3(nb), dim3(256), 0, h->stream, s->LC, s->LA, h->N);
+  hipLaunchKernelGGL(k_fill, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LB, 0.0, h->N * 4);
+  if (int rc = adj_vjp(h, h->U0, s->LA, 1.0, s->LB)) return rc;
+  hipLaunchKernelGGL(k_soa_to_aos4, dim3(nb), dim3(256), 0, h->stream, s->LB, s->LC, h->N);
+  AHIP(hipGetLastError());
+  AHIP(hipMemcpyAsync(grad, s->LC, s->state_bytes, hipMemcpyDeviceToHost, h->stream));
+  AHIP(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
+int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_every) {
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
+  if (nsteps < 1 || checkpoint_every < 1 || !(dt > 0.0))
+    return cfd_fail(CFD_E_ARG, "need nsteps >= 1, checkpoint_every >= 1 and dt > 0");
+  if (int rc = adj_ensure(h)) return rc;
+  AdjointStore* s = h->adj;
+  const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+  while ((int)s->ckpt.size() > nck) {
+    AHIP(hipFree(s->ckpt.back()));
+    s->ckpt.pop_back();
+  }
+  while ((int)s->ckpt.size() < nck) {
+    double* p = nullptr;
+    AHIP(hipMalloc(&p, s->state_bytes));
+    s->ckpt.push_back(p);
+  }
+  const int seg = std::min(checkpoint_every, nsteps);
+  const int need = 3 * seg + 1;
+  if (s->stage_cap < need) {
+    if (s->stage) AHIP(hipFree(s->stage));
+    s->stage = nullptr;
+    s->stage_cap = 0;
+    AHIP(hipMalloc(&s->stage, s->state_bytes * need));
+    s->stage_cap = need;
+  }
+  s->nsteps = nsteps;
+  s->every = checkpoint_every;
+  s->dt = dt;
+  hipLaunchKernelGGL(k_put_dt, dim3(1), dim3(1), 0, h->stream, h->scal, dt);
+  for (int st = 0; st < nsteps; st++) {
+    if (st % checkpoint_every == 0)
+      AHIP(hipMemcpyAsync(s->ckpt[st / checkpoint_every], h->U0, s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+    int rc;
+    if ((rc = rk_stage(h, 1, h->U0, h->U0, h->Ua, nullptr))) return rc;
+    if ((rc = rk_stage(h, 2, h->Ua, h->U0, h->Ub, nullptr))) return rc;
+    if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->U0, nullptr))) return rc;
+  }
+  AHIP(hipGetLastError());
+  AHIP(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
+int cfd_adjoint_backward(cfd_handle* h, const double* cot_final, double* grad_initial) {
+  if (!h || !cot_final || !grad_initial) return cfd_fail(CFD_E_ARG, "null argument");
+  if (!h->adj || h->adj->nsteps < 1) return cfd_fail(CFD_E_ARG, "cfd_adjoint_forward has not run");
+  AdjointStore* s = h->adj;
+  const unsigned nb = (unsigned)((h->N + 255) / 256);
+  const size_t nd = (size_t)h->N * 4;
+  AHIP(hipMemcpyAsync(s->LB, cot_final, s->state_bytes, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_aos_to_soa4, dim3(nb), dim3(256), 0, h->stream, s->LB, s->LA, h->N);
+  hipLaunchKernelGGL(k_put_dt, dim3(1), dim3(1), 0, h->stream, h->scal, s->dt);
+  const int nck = (int)s->ckpt.size();
+  for (int m = nck - 1; m >= 0; m--) {
+    const int first = m * s->every;
+    const int len = std::min(s->every, s->nsteps - first);
+    // stage states of the segment: S_i at 3i, u1_i at 3i+1, u2_i at 3i+2
+    auto S = [&](int i) { return s->stage + (size_t)(3 * i) * nd; };
+    auto U1 = [&](int i) { return s->stage + (size_t)(3 * i + 1) * nd; };
+    auto U2 = [&](int i) { return s->stage + (size_t)(3 * i + 2) * nd; };
+    AHIP(hipMemcpyAsync(S(0), s->ckpt[m], s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+    int rc;
+    for (int i = 0; i < len; i++) {
+      if ((rc = rk_stage(h, 1, S(i), S(i), U1(i), nullptr))) return rc;
+      if ((rc = rk_stage(h, 2, U1(i), S(i), U2(i), nullptr))) return rc;
+      if (i + 1 < len)
+        if ((rc = rk_stage(h, 3, U2(i), S(i), S(i + 1), nullptr))) return rc;
+    }
+    for (int i = len - 1; i >= 0; i--)
+      if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt))) return rc;
+  }
+  hipLaunchKernelGGL(k_soa_to_aos4, dim3(nb), dim3(256), 0, h->stream, s->LA, s->LB, h->N);
+  AHIP(hipGetLastError());
+  AHIP(hipMemcpyAsync(grad_initial, s->LB, s->state_bytes, hipMemcpyDeviceToHost, h->stream));
+  AHIP(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
+}  // extern "C"

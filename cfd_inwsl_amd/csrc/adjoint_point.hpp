@@ -141,8 +141,17 @@ template <int AX> struct HSide {
   double H, Vn;
 };
 
+// Branch bits of one HLLC evaluation: bit 0 pressure clipped to 0, bit 1 left
+// shock factor active, bit 2 right shock factor active, bits 3-5 the wave
+// region (0 SL > 0, 1 S* > 0, 2 S* == 0, 3 SR > 0, 4 otherwise).
+enum { HB_CLIP = 1, HB_SHL = 2, HB_SHR = 4 };
+CFD_HD inline int hb_region(int b) { return (b >> 3) & 7; }
+
+// force >= 0: take the branches it names instead of deciding them from this
+// function's own arithmetic (the reverse pass follows the branches of the
+// forward evaluation, whose rounding can differ where a predicate is tied).
 template <int AX> CFD_HD inline void hllc(const double* L, const double* R, double* F, double* lL = nullptr,
-                                          double* lR = nullptr, const double* lF = nullptr) {
+                                          double* lR = nullptr, const double* lF = nullptr, int force = -1) {
   const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
   const double rl = L[0], ul = L[1], vl = L[2], pl = L[3];
   const double rr = R[0], ur = R[1], vr = R[2], pr = R[3];
@@ -152,10 +161,11 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   const double cl = sqrt(G * pl / rl), cr = sqrt(G * pr / rr);
   const double rBar = 0.5 * (rl + rr), cBar = 0.5 * (cl + cr);
   const double psr = 0.5 * (pl + pr) - 0.5 * rBar * cBar * (VnR - VnL);
-  const bool clip = !(0.0 < psr);
+  const bool clip = force >= 0 ? (force & HB_CLIP) != 0 : !(0.0 < psr);
   const double ps = clip ? 0.0 : psr;
   const double kap = (G + 1.0) / (2.0 * G);
-  const bool shl = !(ps <= pl), shr = !(ps <= pr);
+  const bool shl = force >= 0 ? (force & HB_SHL) != 0 : !(ps <= pl);
+  const bool shr = force >= 0 ? (force & HB_SHR) != 0 : !(ps <= pr);
   const double ql = shl ? sqrt(1.0 + kap * (ps / pl - 1.0)) : 1.0;
   const double qr = shr ? sqrt(1.0 + kap * (ps / pr - 1.0)) : 1.0;
   const double SL = VnL - cl * ql, SR = VnR + cr * qr;
@@ -272,7 +282,8 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   };
 
   int br;
-  if (SL > 0) br = 0;
+  if (force >= 0) br = hb_region(force);
+  else if (SL > 0) br = 0;
   else if (Ss > 0) br = 1;
   else if (Ss == 0) br = 2;
   else if (SR > 0) br = 3;
@@ -599,13 +610,46 @@ CFD_HD inline GhostSrc ghost_source(int bc, int g, int n) {
   return s;
 }
 
-// ---- register-lean reverse pass of interface<AX> ----------------------------------
-// Same derivative as interface<AX>(w, F, lw, lF), for device use: the six cells
-// are read through W(j, v) (LDS on the device) and the per-wave projections
-// and WENO intermediates are recomputed in the reverse sweep instead of held,
-// so the live state is the eigenvector matrices, the cotangent accumulators
-// and lw.  lw is accumulated (+=).
-template <int AX, class W> CFD_HD inline void interface_vjp_lean(const W& w, const double* lF, double lw[6][4]) {
+// ---- interface flux as a forward record + reverse pass ----------------------------
+// What the reverse pass of one interface needs from its forward evaluation.
+struct IfaceFwd {
+  double L[4][4], R[4][4];  // left eigenvectors (rows = waves) and right (R[i][k])
+  double QL[4], QR[4];      // characteristic half-node values
+  double cL[4], cR[4];      // conservative half-node states
+  double oL[4], oR[4];      // primitive states given to the flux
+  bool fbL, fbR;            // positivity fallback taken
+  int hb;                   // HLLC branch bits, or -1: decided by the reverse pass
+};
+
+// HLLC branch bits in HLLCFlux2D2's own arithmetic (fluxSchemes.hpp:202-327),
+// expression for expression, so that with the forward's half-node states the
+// bits are the branches the forward took.  Requires -ffp-contract=off.
+template <int AX> CFD_HD inline int hllc_branch_bits(const double* Lp, const double* Rp) {
+  const double rl = Lp[0], rr = Rp[0], pl = Lp[3], pr = Rp[3];
+  const double VnL = Lp[1 + AX], VnR = Rp[1 + AX];
+  const double cl = sqrt(pl / rl * 1.4), cr = sqrt(pr / rr * 1.4);
+  const double rBar = (rl + rr) / 2, cBar = (cl + cr) / 2;
+  const double psr = (pl + pr) / 2 - rBar * cBar * (VnR - VnL) / 2;
+  const bool clip = !(0.0 < psr);
+  const double ps = clip ? 0.0 : psr;
+  const bool shl = !(ps <= pl), shr = !(ps <= pr);
+  const double kap = (1.4 + 1.0) / (2.0 * 1.4);
+  const double ql = shl ? sqrt(1.0 + kap * (ps / pl - 1.0)) : 1.0;
+  const double qr = shr ? sqrt(1.0 + kap * (ps / pr - 1.0)) : 1.0;
+  const double SL = VnL - cl * ql, SR = VnR + cr * qr;
+  const double Ss = ((pr - pl) + (rl * VnL * (SL - VnL) - rr * VnR * (SR - VnR))) / (rl * (SL - VnL) - rr * (SR - VnR));
+  int reg;
+  if (SL > 0) reg = 0;
+  else if (Ss > 0) reg = 1;
+  else if (Ss == 0) reg = 2;
+  else if (SR > 0) reg = 3;
+  else reg = 4;
+  return (clip ? HB_CLIP : 0) | (shl ? HB_SHL : 0) | (shr ? HB_SHR : 0) | (reg << 3);
+}
+
+// Forward record in this file's own arithmetic (host checks).  The six
+// cells are read through W(j, v); the interface lies between cells 2 and 3.
+template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd& f) {
   double pl[4], pr[4];
   for (int v = 0; v < 4; v++) {
     pl[v] = w(2, v);
@@ -613,78 +657,84 @@ template <int AX, class W> CFD_HD inline void interface_vjp_lean(const W& w, con
   }
   Roe2<AX> es;
   es.fwd(pl, pr);
-  double L[4][4], R[4][4];
-  es.mats(L, R);
-  // characteristic values of wave k on the six cells
-  auto project = [&](int k, double* ch) {
+  es.mats(f.L, f.R);
+  for (int k = 0; k < 4; k++) {
+    double ch[6];
     for (int j = 0; j < 6; j++) {
       double p[4], c[4];
       for (int v = 0; v < 4; v++) p[v] = w(j, v);
       p2c(p, c);
-      ch[j] = L[k][0] * c[0] + L[k][1] * c[1] + L[k][2] * c[2] + L[k][3] * c[3];
+      ch[j] = f.L[k][0] * c[0] + f.L[k][1] * c[1] + f.L[k][2] * c[2] + f.L[k][3] * c[3];
     }
-  };
-  double QL[4], QR[4];
-  for (int k = 0; k < 4; k++) {
-    double ch[6];
-    project(k, ch);
     double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
     double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
     JS a, b;
     a.fwd(ql);
     b.fwd(qr);
-    QL[k] = a.out;
-    QR[k] = b.out;
+    f.QL[k] = a.out;
+    f.QR[k] = b.out;
   }
-  double cL[4], cR[4];
   for (int i = 0; i < 4; i++) {
-    cL[i] = R[i][0] * QL[0] + R[i][1] * QL[1] + R[i][2] * QL[2] + R[i][3] * QL[3];
-    cR[i] = R[i][0] * QR[0] + R[i][1] * QR[1] + R[i][2] * QR[2] + R[i][3] * QR[3];
+    f.cL[i] = f.R[i][0] * f.QL[0] + f.R[i][1] * f.QL[1] + f.R[i][2] * f.QL[2] + f.R[i][3] * f.QL[3];
+    f.cR[i] = f.R[i][0] * f.QR[0] + f.R[i][1] * f.QR[1] + f.R[i][2] * f.QR[2] + f.R[i][3] * f.QR[3];
   }
-  double oL[4], oR[4];
-  c2p(cL, oL);
-  c2p(cR, oR);
-  const bool fbL = oL[0] < 0 || oL[3] < 0, fbR = oR[0] < 0 || oR[3] < 0;
-  if (fbL)
-    for (int v = 0; v < 4; v++) oL[v] = pl[v];
-  if (fbR) {
-    oR[0] = pr[0];
-    oR[1] = pl[1];
-    oR[2] = pl[2];
-    oR[3] = pr[3];
+  c2p(f.cL, f.oL);
+  c2p(f.cR, f.oR);
+  f.fbL = f.oL[0] < 0 || f.oL[3] < 0;
+  f.fbR = f.oR[0] < 0 || f.oR[3] < 0;
+  if (f.fbL)
+    for (int v = 0; v < 4; v++) f.oL[v] = pl[v];
+  if (f.fbR) {
+    f.oR[0] = pr[0];
+    f.oR[1] = pl[1];
+    f.oR[2] = pl[2];
+    f.oR[3] = pr[3];
   }
+  f.hb = -1;
+}
+
+// Reverse pass of the interface flux given its forward record: lw[j][v] +=
+// (dF / dw[j][v])^T lF.  The per-wave WENO intermediates and projections are
+// recomputed rather than held (register-lean for device use).
+template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF,
+                                                             double lw[6][4]) {
   double loL[4] = {0, 0, 0, 0}, loR[4] = {0, 0, 0, 0};
   {
     double Fd[4];
-    hllc<AX>(oL, oR, Fd, loL, loR, lF);
+    hllc<AX>(f.oL, f.oR, Fd, loL, loR, lF, f.hb);
   }
   double lcL[4] = {0, 0, 0, 0}, lcR[4] = {0, 0, 0, 0};
-  if (fbL) {
+  if (f.fbL) {
     for (int v = 0; v < 4; v++) lw[2][v] += loL[v];
   } else {
-    c2p_vjp(cL, loL, lcL);
+    c2p_vjp(f.cL, loL, lcL);
   }
-  if (fbR) {
+  if (f.fbR) {
     lw[3][0] += loR[0];
     lw[2][1] += loR[1];
     lw[2][2] += loR[2];
     lw[3][3] += loR[3];
   } else {
-    c2p_vjp(cR, loR, lcR);
+    c2p_vjp(f.cR, loR, lcR);
   }
   double gR[4][4], gL[4][4];
   double lQL[4], lQR[4];
   for (int k = 0; k < 4; k++) {
-    lQL[k] = R[0][k] * lcL[0] + R[1][k] * lcL[1] + R[2][k] * lcL[2] + R[3][k] * lcL[3];
-    lQR[k] = R[0][k] * lcR[0] + R[1][k] * lcR[1] + R[2][k] * lcR[2] + R[3][k] * lcR[3];
-    for (int i = 0; i < 4; i++) gR[i][k] = QL[k] * lcL[i] + QR[k] * lcR[i];
+    lQL[k] = f.R[0][k] * lcL[0] + f.R[1][k] * lcL[1] + f.R[2][k] * lcL[2] + f.R[3][k] * lcL[3];
+    lQR[k] = f.R[0][k] * lcR[0] + f.R[1][k] * lcR[1] + f.R[2][k] * lcR[2] + f.R[3][k] * lcR[3];
+    for (int i = 0; i < 4; i++) gR[i][k] = f.QL[k] * lcL[i] + f.QR[k] * lcR[i];
   }
   double lc[6][4];
   for (int j = 0; j < 6; j++)
     for (int v = 0; v < 4; v++) lc[j][v] = 0.0;
   for (int k = 0; k < 4; k++) {
     double ch[6];
-    project(k, ch);
+    for (int j = 0; j < 6; j++) {
+      double p[4], c[4];
+      for (int v = 0; v < 4; v++) p[v] = w(j, v);
+      p2c(p, c);
+      ch[j] = f.L[k][0] * c[0] + f.L[k][1] * c[1] + f.L[k][2] * c[2] + f.L[k][3] * c[3];
+    }
     double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
     double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
     double gl[5] = {0, 0, 0, 0, 0}, gr[5] = {0, 0, 0, 0, 0};
@@ -708,17 +758,31 @@ template <int AX, class W> CFD_HD inline void interface_vjp_lean(const W& w, con
       for (int v = 0; v < 4; v++) p[v] = w(j, v);
       p2c(p, c);
       for (int v = 0; v < 4; v++) {
-        lc[j][v] += lch[j] * L[k][v];
+        lc[j][v] += lch[j] * f.L[k][v];
         gL[k][v] += lch[j] * c[v];
       }
     }
   }
+  double pl[4], pr[4];
+  for (int v = 0; v < 4; v++) {
+    pl[v] = w(2, v);
+    pr[v] = w(3, v);
+  }
+  Roe2<AX> es;
+  es.fwd(pl, pr);
   es.bwd(pl, pr, gL, gR, lw[2], lw[3]);
   for (int j = 0; j < 6; j++) {
     double p[4];
     for (int v = 0; v < 4; v++) p[v] = w(j, v);
     p2c_vjp(p, lc[j], lw[j]);
   }
+}
+
+// Same derivative as interface<AX>(w, F, lw, lF), from the two passes above.
+template <int AX, class W> CFD_HD inline void interface_vjp_lean(const W& w, const double* lF, double lw[6][4]) {
+  IfaceFwd f;
+  iface_forward<AX>(w, f);
+  iface_backward<AX>(w, f, lF, lw);
 }
 
 }  // namespace cfdadj

@@ -88,6 +88,7 @@ EXPORTS = [
     "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf", "cfd_autotune", "cfd_launch_plan", "cfd_set_launch_plan",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_state", "cfd_batch_get_state", "cfd_batch_steps_cfl",
     "cfd_batch_times", "cfd_batch_autotune", "cfd_batch_synchronize",
+    "cfd_rhs_vjp", "cfd_adjoint_forward", "cfd_adjoint_backward",
 ]
 
 
@@ -140,6 +141,9 @@ def load_library():
     lib.cfd_batch_times.argtypes = [P, pd]
     lib.cfd_batch_autotune.argtypes = [P, ctypes.c_int]
     lib.cfd_batch_synchronize.argtypes = [P]
+    lib.cfd_rhs_vjp.argtypes = [P, pd, pd]
+    lib.cfd_adjoint_forward.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    lib.cfd_adjoint_backward.argtypes = [P, pd, pd]
     _lib = lib
     return lib
 
@@ -285,6 +289,25 @@ class Solver:
         t = (ctypes.c_int * 3)(*(list(tiles) + [1] * (3 - len(tiles))))
         g = (ctypes.c_int * 3)(*(list(segs) + [1] * (3 - len(segs))))
         self._check(self.lib.cfd_set_launch_plan(self.h, t, g))
+
+    # ---- discrete adjoint (2D FP64, char-wise WCNS5 + HLLC) ----
+    def rhs_vjp(self, cot):
+        """(d rhs / d U)^T cot at the current state; cot and result AoS [cell][nv]."""
+        cot = np.ascontiguousarray(cot, dtype=np.float64).reshape(self.ncell, self.nv)
+        out = np.empty((self.ncell, self.nv))
+        self._check(self.lib.cfd_rhs_vjp(self.h, _pd(cot), _pd(out)))
+        return out
+
+    def adjoint_forward(self, dt, nsteps, checkpoint_every=20):
+        """nsteps fixed-dt RK3 steps, keeping checkpoints for adjoint_backward."""
+        self._check(self.lib.cfd_adjoint_forward(self.h, float(dt), int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, cot_final):
+        """(d U_end / d U_0)^T cot_final for the last adjoint_forward."""
+        cot = np.ascontiguousarray(cot_final, dtype=np.float64).reshape(self.ncell, self.nv)
+        out = np.empty((self.ncell, self.nv))
+        self._check(self.lib.cfd_adjoint_backward(self.h, _pd(cot), _pd(out)))
+        return out
 
     def device_buffers(self):
         s, r = ctypes.c_void_p(), ctypes.c_void_p()

@@ -190,6 +190,28 @@ int cfd_batch_times(cfd_batch* b, double* t_out);
 int cfd_batch_autotune(cfd_batch* b, int reps);
 int cfd_batch_synchronize(cfd_batch* b);
 
+/* ---- discrete adjoint -----------------------------------------------------
+ * Reverse-mode derivative of the discrete RHS and of fixed-dt SSP-RK3 steps,
+ * hand-written (the reference has no adjoint; the map differentiated is
+ * SpDistributor::rhsSolve, zone_FD/src/src/sp_distributor.cpp:9-78, with
+ * BlockSolver::RK3_SSP, zone_FD/src/src/blockSolver.cpp:43-87).  Covers 2D,
+ * FP64, characteristic-wise WCNS5 (WENO-JS) + HLLC, any boundary codes;
+ * other configurations return CFD_E_UNSUPP.  Branches (HLLC wave pattern,
+ * pressure clipping, positivity fallback) take the derivative of the branch
+ * the forward evaluation takes.  Arrays are host AoS [cell][4]. */
+
+/* grad = (d rhs / d U)^T cot at the current state. */
+int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad);
+
+/* Advances the current state by nsteps RK3 steps of the fixed dt, keeping the
+ * state every checkpoint_every steps on the device for cfd_adjoint_backward. */
+int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_every);
+
+/* Pulls the cotangent of the state after the last cfd_adjoint_forward back to
+ * its initial state: grad_initial = (d U(t_end) / d U(0))^T cot_final.  Each
+ * checkpoint segment's stage states are recomputed on the device. */
+int cfd_adjoint_backward(cfd_handle* h, const double* cot_final, double* grad_initial);
+
 /* Device pointer of the current state (structure of arrays, nv planes of
  * n[0]*n[1]*n[2] doubles) and of the rhs buffer. */
 int cfd_device_buffers(cfd_handle* h, double** state, double** rhs);

@@ -68,6 +68,8 @@ def lib():
         _lib.oracle_recon.argtypes = [i, i, i, pd, i, pd, pd]
         _lib.oracle_accuracy_rhs.argtypes = [i, i, pd, pd, pd, d, pd]
         _lib.oracle_max_threads.restype = i
+        _lib.oracle_dual_rhs.argtypes = [pg, pd, pd, pd, pd, i]
+        _lib.oracle_dual_rk3.argtypes = [pg, pd, pd, d, i, i]
     return _lib
 
 
@@ -118,6 +120,27 @@ def rk3(grid, cons, dt=0.0, cfl=0.0, nsteps=1, nthreads=None):
                           nthreads or nthreads_default(), ctypes.byref(t))
     assert rc == 0
     return c, t.value
+
+
+def dual_rhs(grid, cons, dcons, nthreads=None):
+    """rhs and its directional derivative along dcons (forward-mode duals)."""
+    cons, dcons = _f64(cons), _f64(dcons)
+    assert cons.shape == dcons.shape
+    out, dout = np.empty_like(cons), np.empty_like(cons)
+    rc = lib().oracle_dual_rhs(ctypes.byref(grid), _p(cons), _p(dcons), _p(out), _p(dout),
+                               nthreads or nthreads_default())
+    assert rc == 0
+    return out, dout
+
+
+def dual_rk3(grid, cons, dcons, dt, nsteps, nthreads=None):
+    """nsteps fixed-dt RK3 steps of the state and of its tangent dcons."""
+    c, dc = _f64(cons).copy(), _f64(dcons).copy()
+    assert c.shape == dc.shape
+    rc = lib().oracle_dual_rk3(ctypes.byref(grid), _p(c), _p(dc), float(dt), int(nsteps),
+                               nthreads or nthreads_default())
+    assert rc == 0
+    return c, dc
 
 
 def cfl_dt(grid, cons, cfl, nthreads=None):

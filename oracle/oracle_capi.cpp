@@ -2,6 +2,7 @@
 // test suite, the smoke check and the benchmark's CPU-baseline leg through
 // ctypes; never by the product library).
 #include "oracle.hpp"
+#include "dual.hpp"
 
 #include <omp.h>
 
@@ -175,6 +176,54 @@ void oracle_accuracy_rhs(int scheme, int n, const double* cells, const double* g
     rhs[i] = 0.0;
     rhs[i] += w1 * (at(i + 1) - at(i - 1)) / h + w0 * (F[i + 2] - F[i + 1]) / h + w2 * (F[i + 3] - F[i]) / h;
   }
+}
+
+// ---- forward-mode dual-number build of the RHS and RK3 -----------------------
+// The oracle instantiated on oracle::Dual: values identical to oracle_rhs /
+// oracle_rk3, plus the directional derivative along the tangent dcons.
+static void dual_pack(long n, const double* x, const double* dx, std::vector<Dual>& out) {
+  out.resize(n);
+  for (long i = 0; i < n; i++) out[i] = Dual(x[i], dx[i]);
+}
+
+int oracle_dual_rhs(const oracle_grid* c, const double* cons, const double* dcons, double* rhs, double* drhs,
+                    int nthreads) {
+  Grid g = to_grid(c);
+  const long N = (long)g.n[0] * g.n[1] * g.n[2] * (g.dim + 2);
+  std::vector<Dual> u, prim(N), r(N);
+  dual_pack(N, cons, dcons, u);
+  switch (g.dim) {
+  case 1: rhs_eval<Dual, 1>(g, u.data(), prim.data(), r.data(), nthreads); break;
+  case 2: rhs_eval<Dual, 2>(g, u.data(), prim.data(), r.data(), nthreads); break;
+  case 3: rhs_eval<Dual, 3>(g, u.data(), prim.data(), r.data(), nthreads); break;
+  default: return -1;
+  }
+  for (long i = 0; i < N; i++) {
+    rhs[i] = r[i].v;
+    drhs[i] = r[i].d;
+  }
+  return 0;
+}
+
+// nsteps SSP-RK3 steps with the fixed dt, state and tangent advanced in place.
+int oracle_dual_rk3(const oracle_grid* c, double* cons, double* dcons, double dt, int nsteps, int nthreads) {
+  Grid g = to_grid(c);
+  const long N = (long)g.n[0] * g.n[1] * g.n[2] * (g.dim + 2);
+  std::vector<Dual> u;
+  dual_pack(N, cons, dcons, u);
+  for (int s = 0; s < nsteps; s++) {
+    switch (g.dim) {
+    case 1: rk3_step<Dual, 1>(g, u.data(), dt, nthreads); break;
+    case 2: rk3_step<Dual, 2>(g, u.data(), dt, nthreads); break;
+    case 3: rk3_step<Dual, 3>(g, u.data(), dt, nthreads); break;
+    default: return -1;
+    }
+  }
+  for (long i = 0; i < N; i++) {
+    cons[i] = u[i].v;
+    dcons[i] = u[i].d;
+  }
+  return 0;
 }
 
 }  // extern "C"

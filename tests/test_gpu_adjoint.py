@@ -1,0 +1,185 @@
+"""GPU adjoint (cfd_rhs_vjp, cfd_adjoint_forward / cfd_adjoint_backward).
+
+* The RHS vector-Jacobian product on the GPU against the host statement of
+  the same product (tie-free states, every boundary family) and against the
+  dual-number oracle on the shock workload, where pressure equilibrium ties
+  the HLLC predicates and only the forward's own branches give the oracle's
+  derivative.
+* The checkpointed RK3 adjoint against the dual-number oracle RK3 on 32
+  seeded directions of the initial density, independence of the checkpoint
+  interval (bit for bit), and the torch.autograd harness.
+* Central finite differences of the GPU forward on the full workload
+  (512^2, 400 steps, checkpoint every 20).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cfd_inwsl_amd import BC, FluxMethod, InterMethod, Recon, Solver, adjoint, cases
+from oracle import oracle as O
+from tests.helpers import oracle_grid, random_state
+from tests.test_adjoint_host import BC_SETS, host_vjp, lib  # noqa: F401  (fixture)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _solver(n, h, bc, bc_prim=None):
+    return Solver(2, n, h, InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, list(bc) + [BC.OUTFLOW] * 2, bc_prim)
+
+
+@pytest.mark.parametrize("name", list(BC_SETS))
+def test_rhs_vjp_matches_host(lib, name):  # noqa: F811
+    bc, bc_prim = BC_SETS[name]
+    for n, seed in (((40, 36), 3), ((130, 70), 4)):
+        h = (0.07, 0.09)
+        cons = random_state(2, n, seed=seed, jumps=True)
+        mu = np.random.default_rng(seed).normal(size=cons.shape)
+        s = _solver(n, h, bc, bc_prim)
+        s.set_state(cons)
+        g = s.rhs_vjp(mu)
+        s.close()
+        ref = host_vjp(lib, n, h, bc, bc_prim if bc_prim is not None else np.zeros((6, 5)), cons, mu)
+        err = np.abs(g - ref).max() / np.abs(ref).max()
+        _say(f"rhs_vjp {name} {n}: max rel {err:.2e}")
+        assert err < 1e-12
+
+
+def _shock(n, amp=1e-2):
+    case = cases.adjoint_shock(n, amp=amp)
+    U0 = cases.adjoint_state(case)
+    return case, U0
+
+
+def _drho_to_dU(case, drho):
+    """Tangent of the conservative state for a density tangent (u, v, p fixed)."""
+    f = case["rho_factor"] * drho
+    u, v = case["u"], case["v"]
+    return np.stack([f, f * u, f * v, 0.5 * f * (u * u + v * v)], -1).reshape(-1, 4)
+
+
+def _grad_rho(case, gU):
+    n = case["rho0"].shape[0]
+    g = gU.reshape(n, n, 4)
+    u, v = case["u"], case["v"]
+    return case["rho_factor"] * (g[..., 0] + g[..., 1] * u + g[..., 2] * v + g[..., 3] * 0.5 * (u * u + v * v))
+
+
+def _directions(n, count, seed):
+    rng = np.random.default_rng(seed)
+    return [cases._random_phase_field_2d(n, 8, rng) for _ in range(count)]
+
+
+def test_rhs_vjp_on_tied_predicates_matches_dual_oracle():
+    n = 48
+    case, U0 = _shock(n)
+    h = case["h"]
+    # advance a little so that the shock is resolved by the scheme itself
+    g = oracle_grid(2, (n, n), h, InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 4)
+    dt = cases.cfl_dt_2d(U0, n, h)
+    U, _ = O.rk3(g, U0, dt=dt, nsteps=5)
+    s = _solver((n, n), (h, h), [BC.PERIODIC] * 4)
+    s.set_state(U)
+    rng = np.random.default_rng(8)
+    worst = 0.0
+    for _ in range(6):
+        mu = rng.normal(size=U.shape)
+        d = rng.normal(size=U.shape)
+        gt = s.rhs_vjp(mu)
+        _, jd = O.dual_rhs(g, U, d)
+        lhs, rhs = float((gt * d).sum()), float((mu * jd).sum())
+        worst = max(worst, abs(lhs - rhs) / np.abs(gt * d).sum())
+    s.close()
+    _say(f"<vjp, d> vs <mu, jvp> on the shock state: worst {worst:.2e}")
+    assert worst < 1e-12
+
+
+def _gpu_gradient(case, U0, dt, nsteps, every):
+    n = case["rho0"].shape[0]
+    s = _solver((n, n), (case["h"], case["h"]), [BC.PERIODIC] * 4)
+    s.set_state(U0)
+    s.adjoint_forward(dt, nsteps, every)
+    UT = s.get_state()
+    J, gJ = adjoint.window_pressure_objective(UT, case)
+    gU = s.adjoint_backward(gJ)
+    s.close()
+    return J, UT, gU
+
+
+def test_rk3_adjoint_matches_dual_oracle_32_directions():
+    n, nsteps = 64, 60
+    case, U0 = _shock(n)
+    dt = cases.cfl_dt_2d(U0, n, case["h"])
+    J, UT, gU = _gpu_gradient(case, U0, dt, nsteps, 20)
+    g = oracle_grid(2, (n, n), case["h"], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 4)
+    grho = _grad_rho(case, gU)
+    _, gJ = adjoint.window_pressure_objective(UT, case)
+    worst = 0.0
+    for d in _directions(n, 32, 77):
+        uT, duT = O.dual_rk3(g, U0, _drho_to_dU(case, d), dt, nsteps)
+        assert np.array_equal(uT, UT)  # the GPU forward is the oracle, bit for bit
+        jvp = float((gJ * duT).sum())
+        adj = float((grho * d).sum())
+        worst = max(worst, abs(adj - jvp) / abs(jvp))
+    _say(f"adjoint vs dual oracle, {n}^2 x {nsteps} steps, 32 directions: worst rel {worst:.2e}")
+    assert worst < 1e-9
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    n, nsteps = 48, 25
+    case, U0 = _shock(n)
+    dt = cases.cfl_dt_2d(U0, n, case["h"])
+    ref = _gpu_gradient(case, U0, dt, nsteps, 1)[2]
+    for every in (7, 25, 40):
+        assert np.array_equal(_gpu_gradient(case, U0, dt, nsteps, every)[2], ref), every
+
+
+def test_torch_harness_equals_direct_call():
+    torch = pytest.importorskip("torch")
+    n, nsteps = 48, 12
+    case, U0 = _shock(n)
+    dt = cases.cfl_dt_2d(U0, n, case["h"])
+    s = _solver((n, n), (case["h"], case["h"]), [BC.PERIODIC] * 4)
+    rho0 = torch.tensor(case["rho0"], dtype=torch.float64, requires_grad=True)
+    u0 = adjoint.initial_state_torch(case, rho0)
+    uT = adjoint.rk3_function().apply(u0, s, dt, nsteps, 5)
+    J = adjoint.window_pressure_objective_torch(uT, case)
+    J.backward()
+    s.close()
+    Jd, _, gU = _gpu_gradient(case, U0, dt, nsteps, 5)
+    assert abs(float(J) - Jd) <= 1e-14 * abs(Jd)
+    np.testing.assert_allclose(rho0.grad.numpy(), _grad_rho(case, gU), rtol=1e-12, atol=1e-14 * np.abs(gU).max())
+
+
+def test_full_workload_against_central_differences():
+    """512^2, 400 steps, checkpoint every 20: adjoint vs central FD of the GPU
+    forward on 8 seeded density directions."""
+    n, nsteps = 512, 400
+    case, U0 = _shock(n)
+    dt = cases.cfl_dt_2d(U0, n, case["h"])
+    J, UT, gU = _gpu_gradient(case, U0, dt, nsteps, 20)
+    grho = _grad_rho(case, gU)
+    s = _solver((n, n), (case["h"], case["h"]), [BC.PERIODIC] * 4)
+
+    def Jof(rho):
+        s.set_state(cases.adjoint_state(case, rho))
+        for _ in range(nsteps):
+            s.rk3_step(dt)
+        return adjoint.window_pressure_objective(s.get_state(), case)[0]
+
+    worst = 0.0
+    eps = 1e-5
+    for d in _directions(n, 8, 99):
+        fd = (Jof(case["rho0"] + eps * d) - Jof(case["rho0"] - eps * d)) / (2 * eps)
+        adj = float((grho * d).sum())
+        rel = abs(adj - fd) / abs(fd)
+        _say(f"  FD {fd:.12e} adjoint {adj:.12e} rel {rel:.2e}")
+        worst = max(worst, rel)
+    s.close()
+    _say(f"adjoint vs central FD, 512^2 x 400 steps: worst rel {worst:.2e}")
+    assert worst < 1e-6
