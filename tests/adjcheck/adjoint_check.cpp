@@ -1,5 +1,6 @@
-// Host build of the adjoint point kernels (adjoint_point.hpp) with C entry
-// points, for unit tests against finite differences on the CPU.
+// Host build of the adjoint point kernels (cfd_inwsl_amd/csrc/adjoint_point.hpp)
+// with C entry points, for CPU tests against finite differences and the
+// dual-number oracle (test infrastructure only).
 #include "adjoint_point.hpp"
 
 #include <vector>

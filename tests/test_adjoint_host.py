@@ -20,15 +20,16 @@ from oracle import oracle as O
 from tests.helpers import oracle_grid, random_state
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "cfd_inwsl_amd", "_build", "libadjcheck.so")
+LIB = os.path.join(ROOT, "tests", "adjcheck", "libadjcheck.so")
 PD = ctypes.POINTER(ctypes.c_double)
 PI = ctypes.POINTER(ctypes.c_int)
 
 
 @pytest.fixture(scope="module")
 def lib():
-    import subprocess
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "cfd_inwsl_amd", "csrc"), "adjcheck"], check=True)
+    if not os.path.exists(LIB):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "adjcheck")], check=True)
     L = ctypes.CDLL(LIB)
     L.adj_rhs_vjp.argtypes = [PI, PD, PI, PD, PD, PD, PD]
     L.adj_interface_vjp.argtypes = [ctypes.c_int, PD, PD, PD]

@@ -11,14 +11,14 @@ import pytest
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "cfd_inwsl_amd", "_build", "libadjcheck.so")
+LIB = os.path.join(ROOT, "tests", "adjcheck", "libadjcheck.so")
 
 
 @pytest.fixture(scope="module")
 def lib():
     if not os.path.exists(LIB):
         import subprocess
-        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "cfd_inwsl_amd", "csrc"), "adjcheck"], check=True)
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "adjcheck")], check=True)
     L = ctypes.CDLL(LIB)
     pd = ctypes.POINTER(ctypes.c_double)
     L.adj_interface_fwd.argtypes = [ctypes.c_int, pd, pd]
