@@ -131,6 +131,68 @@ def bench_fp32(steps_err=100):
     return out
 
 
+def bench_adjoint(n=512, nsteps=400, every=20, ndir=32):
+    """Adjoint workload: 512^2 periodic, WCNS5-JS char-wise + HLLC, Mach-1.5
+    shock, 400 fixed-dt RK3 steps, J = int p^2 over a 64^2 window.  Times the
+    checkpointed forward and the backward (stage recomputation + three RHS
+    vector-Jacobian products per step), and checks dJ/drho0 against central
+    differences of the GPU forward and, when present, the dual-number oracle
+    values in tests/golden/adjoint_dual_512.json on the same 32 directions."""
+    from cfd_inwsl_amd import adjoint
+    case = cases.adjoint_shock(n)
+    U0 = cases.adjoint_state(case)
+    dt = cases.cfl_dt_2d(U0, n, case["h"])
+    s = Solver(2, (n, n), (case["h"], case["h"]), InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR,
+               [BC.PERIODIC] * 6)
+    s.set_state(U0)
+    s.adjoint_forward(dt, every, every)
+    s.adjoint_backward(np.zeros_like(U0))
+    s.set_state(U0)
+    t0 = time.perf_counter()
+    s.adjoint_forward(dt, nsteps, every)
+    t1 = time.perf_counter()
+    UT = s.get_state()
+    J, gJ = adjoint.window_pressure_objective(UT, case)
+    t2 = time.perf_counter()
+    gU = s.adjoint_backward(gJ)
+    t3 = time.perf_counter()
+    g = gU.reshape(n, n, 4)
+    u, v = case["u"], case["v"]
+    grho = case["rho_factor"] * (g[..., 0] + g[..., 1] * u + g[..., 2] * v + g[..., 3] * 0.5 * (u * u + v * v))
+    fwd_ms, bwd_ms = (t1 - t0) * 1e3, (t3 - t2) * 1e3
+    log(f"[bench] adjoint: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, ratio {bwd_ms / fwd_ms:.2f}")
+
+    def Jof(rho):
+        s.set_state(cases.adjoint_state(case, rho))
+        for _ in range(nsteps):
+            s.rk3_step(dt)
+        return adjoint.window_pressure_objective(s.get_state(), case)[0]
+
+    rng = np.random.default_rng(99)
+    dirs = [cases._random_phase_field_2d(n, 8, rng) for _ in range(ndir)]
+    adj = [float((grho * d).sum()) for d in dirs]
+    eps = 1e-7
+    fd_rel = []
+    for d, a in zip(dirs, adj):
+        fd = (Jof(case["rho0"] + eps * d) - Jof(case["rho0"] - eps * d)) / (2 * eps)
+        fd_rel.append(abs(a - fd) / abs(fd))
+    s.close()
+    log(f"[bench] adjoint vs central FD (eps {eps}): median {np.median(fd_rel):.2e} max {max(fd_rel):.2e}")
+    out = {"workload": f"{n}^2 periodic, WCNS5-JS char-wise + HLLC, Mach-1.5 shock, {nsteps} fixed-dt RK3 steps, "
+                       f"J = int p^2 over 64^2 window, checkpoint every {every}",
+           "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+           "adjoint_over_forward": round(bwd_ms / fwd_ms, 3), "J": J,
+           "fd": {"eps": eps, "directions": ndir, "median_rel": float(np.median(fd_rel)), "max_rel": max(fd_rel)}}
+    golden = os.path.join(ROOT, "tests", "golden", "adjoint_dual_512.json")
+    if os.path.exists(golden):
+        ref = json.load(open(golden))
+        k = len(ref["jvp"])
+        rel = [abs(a - b) / abs(b) for a, b in zip(adj[:k], ref["jvp"])]
+        out["dual_oracle"] = {"directions": k, "max_rel": max(rel), "J_rel": abs(J - ref["J"]) / abs(ref["J"])}
+        log(f"[bench] adjoint vs dual oracle ({k} directions): max {max(rel):.2e}")
+    return out
+
+
 def cpu_baseline(seconds_target=15.0):
     """In-repo oracle (OpenMP over pencils, like SpDistributor::rhsSolve) timed
     on the host cores on a bounded sample of the same workload."""
@@ -161,6 +223,7 @@ def main():
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D 256^3 secondary figure")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-zones figure")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage figures")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -203,6 +266,11 @@ def main():
         log("[bench] FP32 storage variant")
         fp32 = bench_fp32()
 
+    adj = None
+    if not args.no_adjoint:
+        log("[bench] adjoint")
+        adj = bench_adjoint()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -230,6 +298,7 @@ def main():
         "secondary": secondary,
         "batch": batch,
         "fp32": fp32,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

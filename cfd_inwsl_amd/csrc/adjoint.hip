@@ -45,13 +45,37 @@
 
 namespace {
 
-constexpr int ADJ_CA = 16;  // interfaces per chunk
-constexpr int ADJ_CP = 16;  // pencils per workgroup
+// Workgroup tile of the VJP sweep per direction: CA interfaces per chunk x CP
+// pencils (256 lanes), and the waves per SIMD it is compiled for.
+#ifndef CFD_ADJ_CA0
+#define CFD_ADJ_CA0 32
+#define CFD_ADJ_CP0 8
+#endif
+#ifndef CFD_ADJ_CA1
+#define CFD_ADJ_CA1 16
+#define CFD_ADJ_CP1 16
+#endif
+#ifndef CFD_ADJ_OCC
+#define CFD_ADJ_OCC 1
+#endif
+template <int AX> struct VjpTile {
+  static constexpr int CA = AX == 0 ? CFD_ADJ_CA0 : CFD_ADJ_CA1;
+  static constexpr int CP = AX == 0 ? CFD_ADJ_CP0 : CFD_ADJ_CP1;
+};
+
+// OUT = base + coef * (d rhs_AX / d U)^T MU, with base = OUT, or (init) the
+// combination sum_i b_i X_i of up to three SoA arrays (null X_i: absent).
+struct VjpInit {
+  const double* X[3];
+  double b[3];
+};
 
 struct VjpArgs {
   const double* U;   // linearisation state, SoA planes of N
   const double* MU;  // cotangent of the rhs, SoA
-  double* OUT;       // += coef * (d rhs_AX / d U)^T MU, SoA
+  double* OUT;       // result, SoA (may alias an X of init)
+  VjpInit init;
+  int use_init;
   double* GH;        // ghost cotangents [pencil][8][4]
   double coef;
   long N;
@@ -123,7 +147,7 @@ template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(c
 }
 
 template <int AX, int CA, int CP>
-__global__ __launch_bounds__(CA* CP) void k_rhs_vjp(VjpArgs a) {
+__global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
   __shared__ double wc[4 * RC * CP];  // primitive state
   __shared__ double mc[4 * RC * CP];  // coef * cotangent
@@ -149,6 +173,7 @@ __global__ __launch_bounds__(CA* CP) void k_rhs_vjp(VjpArgs a) {
   const int e1 = c1 == n ? n + 4 : c1;
   const int kmin = max(e0 - 2, -1), kmax = min(e1 + 2, n + 1);
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
+  const double ih = 1.0 / a.h;
 
   auto load = [&](int c) {
     const int s = slot_of(c);
@@ -204,7 +229,7 @@ __global__ __launch_bounds__(CA* CP) void k_rhs_vjp(VjpArgs a) {
       for (int v = 0; v < 4; v++) lw[j][v] = 0.0;
     if (act) {
       double lG[4];
-      for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) / a.h;
+      for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) * ih;
       Cells cells{wc, k, lp};
       cfdadj::IfaceFwd f;
       iface_forward_exact<AX>(cells, f);
@@ -227,7 +252,7 @@ __global__ __launch_bounds__(CA* CP) void k_rhs_vjp(VjpArgs a) {
           double p[4], lf[4];
           for (int v = 0; v < 4; v++) {
             p[v] = wc[idx(v, s, lp)];
-            lf[v] = w1 * (M(c - 1, v) - M(c + 1, v)) / a.h;
+            lf[v] = w1 * (M(c - 1, v) - M(c + 1, v)) * ih;
           }
           cfdadj::node_flux_vjp<AX>(p, lf, lwc);
         }
@@ -236,7 +261,18 @@ __global__ __launch_bounds__(CA* CP) void k_rhs_vjp(VjpArgs a) {
           double u[4], g[4] = {0.0, 0.0, 0.0, 0.0};
           for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
           cfdadj::c2p_vjp(u, lwc, g);
-          for (int v = 0; v < 4; v++) a.OUT[v * a.N + i] += g[v];
+          for (int v = 0; v < 4; v++) {
+            const long e = v * a.N + i;
+            double base;
+            if (a.use_init) {
+              base = 0.0;
+              for (int t = 0; t < 3; t++)
+                if (a.init.X[t]) base += a.init.b[t] * a.init.X[t][e];
+            } else {
+              base = a.OUT[e];
+            }
+            a.OUT[e] = base + g[v];
+          }
         } else {
           const int gs = c < 0 ? c + 4 : c - n + 4;  // 0..7
           for (int v = 0; v < 4; v++) a.GH[(gp * 8 + gs) * 4 + v] = lwc[v];
@@ -247,45 +283,45 @@ __global__ __launch_bounds__(CA* CP) void k_rhs_vjp(VjpArgs a) {
   }
 }
 
-// Ghost cotangents onto their source cells, one thread per pencil, ghosts in
-// a fixed order (deterministic sums).
+// Ghost cotangents onto their source cells.  One thread per (pencil, ghost):
+// the first ghost (in slot order) of each source cell sums every ghost that
+// copies that cell, in slot order, and applies the transpose of consToPrim
+// once -- distinct threads write distinct cells and the sums are
+// deterministic.
 template <int AX> __global__ __launch_bounds__(256) void k_ghost_fold(VjpArgs a) {
-  const long gp = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long gp = t >> 3;
+  const int g = (int)(t & 7);
   const int nx = a.n[0], ny = a.n[1];
   const int n = a.n[AX];
   const long npencil = AX == 0 ? ny : nx;
   if (gp >= npencil) return;
-  const long sa = AX == 0 ? 1 : nx;
-  const long base = AX == 0 ? gp * nx : gp;
-  for (int g = 0; g < 8; g++) {
-    const int c = g < 4 ? g - 4 : n + g - 4;
-    const cfdadj::GhostSrc gs = cfdadj::ghost_source(g < 4 ? a.bc_lo : a.bc_hi, c, n);
-    if (gs.fixed) continue;
-    double l[4];
-    for (int v = 0; v < 4; v++) l[v] = a.GH[(gp * 8 + g) * 4 + v];
-    if (gs.flip >= 0) l[gs.flip] = -l[gs.flip];
-    const long i = base + (long)gs.src * sa;
-    double u[4], gr[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
-    cfdadj::c2p_vjp(u, l, gr);
-    for (int v = 0; v < 4; v++) a.OUT[v * a.N + i] += gr[v];
+  auto src_of = [&](int q) {
+    return cfdadj::ghost_source(q < 4 ? a.bc_lo : a.bc_hi, q < 4 ? q - 4 : n + q - 4, n);
+  };
+  const cfdadj::GhostSrc gs = src_of(g);
+  if (gs.fixed) return;
+  for (int q = 0; q < g; q++) {
+    const cfdadj::GhostSrc o = src_of(q);
+    if (!o.fixed && o.src == gs.src) return;
   }
+  double l[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = g; q < 8; q++) {
+    const cfdadj::GhostSrc o = src_of(q);
+    if (o.fixed || o.src != gs.src) continue;
+    for (int v = 0; v < 4; v++) {
+      const double x = a.GH[(gp * 8 + q) * 4 + v];
+      l[v] += v == o.flip ? -x : x;
+    }
+  }
+  const long sa = AX == 0 ? 1 : nx;
+  const long i = (AX == 0 ? gp * nx : gp) + (long)gs.src * sa;
+  double u[4], gr[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
+  cfdadj::c2p_vjp(u, l, gr);
+  for (int v = 0; v < 4; v++) a.OUT[v * a.N + i] += gr[v];
 }
 
-__global__ __launch_bounds__(256) void k_fill(double* x, double val, long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = val;
-}
-__global__ __launch_bounds__(256) void k_scale(double* __restrict__ out, const double* __restrict__ in, double s,
-                                               long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    out[i] = s * in[i];
-}
-// l0 = 1/3 l3 + 3/4 l2 + l1 (in place over l3)
-__global__ __launch_bounds__(256) void k_comb0(double* __restrict__ l3, const double* __restrict__ l2,
-                                               const double* __restrict__ l1, long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    l3[i] = 1.0 / 3.0 * l3[i] + 0.75 * l2[i] + l1[i];
-}
 __global__ void k_aos_to_soa4(const double* __restrict__ in, double* __restrict__ out, long N) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N)
@@ -339,6 +375,40 @@ static int adj_supported(cfd_handle* h) {
   return 0;
 }
 
+// Segment length of the VJP sweep along a direction of n cells.  A segment of
+// L cells runs interfaces over L + 5 cells (L + 9 at a pencil end, which also
+// finalises the ghosts), in chunks of CA; lengths m*CA - 5 keep every chunk of
+// an interior segment full.  Cost = lane slots / the fraction of the last
+// round of resident workgroups that is busy (the chip holds 256 * occupancy
+// workgroups of 256 lanes at once).
+static int vjp_plan_seg(int n, long pencil_blocks, int ca, int cp) {
+  const long resident = 256L * CFD_ADJ_OCC;
+  double best = 0.0;
+  int best_seg = n;
+  for (int m = 1; m <= 4096; m++) {
+    int seg = m * ca - 5;
+    if (seg < 4) continue;
+    const bool last = seg >= n;
+    if (last) seg = n;
+    long chunks = 0;
+    for (int c0 = 0; c0 < n; c0 += seg) {
+      const int len = std::min(seg, n - c0);
+      const int extra = 5 + (c0 == 0 ? 2 : 0) + (c0 + len == n ? 2 : 0);
+      chunks += (len + extra + ca - 1) / ca;
+    }
+    const long blocks = pencil_blocks * ((n + seg - 1) / seg);
+    const long rounds = (blocks + resident - 1) / resident;
+    const double busy = (double)blocks / (double)(rounds * resident);
+    const double cost = (double)pencil_blocks * chunks * ca * cp / busy;
+    if (best == 0.0 || cost < best) {
+      best = cost;
+      best_seg = seg;
+    }
+    if (last) break;
+  }
+  return best_seg;
+}
+
 static int adj_ensure(cfd_handle* h) {
   if (int rc = adj_supported(h)) return rc;
   if (h->cfg.n[0] < 8 || h->cfg.n[1] < 8) return cfd_fail(CFD_E_UNSUPP, "the adjoint needs at least 8 cells per direction");
@@ -351,22 +421,19 @@ static int adj_ensure(cfd_handle* h) {
   AHIP(hipMalloc(&s->LB, s->state_bytes));
   AHIP(hipMalloc(&s->LC, s->state_bytes));
   AHIP(hipMalloc(&s->GH, sizeof(double) * np * 8 * 4));
-  // segment length per direction: enough workgroups to fill the chip
   for (int ax = 0; ax < 2; ax++) {
-    const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
-    const long pb = (pencils + ADJ_CP - 1) / ADJ_CP;
-    long nseg = (2048 + pb - 1) / pb;
-    nseg = std::max<long>(1, std::min<long>(nseg, n / 16));
-    s->vjp_seg[ax] = (int)((n + nseg - 1) / nseg);
+    const int ca = ax == 0 ? VjpTile<0>::CA : VjpTile<1>::CA;
+    const int cp = ax == 0 ? VjpTile<0>::CP : VjpTile<1>::CP;
+    s->vjp_seg[ax] = vjp_plan_seg(h->cfg.n[ax], (pencils + cp - 1) / cp, ca, cp);
   }
   return 0;
 }
 
-static const int kFillBlocks = 1024;
-
-// OUT += coef * (d rhs / d U)^T MU at the state U, both directions.
-static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef, double* OUT) {
+// OUT = base + coef * (d rhs / d U)^T MU at the state U, both directions;
+// base = init's combination (null init: the current OUT).
+static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef, double* OUT,
+                   const VjpInit* init) {
   AdjointStore* s = h->adj;
   for (int ax = 0; ax < 2; ax++) {
     VjpArgs a;
@@ -376,6 +443,8 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     a.OUT = OUT;
     a.GH = s->GH;
     a.coef = coef;
+    a.use_init = ax == 0 && init != nullptr;
+    if (a.use_init) a.init = *init;
     a.N = h->N;
     a.n[0] = h->cfg.n[0];
     a.n[1] = h->cfg.n[1];
@@ -389,13 +458,17 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     }
     const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
-    const dim3 grid((unsigned)((pencils + ADJ_CP - 1) / ADJ_CP), (unsigned)((n + a.seg_len - 1) / a.seg_len));
-    const dim3 fold((unsigned)((pencils + 255) / 256));
+    const unsigned nseg = (unsigned)((n + a.seg_len - 1) / a.seg_len);
+    const dim3 fold((unsigned)((pencils * 8 + 255) / 256));
     if (ax == 0) {
-      hipLaunchKernelGGL((k_rhs_vjp<0, ADJ_CA, ADJ_CP>), grid, dim3(ADJ_CA * ADJ_CP), 0, h->stream, a);
+      constexpr int CA = VjpTile<0>::CA, CP = VjpTile<0>::CP;
+      const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
+      hipLaunchKernelGGL((k_rhs_vjp<0, CA, CP>), grid, dim3(CA * CP), 0, h->stream, a);
       hipLaunchKernelGGL(k_ghost_fold<0>, fold, dim3(256), 0, h->stream, a);
     } else {
-      hipLaunchKernelGGL((k_rhs_vjp<1, ADJ_CA, ADJ_CP>), grid, dim3(ADJ_CA * ADJ_CP), 0, h->stream, a);
+      constexpr int CA = VjpTile<1>::CA, CP = VjpTile<1>::CP;
+      const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
+      hipLaunchKernelGGL((k_rhs_vjp<1, CA, CP>), grid, dim3(CA * CP), 0, h->stream, a);
       hipLaunchKernelGGL(k_ghost_fold<1>, fold, dim3(256), 0, h->stream, a);
     }
     AHIP(hipGetLastError());
@@ -406,15 +479,13 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
 // One RK3 step backwards: s->LA holds l3 on entry and l0 on exit.
 static int adj_step_back(cfd_handle* h, const double* u0, const double* u1, const double* u2, double dt) {
   AdjointStore* s = h->adj;
-  const long n = h->N * 4;
   int rc;
-  hipLaunchKernelGGL(k_scale, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LB, s->LA, 2.0 / 3.0, n);
-  if ((rc = adj_vjp(h, u2, s->LA, -2.0 / 3.0 * dt, s->LB))) return rc;
-  hipLaunchKernelGGL(k_scale, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LC, s->LB, 0.25, n);
-  if ((rc = adj_vjp(h, u1, s->LB, -0.25 * dt, s->LC))) return rc;
-  hipLaunchKernelGGL(k_comb0, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LA, s->LB, s->LC, n);
-  if ((rc = adj_vjp(h, u0, s->LC, -dt, s->LA))) return rc;
-  AHIP(hipGetLastError());
+  const VjpInit i2 = {{s->LA, nullptr, nullptr}, {2.0 / 3.0, 0.0, 0.0}};
+  if ((rc = adj_vjp(h, u2, s->LA, -2.0 / 3.0 * dt, s->LB, &i2))) return rc;
+  const VjpInit i1 = {{s->LB, nullptr, nullptr}, {0.25, 0.0, 0.0}};
+  if ((rc = adj_vjp(h, u1, s->LB, -0.25 * dt, s->LC, &i1))) return rc;
+  const VjpInit i0 = {{s->LA, s->LB, s->LC}, {1.0 / 3.0, 0.75, 1.0}};
+  if ((rc = adj_vjp(h, u0, s->LC, -dt, s->LA, &i0))) return rc;
   return 0;
 }
 
@@ -427,8 +498,8 @@ int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad) {
   const unsigned nb = (unsigned)((h->N + 255) / 256);
   AHIP(hipMemcpyAsync(s->LC, cot, s->state_bytes, hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_aos_to_soa4, dim3(nb), dim3(256), 0, h->stream, s->LC, s->LA, h->N);
-  hipLaunchKernelGGL(k_fill, dim3(kFillBlocks), dim3(256), 0, h->stream, s->LB, 0.0, h->N * 4);
-  if (int rc = adj_vjp(h, h->U0, s->LA, 1.0, s->LB)) return rc;
+  const VjpInit zero = {{nullptr, nullptr, nullptr}, {0.0, 0.0, 0.0}};
+  if (int rc = adj_vjp(h, h->U0, s->LA, 1.0, s->LB, &zero)) return rc;
   hipLaunchKernelGGL(k_soa_to_aos4, dim3(nb), dim3(256), 0, h->stream, s->LB, s->LC, h->N);
   AHIP(hipGetLastError());
   AHIP(hipMemcpyAsync(grad, s->LC, s->state_bytes, hipMemcpyDeviceToHost, h->stream));

@@ -20,8 +20,10 @@
 
 #if defined(__HIPCC__)
 #define CFD_HD __host__ __device__
+#define CFD_UNROLL _Pragma("unroll")
 #else
 #define CFD_HD
+#define CFD_UNROLL
 #endif
 
 namespace cfdadj {
@@ -29,10 +31,25 @@ namespace cfdadj {
 constexpr double G = 1.4;
 constexpr double G1 = G - 1.0;          // gamma - 1
 constexpr double GG1 = G / (G - 1.0);   // gamma / (gamma - 1)
+constexpr double IG1 = 1.0 / G1;        // 1 / (gamma - 1)
+
+// Reciprocal for the reverse pass, which needs derivatives, not the forward's
+// rounding: on the device the hardware estimate refined by two Newton steps
+// (relative error at the level of one rounding), on the host a division.
+CFD_HD inline double rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+#else
+  return 1.0 / x;
+#endif
+}
 
 // ---- WENO-JS half-node value and its gradient ---------------------------------
 struct JS {
-  double b[3], a[3], u[3], e[3], S, out;
+  double b[3], a[3], u[3], e[3], S, iS, out;
   CFD_HD void fwd(const double* q) {
     double A0 = q[0] - 2.0 * q[1] + q[2], B0 = q[0] - 4.0 * q[1] + 3.0 * q[2];
     double A1 = q[1] - 2.0 * q[2] + q[3], B1 = q[1] - q[3];
@@ -45,15 +62,16 @@ struct JS {
     u[2] = 0.375 * q[2] + 0.75 * q[3] - 0.125 * q[4];
     const double d[3] = {0.0625, 0.625, 0.3125};
     for (int k = 0; k < 3; k++) {
-      e[k] = 1.0 / (1e-6 + b[k]);
+      e[k] = rcp(1e-6 + b[k]);
       a[k] = d[k] * e[k] * e[k];
     }
     S = a[0] + a[1] + a[2];
-    out = (a[0] * u[0] + a[1] * u[1] + a[2] * u[2]) / S;
+    iS = rcp(S);
+    out = (a[0] * u[0] + a[1] * u[1] + a[2] * u[2]) * iS;
   }
   // adds lam * d(out)/dq to g
   CFD_HD void bwd(const double* q, double lam, double* g) const {
-    const double iS = lam / S;
+    const double iS = lam * this->iS;
     double lu[3], lb[3];
     for (int k = 0; k < 3; k++) {
       lu[k] = a[k] * iS;
@@ -85,7 +103,7 @@ CFD_HD inline void p2c(const double* p, double* c) {
   c[0] = p[0];
   c[1] = p[0] * p[1];
   c[2] = p[0] * p[2];
-  c[3] = p[3] / G1 + 0.5 * p[0] * (p[1] * p[1] + p[2] * p[2]);
+  c[3] = p[3] * IG1 + 0.5 * p[0] * (p[1] * p[1] + p[2] * p[2]);
 }
 // g_p += (dc/dp)^T lc
 CFD_HD inline void p2c_vjp(const double* p, const double* lc, double* gp) {
@@ -93,10 +111,10 @@ CFD_HD inline void p2c_vjp(const double* p, const double* lc, double* gp) {
   gp[0] += lc[0] + lc[1] * p[1] + lc[2] * p[2] + lc[3] * ek;
   gp[1] += lc[1] * p[0] + lc[3] * p[0] * p[1];
   gp[2] += lc[2] * p[0] + lc[3] * p[0] * p[2];
-  gp[3] += lc[3] / G1;
+  gp[3] += lc[3] * IG1;
 }
 CFD_HD inline void c2p(const double* c, double* p) {
-  double r = c[0], u = c[1] / r, v = c[2] / r;
+  double r = c[0], ir = rcp(r), u = c[1] * ir, v = c[2] * ir;
   p[0] = r;
   p[1] = u;
   p[2] = v;
@@ -104,7 +122,7 @@ CFD_HD inline void c2p(const double* c, double* p) {
 }
 // g_c += (dp/dc)^T lp
 CFD_HD inline void c2p_vjp(const double* c, const double* lp, double* gc) {
-  double r = c[0], ir = 1.0 / r, u = c[1] * ir, v = c[2] * ir;
+  double r = c[0], ir = rcp(r), u = c[1] * ir, v = c[2] * ir;
   // p1 = c1/r, p2 = c2/r, p3 = G1 (c3 - (c1^2 + c2^2)/(2r))
   gc[0] += lp[0] - lp[1] * u * ir - lp[2] * v * ir + lp[3] * G1 * 0.5 * (u * u + v * v);
   gc[1] += lp[1] * ir - lp[3] * G1 * u;
@@ -155,10 +173,11 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
   const double rl = L[0], ul = L[1], vl = L[2], pl = L[3];
   const double rr = R[0], ur = R[1], vr = R[2], pr = R[3];
-  const double HL = 0.5 * (ul * ul + vl * vl) + GG1 * pl / rl;
-  const double HR = 0.5 * (ur * ur + vr * vr) + GG1 * pr / rr;
+  const double irl = rcp(rl), irr = rcp(rr), ipl = rcp(pl), ipr = rcp(pr);
+  const double HL = 0.5 * (ul * ul + vl * vl) + GG1 * pl * irl;
+  const double HR = 0.5 * (ur * ur + vr * vr) + GG1 * pr * irr;
   const double VnL = AX == 0 ? ul : vl, VnR = AX == 0 ? ur : vr;
-  const double cl = sqrt(G * pl / rl), cr = sqrt(G * pr / rr);
+  const double cl = sqrt(G * pl * irl), cr = sqrt(G * pr * irr);
   const double rBar = 0.5 * (rl + rr), cBar = 0.5 * (cl + cr);
   const double psr = 0.5 * (pl + pr) - 0.5 * rBar * cBar * (VnR - VnL);
   const bool clip = force >= 0 ? (force & HB_CLIP) != 0 : !(0.0 < psr);
@@ -166,13 +185,14 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   const double kap = (G + 1.0) / (2.0 * G);
   const bool shl = force >= 0 ? (force & HB_SHL) != 0 : !(ps <= pl);
   const bool shr = force >= 0 ? (force & HB_SHR) != 0 : !(ps <= pr);
-  const double ql = shl ? sqrt(1.0 + kap * (ps / pl - 1.0)) : 1.0;
-  const double qr = shr ? sqrt(1.0 + kap * (ps / pr - 1.0)) : 1.0;
+  const double ql = shl ? sqrt(1.0 + kap * (ps * ipl - 1.0)) : 1.0;
+  const double qr = shr ? sqrt(1.0 + kap * (ps * ipr - 1.0)) : 1.0;
   const double SL = VnL - cl * ql, SR = VnR + cr * qr;
   const double aL = SL - VnL, aR = SR - VnR;
   const double num = (pr - pl) + (rl * VnL * aL - rr * VnR * aR);
   const double den = rl * aL - rr * aR;
-  const double Ss = num / den;
+  const double iden = rcp(den);
+  const double Ss = num * iden;
 
   // forward side / star fluxes; reverse accumulators for (r, u, v, p, S, Ss)
   auto side = [&](double r, double u, double v, double p, double H, double Vn, double* out) {
@@ -194,20 +214,21 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
     // H = (u^2+v^2)/2 + GG1 p / r
     lu += lH * u;
     lv += lH * v;
-    lp += lH * GG1 / r;
-    lr += -lH * GG1 * p / (r * r);
+    const double ir = rcp(r);
+    lp += lH * GG1 * ir;
+    lr += -lH * GG1 * p * ir * ir;
     if (AX == 0) lu += lVn;
     else lv += lVn;
   };
   auto star = [&](double r, double u, double v, double p, double H, double Vn, double S, double* out) {
     double Fs[4];
     side(r, u, v, p, H, Vn, Fs);
-    double Usf = (S - Vn) / (S - Ss);
+    double Usf = (S - Vn) * rcp(S - Ss);
     double U3 = r * H - p;
     double Us0 = Usf * r;
     double Us1 = Usf * (Ss * nx + u * ny) * r;
     double Us2 = Usf * (Ss * ny + v * nx) * r;
-    double Us3 = Usf * (U3 + (Ss - Vn) * (r * Ss + p / (S - Vn)));
+    double Us3 = Usf * (U3 + (Ss - Vn) * (r * Ss + p * rcp(S - Vn)));
     out[0] = Fs[0] + S * (Us0 - r);
     out[1] = Fs[1] + S * (Us1 - r * u);
     out[2] = Fs[2] + S * (Us2 - r * v);
@@ -218,9 +239,10 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
                     double& lr, double& lu, double& lv, double& lp, double& lS, double& lSs) {
     double l0 = w * lo[0], l1 = w * lo[1], l2 = w * lo[2], l3 = w * lo[3];
     double D = S - Ss, A = S - Vn;
-    double Usf = A / D;
+    const double iA = rcp(A), iD = rcp(D);
+    double Usf = A * iD;
     double U3 = r * H - p;
-    double Pt = r * Ss + p / A;
+    double Pt = r * Ss + p * iA;
     double Us0 = Usf * r, Us1 = Usf * (Ss * nx + u * ny) * r, Us2 = Usf * (Ss * ny + v * nx) * r;
     double Us3 = Usf * (U3 + (Ss - Vn) * Pt);
     // out = Fs + S (Us - U)
@@ -259,11 +281,11 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
     // Pt = r Ss + p / A
     lr += lPt * Ss;
     lSs += lPt * r;
-    lp += lPt / A;
-    double lA = -lPt * p / (A * A);
+    lp += lPt * iA;
+    double lA = -lPt * p * iA * iA;
     // Usf = A / D
-    lA += lUsf / D;
-    double lD = -lUsf * A / (D * D);
+    lA += lUsf * iD;
+    double lD = -lUsf * A * iD * iD;
     // A = S - Vn ; D = S - Ss
     lS += lA + lD;
     lVn += -lA;
@@ -275,8 +297,9 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
     // H and Vn
     lu += lH * u;
     lv += lH * v;
-    lp += lH * GG1 / r;
-    lr += -lH * GG1 * p / (r * r);
+    const double ir = rcp(r);
+    lp += lH * GG1 * ir;
+    lr += -lH * GG1 * p * ir * ir;
     if (AX == 0) lu += lVn;
     else lv += lVn;
   };
@@ -312,7 +335,7 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
     if (wr != 0.0) star_b(rr, ur, vr, pr, HR, VnR, SR, lF, wr, lrr, lur, lvr, lpr, lSR, lSs);
   }
   // Ss = num / den
-  double lnum = lSs / den, lden = -lSs * Ss / den;
+  double lnum = lSs * iden, lden = -lSs * Ss * iden;
   // num = (pr - pl) + rl VnL aL - rr VnR aR ; den = rl aL - rr aR
   lpr += lnum;
   lpl += -lnum;
@@ -334,14 +357,14 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   // ql = sqrt(1 + kap (ps/pl - 1)) when shocked
   double lps = 0;
   if (shl) {
-    double g = lql * 0.5 / ql * kap;
-    lps += g / pl;
-    lpl += -g * ps / (pl * pl);
+    double g = lql * 0.5 * rcp(ql) * kap;
+    lps += g * ipl;
+    lpl += -g * ps * ipl * ipl;
   }
   if (shr) {
-    double g = lqr * 0.5 / qr * kap;
-    lps += g / pr;
-    lpr += -g * ps / (pr * pr);
+    double g = lqr * 0.5 * rcp(qr) * kap;
+    lps += g * ipr;
+    lpr += -g * ps * ipr * ipr;
   }
   // ps = max(0, psr); psr = (pl + pr)/2 - rBar cBar (VnR - VnL)/2
   if (!clip) {
@@ -358,12 +381,12 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   }
   // cl = sqrt(G pl / rl)
   {
-    double g = lcl * 0.5 / cl * G;
-    lpl += g / rl;
-    lrl += -g * pl / (rl * rl);
-    double h = lcr * 0.5 / cr * G;
-    lpr += h / rr;
-    lrr += -h * pr / (rr * rr);
+    double g = lcl * 0.5 * rcp(cl) * G;
+    lpl += g * irl;
+    lrl += -g * pl * irl * irl;
+    double h = lcr * 0.5 * rcp(cr) * G;
+    lpr += h * irr;
+    lrr += -h * pr * irr * irr;
   }
   if (AX == 0) {
     lul += lVnL;
@@ -385,112 +408,127 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
 // ---- Roe-averaged characteristic system of eigensystemEuler2D ------------------
 template <int AX> struct Roe2 {
   double c1, c2, D, HL, HR;
-  double u, v, ht, Vn, ek, h, c;
+  double u, v, ht, Vn, ek, h, c, iD, ic, ih;
   CFD_HD void fwd(const double* pl, const double* pr) {
-    HL = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2]) + GG1 * pl[3] / pl[0];
-    HR = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]) + GG1 * pr[3] / pr[0];
+    HL = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2]) + GG1 * pl[3] * rcp(pl[0]);
+    HR = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]) + GG1 * pr[3] * rcp(pr[0]);
     c1 = sqrt(pl[0]);
     c2 = sqrt(pr[0]);
     D = c1 + c2;
-    u = (c1 * pl[1] + c2 * pr[1]) / D;
-    v = (c1 * pl[2] + c2 * pr[2]) / D;
-    ht = (c1 * HL + c2 * HR) / D;
+    iD = rcp(D);
+    u = (c1 * pl[1] + c2 * pr[1]) * iD;
+    v = (c1 * pl[2] + c2 * pr[2]) * iD;
+    ht = (c1 * HL + c2 * HR) * iD;
     Vn = AX == 0 ? u : v;
     ek = 0.5 * (u * u + v * v);
     h = ht - ek;
     c = sqrt(G1 * h);
+    ic = rcp(c);
+    ih = rcp(h);
   }
   // left eigenvector matrix (rows = waves) and right (R[i][k])
   CFD_HD void mats(double L[4][4], double R[4][4]) const {
     const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
     L[0][0] = -ny * u + nx * v; L[0][1] = ny; L[0][2] = -nx; L[0][3] = 0.0;
     L[1][0] = h - ek; L[1][1] = u; L[1][2] = v; L[1][3] = -1.0;
-    L[2][0] = 0.5 * (Vn / c + ek / h); L[2][1] = 0.5 * (-nx / c - u / h); L[2][2] = 0.5 * (-ny / c - v / h); L[2][3] = 0.5 / h;
-    L[3][0] = 0.5 * (-Vn / c + ek / h); L[3][1] = 0.5 * (nx / c - u / h); L[3][2] = 0.5 * (ny / c - v / h); L[3][3] = 0.5 / h;
-    R[0][0] = 0.0; R[0][1] = 1.0 / h; R[0][2] = 1.0; R[0][3] = 1.0;
-    R[1][0] = ny; R[1][1] = u / h; R[1][2] = u - nx * c; R[1][3] = u + nx * c;
-    R[2][0] = -nx; R[2][1] = v / h; R[2][2] = v - ny * c; R[2][3] = v + ny * c;
-    R[3][0] = ny * u - nx * v; R[3][1] = ek / h; R[3][2] = h + ek - Vn * c; R[3][3] = h + ek + Vn * c;
+    L[2][0] = 0.5 * (Vn * ic + ek * ih); L[2][1] = 0.5 * (-nx * ic - u * ih); L[2][2] = 0.5 * (-ny * ic - v * ih); L[2][3] = 0.5 * ih;
+    L[3][0] = 0.5 * (-Vn * ic + ek * ih); L[3][1] = 0.5 * (nx * ic - u * ih); L[3][2] = 0.5 * (ny * ic - v * ih); L[3][3] = 0.5 * ih;
+    R[0][0] = 0.0; R[0][1] = ih; R[0][2] = 1.0; R[0][3] = 1.0;
+    R[1][0] = ny; R[1][1] = u * ih; R[1][2] = u - nx * c; R[1][3] = u + nx * c;
+    R[2][0] = -nx; R[2][1] = v * ih; R[2][2] = v - ny * c; R[2][3] = v + ny * c;
+    R[3][0] = ny * u - nx * v; R[3][1] = ek * ih; R[3][2] = h + ek - Vn * c; R[3][3] = h + ek + Vn * c;
   }
-  // cotangents of L and R -> cotangents of pl, pr (added)
-  CFD_HD void bwd(const double* pl, const double* pr, const double gL[4][4], const double gR[4][4], double* lpl,
-                  double* lpr) const {
-    const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+  // Reverse pass in pieces, for register-lean callers: cotangents of the
+  // Roe-averaged quantities are accumulated entry by entry of L and R, then
+  // chained back to the two cell states.
+  struct Acc {
     double lu = 0, lv = 0, lh = 0, lc = 0, lek = 0, lVn = 0;
-    const double ic = 1.0 / c, ih = 1.0 / h, ic2 = ic * ic, ih2 = ih * ih;
-    // L rows
-    lu += -ny * gL[0][0];
-    lv += nx * gL[0][0];
-    lh += gL[1][0];
-    lek += -gL[1][0];
-    lu += gL[1][1];
-    lv += gL[1][2];
-    // row 2
-    lVn += gL[2][0] * 0.5 * ic;
-    lc += -gL[2][0] * 0.5 * Vn * ic2;
-    lek += gL[2][0] * 0.5 * ih;
-    lh += -gL[2][0] * 0.5 * ek * ih2;
-    lc += gL[2][1] * 0.5 * nx * ic2;
-    lu += -gL[2][1] * 0.5 * ih;
-    lh += gL[2][1] * 0.5 * u * ih2;
-    lc += gL[2][2] * 0.5 * ny * ic2;
-    lv += -gL[2][2] * 0.5 * ih;
-    lh += gL[2][2] * 0.5 * v * ih2;
-    lh += -gL[2][3] * 0.5 * ih2;
-    // row 3
-    lVn += -gL[3][0] * 0.5 * ic;
-    lc += gL[3][0] * 0.5 * Vn * ic2;
-    lek += gL[3][0] * 0.5 * ih;
-    lh += -gL[3][0] * 0.5 * ek * ih2;
-    lc += -gL[3][1] * 0.5 * nx * ic2;
-    lu += -gL[3][1] * 0.5 * ih;
-    lh += gL[3][1] * 0.5 * u * ih2;
-    lc += -gL[3][2] * 0.5 * ny * ic2;
-    lv += -gL[3][2] * 0.5 * ih;
-    lh += gL[3][2] * 0.5 * v * ih2;
-    lh += -gL[3][3] * 0.5 * ih2;
-    // R
-    lu += ny * gR[3][0];
-    lv += -nx * gR[3][0];
-    lh += -(gR[0][1] + gR[1][1] * u + gR[2][1] * v + gR[3][1] * ek) * ih2;
-    lu += gR[1][1] * ih;
-    lv += gR[2][1] * ih;
-    lek += gR[3][1] * ih;
-    lu += gR[1][2] + gR[1][3];
-    lv += gR[2][2] + gR[2][3];
-    lc += -nx * gR[1][2] + nx * gR[1][3] - ny * gR[2][2] + ny * gR[2][3];
-    lh += gR[3][2] + gR[3][3];
-    lek += gR[3][2] + gR[3][3];
-    lVn += -c * gR[3][2] + c * gR[3][3];
-    lc += -Vn * gR[3][2] + Vn * gR[3][3];
+  };
+  // g = cotangent of row k of L
+  CFD_HD void acc_Lrow(int k, const double* g, Acc& a) const {
+    const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+    const double ic2 = ic * ic, ih2 = ih * ih;
+    if (k == 0) {
+      a.lu += -ny * g[0];
+      a.lv += nx * g[0];
+    } else if (k == 1) {
+      a.lh += g[0];
+      a.lek += -g[0];
+      a.lu += g[1];
+      a.lv += g[2];
+    } else {
+      const double sg = k == 2 ? 1.0 : -1.0;  // rows Vn - c and Vn + c differ in the 1/c terms
+      a.lVn += sg * g[0] * 0.5 * ic;
+      a.lc += -sg * g[0] * 0.5 * Vn * ic2;
+      a.lek += g[0] * 0.5 * ih;
+      a.lh += -g[0] * 0.5 * ek * ih2;
+      a.lc += sg * g[1] * 0.5 * nx * ic2;
+      a.lu += -g[1] * 0.5 * ih;
+      a.lh += g[1] * 0.5 * u * ih2;
+      a.lc += sg * g[2] * 0.5 * ny * ic2;
+      a.lv += -g[2] * 0.5 * ih;
+      a.lh += g[2] * 0.5 * v * ih2;
+      a.lh += -g[3] * 0.5 * ih2;
+    }
+  }
+  // gR(i, k) = cotangent of R[i][k]
+  template <class GR> CFD_HD void acc_R(const GR& gR, Acc& a) const {
+    const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+    const double ih2 = ih * ih;
+    const double r30 = gR(3, 0), r01 = gR(0, 1), r11 = gR(1, 1), r21 = gR(2, 1), r31 = gR(3, 1);
+    const double r12 = gR(1, 2), r13 = gR(1, 3), r22 = gR(2, 2), r23 = gR(2, 3), r32 = gR(3, 2), r33 = gR(3, 3);
+    a.lu += ny * r30;
+    a.lv += -nx * r30;
+    a.lh += -(r01 + r11 * u + r21 * v + r31 * ek) * ih2;
+    a.lu += r11 * ih;
+    a.lv += r21 * ih;
+    a.lek += r31 * ih;
+    a.lu += r12 + r13;
+    a.lv += r22 + r23;
+    a.lc += -nx * r12 + nx * r13 - ny * r22 + ny * r23;
+    a.lh += r32 + r33;
+    a.lek += r32 + r33;
+    a.lVn += -c * r32 + c * r33;
+    a.lc += -Vn * r32 + Vn * r33;
+  }
+  CFD_HD void finish(const Acc& a0, const double* pl, const double* pr, double* lpl, double* lpr) const {
+    double lu = a0.lu, lv = a0.lv, lh = a0.lh, lek = a0.lek;
     // c = sqrt(G1 h); h = ht - ek; ek = (u^2+v^2)/2; Vn = u | v
-    lh += lc * 0.5 * G1 / c;
-    double lht = lh;
+    lh += a0.lc * 0.5 * G1 * ic;
+    const double lht = lh;
     lek += -lh;
     lu += lek * u;
     lv += lek * v;
-    if (AX == 0) lu += lVn;
-    else lv += lVn;
+    if (AX == 0) lu += a0.lVn;
+    else lv += a0.lVn;
     // u = (c1 ul + c2 ur)/D etc.
-    const double iD = 1.0 / D;
-    double lc1 = (lu * (pl[1] - u) + lv * (pl[2] - v) + lht * (HL - ht)) * iD;
-    double lc2 = (lu * (pr[1] - u) + lv * (pr[2] - v) + lht * (HR - ht)) * iD;
+    const double lc1 = (lu * (pl[1] - u) + lv * (pl[2] - v) + lht * (HL - ht)) * iD;
+    const double lc2 = (lu * (pr[1] - u) + lv * (pr[2] - v) + lht * (HR - ht)) * iD;
     lpl[1] += lu * c1 * iD;
     lpr[1] += lu * c2 * iD;
     lpl[2] += lv * c1 * iD;
     lpr[2] += lv * c2 * iD;
-    double lHL = lht * c1 * iD, lHR = lht * c2 * iD;
-    lpl[0] += lc1 * 0.5 / c1;
-    lpr[0] += lc2 * 0.5 / c2;
+    const double lHL = lht * c1 * iD, lHR = lht * c2 * iD;
+    lpl[0] += lc1 * 0.5 * rcp(c1);
+    lpr[0] += lc2 * 0.5 * rcp(c2);
     // HL = (ul^2+vl^2)/2 + GG1 pl/rl
     lpl[1] += lHL * pl[1];
     lpl[2] += lHL * pl[2];
-    lpl[3] += lHL * GG1 / pl[0];
-    lpl[0] += -lHL * GG1 * pl[3] / (pl[0] * pl[0]);
+    const double ipl0 = rcp(pl[0]), ipr0 = rcp(pr[0]);
+    lpl[3] += lHL * GG1 * ipl0;
+    lpl[0] += -lHL * GG1 * pl[3] * ipl0 * ipl0;
     lpr[1] += lHR * pr[1];
     lpr[2] += lHR * pr[2];
-    lpr[3] += lHR * GG1 / pr[0];
-    lpr[0] += -lHR * GG1 * pr[3] / (pr[0] * pr[0]);
+    lpr[3] += lHR * GG1 * ipr0;
+    lpr[0] += -lHR * GG1 * pr[3] * ipr0 * ipr0;
+  }
+  // cotangents of L and R -> cotangents of pl, pr (added)
+  CFD_HD void bwd(const double* pl, const double* pr, const double gL[4][4], const double gR[4][4], double* lpl,
+                  double* lpr) const {
+    Acc a;
+    for (int k = 0; k < 4; k++) acc_Lrow(k, gL[k], a);
+    acc_R([&](int i, int k) { return gR[i][k]; }, a);
+    finish(a, pl, pr, lpl, lpr);
   }
 };
 
@@ -694,8 +732,10 @@ template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd
 }
 
 // Reverse pass of the interface flux given its forward record: lw[j][v] +=
-// (dF / dw[j][v])^T lF.  The per-wave WENO intermediates and projections are
-// recomputed rather than held (register-lean for device use).
+// (dF / dw[j][v])^T lF.  Register-lean for device use: the per-wave WENO
+// intermediates and projections are recomputed rather than held, the
+// eigenvector cotangents are folded into six scalars as they appear, and the
+// projection cotangents go straight through p2c's transpose wave by wave.
 template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF,
                                                              double lw[6][4]) {
   double loL[4] = {0, 0, 0, 0}, loR[4] = {0, 0, 0, 0};
@@ -717,18 +757,25 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
   } else {
     c2p_vjp(f.cR, loR, lcR);
   }
-  double gR[4][4], gL[4][4];
+  double pl[4], pr[4];
+  for (int v = 0; v < 4; v++) {
+    pl[v] = w(2, v);
+    pr[v] = w(3, v);
+  }
+  Roe2<AX> es;
+  es.fwd(pl, pr);
+  typename Roe2<AX>::Acc acc;
+  es.acc_R([&](int i, int k) { return f.QL[k] * lcL[i] + f.QR[k] * lcR[i]; }, acc);
   double lQL[4], lQR[4];
+  CFD_UNROLL
   for (int k = 0; k < 4; k++) {
     lQL[k] = f.R[0][k] * lcL[0] + f.R[1][k] * lcL[1] + f.R[2][k] * lcL[2] + f.R[3][k] * lcL[3];
     lQR[k] = f.R[0][k] * lcR[0] + f.R[1][k] * lcR[1] + f.R[2][k] * lcR[2] + f.R[3][k] * lcR[3];
-    for (int i = 0; i < 4; i++) gR[i][k] = f.QL[k] * lcL[i] + f.QR[k] * lcR[i];
   }
-  double lc[6][4];
-  for (int j = 0; j < 6; j++)
-    for (int v = 0; v < 4; v++) lc[j][v] = 0.0;
+  CFD_UNROLL
   for (int k = 0; k < 4; k++) {
     double ch[6];
+    CFD_UNROLL
     for (int j = 0; j < 6; j++) {
       double p[4], c[4];
       for (int v = 0; v < 4; v++) p[v] = w(j, v);
@@ -752,30 +799,21 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
     lch[0] = gl[0];
     for (int j = 1; j < 5; j++) lch[j] = gl[j] + gr[5 - j];
     lch[5] = gr[0];
-    for (int v = 0; v < 4; v++) gL[k][v] = 0.0;
+    double gLk[4] = {0, 0, 0, 0};
+    CFD_UNROLL
     for (int j = 0; j < 6; j++) {
-      double p[4], c[4];
+      double p[4], c[4], lt[4];
       for (int v = 0; v < 4; v++) p[v] = w(j, v);
       p2c(p, c);
       for (int v = 0; v < 4; v++) {
-        lc[j][v] += lch[j] * f.L[k][v];
-        gL[k][v] += lch[j] * c[v];
+        gLk[v] += lch[j] * c[v];
+        lt[v] = lch[j] * f.L[k][v];
       }
+      p2c_vjp(p, lt, lw[j]);
     }
+    es.acc_Lrow(k, gLk, acc);
   }
-  double pl[4], pr[4];
-  for (int v = 0; v < 4; v++) {
-    pl[v] = w(2, v);
-    pr[v] = w(3, v);
-  }
-  Roe2<AX> es;
-  es.fwd(pl, pr);
-  es.bwd(pl, pr, gL, gR, lw[2], lw[3]);
-  for (int j = 0; j < 6; j++) {
-    double p[4];
-    for (int v = 0; v < 4; v++) p[v] = w(j, v);
-    p2c_vjp(p, lc[j], lw[j]);
-  }
+  es.finish(acc, pl, pr, lw[2], lw[3]);
 }
 
 // Same derivative as interface<AX>(w, F, lw, lF), from the two passes above.

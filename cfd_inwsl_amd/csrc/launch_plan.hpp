@@ -22,8 +22,10 @@ inline long plan_pencil_blocks(const int* n, int ax, int tile) {
 }
 
 // Lane slots spent by the interface phase (the dominant one): every segment of
-// length L runs ceil((L + 3) / CA) chunks of CA x CP lanes.  Workgroup counts
-// below a few per CU are penalised (the chip has 256 CUs).
+// length L runs ceil((L + 3) / CA) chunks of CA x CP lanes.  The kernel is
+// latency bound, so a launch with fewer workgroups than the chip holds at
+// once (256 CUs x 4 resident workgroups) is charged in proportion to the
+// missing ones.
 inline double plan_cost(const int* n, int ax, int tile, int seg) {
   const int ca = tile_ca(tile);
   const long nb = plan_pencil_blocks(n, ax, tile);
@@ -36,8 +38,8 @@ inline double plan_cost(const int* n, int ax, int tile, int seg) {
   const long nseg = (len + seg - 1) / seg;
   const double blocks = (double)nb * nseg;
   double cost = (double)nb * chunks * 256.0;
-  const double want = 2048.0;
-  if (blocks < want) cost *= 1.0 + 0.5 * (want - blocks) / want;
+  const double want = 1024.0;
+  if (blocks < want) cost *= want / blocks;
   return cost;
 }
 
@@ -53,7 +55,7 @@ inline int plan_candidates(const int* n, int ax, PlanCandidate* out, int maxc) {
     if (na < 256) all[na++] = {tiles[t], len, plan_cost(n, ax, tiles[t], len)};
     for (int k = 1; k * ca - 3 < len && na < 256; k++) {
       int seg = k * ca - 3;
-      if (seg < 16) continue;
+      if (seg < 8) continue;
       all[na++] = {tiles[t], seg, plan_cost(n, ax, tiles[t], seg)};
     }
   }

@@ -5,6 +5,7 @@ group.
     python -m cfd_inwsl_amd.profile_run --case vortex2d --n 2048 --steps 5
     python -m cfd_inwsl_amd.profile_run --case periodic3d --n 256 --steps 2
     python -m cfd_inwsl_amd.profile_run --case vortex2d --n 2048 --copy 5
+    python -m cfd_inwsl_amd.profile_run --case adjoint2d --n 512 --steps 40
 """
 import argparse
 import sys
@@ -17,6 +18,10 @@ def make(case, n):
     if case == "vortex2d":
         cons, h = cases.vortex(n)
         return Solver(2, (n, n), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, [BC.PERIODIC] * 4), cons
+    if case == "adjoint2d":
+        c = cases.adjoint_shock(n)
+        return (Solver(2, (n, n), (c["h"], c["h"]), InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR,
+                       [BC.PERIODIC] * 4), cases.adjoint_state(c))
     if case == "periodic3d":
         cons, h = cases.periodic3d(n)
         return Solver(3, (n, n, n), h, InterMethod.TCNS5, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 6), cons
@@ -38,6 +43,18 @@ def main():
         s.copy_probe(a.copy)
         nbytes = 2 * s.ncell * s.nv * 8
         print(f"[profile_run] copy probe x{a.copy}: {nbytes} bytes moved per launch", flush=True)
+    elif a.case == "adjoint2d":
+        from cfd_inwsl_amd import adjoint
+        c = cases.adjoint_shock(a.n)
+        dt = cases.cfl_dt_2d(cons, a.n, c["h"])
+        s.adjoint_forward(dt, a.steps, 20)
+        t1 = time.perf_counter()
+        _, gJ = adjoint.window_pressure_objective(s.get_state(), c)
+        t2 = time.perf_counter()
+        s.adjoint_backward(gJ)
+        t3 = time.perf_counter()
+        print(f"[profile_run] adjoint {a.steps} steps: forward {t1 - t0:.3f} s, backward {t3 - t2:.3f} s",
+              flush=True)
     else:
         s.rk3_steps_cfl(0.5, a.steps, want_time=False)
         s.synchronize()

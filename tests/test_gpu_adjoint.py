@@ -172,8 +172,13 @@ def test_full_workload_against_central_differences():
             s.rk3_step(dt)
         return adjoint.window_pressure_objective(s.get_state(), case)[0]
 
+    # The map is only piecewise smooth (HLLC wave-region and shock-factor
+    # switches move with the shock; pressure equilibrium ties the shock-factor
+    # predicate upstream), so central differences converge to the active
+    # branch's derivative only as the step shrinks: the step is 1e-7 of the
+    # O(1) direction, where FD rounding noise is still ~1e-9 relative.
     worst = 0.0
-    eps = 1e-5
+    eps = 1e-7
     for d in _directions(n, 8, 99):
         fd = (Jof(case["rho0"] + eps * d) - Jof(case["rho0"] - eps * d)) / (2 * eps)
         adj = float((grho * d).sum())
