@@ -161,6 +161,19 @@ def bench_adjoint(n=512, nsteps=400, every=20, ndir=32):
     grho = case["rho_factor"] * (g[..., 0] + g[..., 1] * u + g[..., 2] * v + g[..., 3] * 0.5 * (u * u + v * v))
     fwd_ms, bwd_ms = (t1 - t0) * 1e3, (t3 - t2) * 1e3
     log(f"[bench] adjoint: forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms, ratio {bwd_ms / fwd_ms:.2f}")
+    # the same with every stage state kept (no recomputation)
+    s.set_state(U0)
+    t4 = time.perf_counter()
+    s.adjoint_forward(dt, nsteps, 0)
+    t5 = time.perf_counter()
+    _, gJ0 = adjoint.window_pressure_objective(s.get_state(), case)
+    t6 = time.perf_counter()
+    gU0 = s.adjoint_backward(gJ0)
+    t7 = time.perf_counter()
+    keep_all = {"forward_ms": round((t5 - t4) * 1e3, 2), "backward_ms": round((t7 - t6) * 1e3, 2),
+                "adjoint_over_forward": round((t7 - t6) / (t5 - t4), 3),
+                "same_gradient_bits": bool(np.array_equal(gU0, gU))}
+    log(f"[bench] adjoint, every stage kept: {keep_all}")
 
     def Jof(rho):
         s.set_state(cases.adjoint_state(case, rho))
@@ -181,7 +194,7 @@ def bench_adjoint(n=512, nsteps=400, every=20, ndir=32):
     out = {"workload": f"{n}^2 periodic, WCNS5-JS char-wise + HLLC, Mach-1.5 shock, {nsteps} fixed-dt RK3 steps, "
                        f"J = int p^2 over 64^2 window, checkpoint every {every}",
            "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
-           "adjoint_over_forward": round(bwd_ms / fwd_ms, 3), "J": J,
+           "adjoint_over_forward": round(bwd_ms / fwd_ms, 3), "J": J, "keep_every_stage": keep_all,
            "fd": {"eps": eps, "directions": ndir, "median_rel": float(np.median(fd_rel)), "max_rel": max(fd_rel)}}
     golden = os.path.join(ROOT, "tests", "golden", "adjoint_dual_512.json")
     if os.path.exists(golden):

@@ -509,11 +509,13 @@ int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad) {
 
 int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_every) {
   if (!h) return cfd_fail(CFD_E_ARG, "null handle");
-  if (nsteps < 1 || checkpoint_every < 1 || !(dt > 0.0))
-    return cfd_fail(CFD_E_ARG, "need nsteps >= 1, checkpoint_every >= 1 and dt > 0");
+  if (nsteps < 1 || checkpoint_every < 0 || !(dt > 0.0))
+    return cfd_fail(CFD_E_ARG, "need nsteps >= 1, checkpoint_every >= 0 and dt > 0");
   if (int rc = adj_ensure(h)) return rc;
   AdjointStore* s = h->adj;
-  const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+  s->nsteps = 0;  // valid again only once this forward completes
+  const bool keep_all = checkpoint_every == 0;
+  const int nck = keep_all ? 0 : (nsteps + checkpoint_every - 1) / checkpoint_every;
   while ((int)s->ckpt.size() > nck) {
     AHIP(hipFree(s->ckpt.back()));
     s->ckpt.pop_back();
@@ -523,7 +525,9 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
     AHIP(hipMalloc(&p, s->state_bytes));
     s->ckpt.push_back(p);
   }
-  const int seg = std::min(checkpoint_every, nsteps);
+  // stage store: S_i at 3i, u1_i at 3i+1, u2_i at 3i+2 for the steps of one
+  // segment (all steps when every stage state is kept), plus the end state
+  const int seg = keep_all ? nsteps : std::min(checkpoint_every, nsteps);
   const int need = 3 * seg + 1;
   if (s->stage_cap < need) {
     if (s->stage) AHIP(hipFree(s->stage));
@@ -532,20 +536,35 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
     AHIP(hipMalloc(&s->stage, s->state_bytes * need));
     s->stage_cap = need;
   }
-  s->nsteps = nsteps;
   s->every = checkpoint_every;
   s->dt = dt;
   hipLaunchKernelGGL(k_put_dt, dim3(1), dim3(1), 0, h->stream, h->scal, dt);
-  for (int st = 0; st < nsteps; st++) {
-    if (st % checkpoint_every == 0)
-      AHIP(hipMemcpyAsync(s->ckpt[st / checkpoint_every], h->U0, s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
-    int rc;
-    if ((rc = rk_stage(h, 1, h->U0, h->U0, h->Ua, nullptr))) return rc;
-    if ((rc = rk_stage(h, 2, h->Ua, h->U0, h->Ub, nullptr))) return rc;
-    if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->U0, nullptr))) return rc;
+  int rc;
+  if (keep_all) {
+    const size_t nd = (size_t)h->N * 4;
+    double* st = s->stage;
+    AHIP(hipMemcpyAsync(st, h->U0, s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+    for (int i = 0; i < nsteps; i++) {
+      double *S0 = st + (size_t)(3 * i) * nd, *S1 = S0 + nd, *S2 = S1 + nd, *S3 = S2 + nd;
+      if ((rc = rk_stage(h, 1, S0, S0, S1, nullptr))) return rc;
+      if ((rc = rk_stage(h, 2, S1, S0, S2, nullptr))) return rc;
+      if ((rc = rk_stage(h, 3, S2, S0, S3, nullptr))) return rc;
+    }
+    AHIP(hipMemcpyAsync(h->U0, st + (size_t)(3 * nsteps) * nd, s->state_bytes, hipMemcpyDeviceToDevice,
+                        h->stream));
+  } else {
+    for (int st = 0; st < nsteps; st++) {
+      if (st % checkpoint_every == 0)
+        AHIP(hipMemcpyAsync(s->ckpt[st / checkpoint_every], h->U0, s->state_bytes, hipMemcpyDeviceToDevice,
+                            h->stream));
+      if ((rc = rk_stage(h, 1, h->U0, h->U0, h->Ua, nullptr))) return rc;
+      if ((rc = rk_stage(h, 2, h->Ua, h->U0, h->Ub, nullptr))) return rc;
+      if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->U0, nullptr))) return rc;
+    }
   }
   AHIP(hipGetLastError());
   AHIP(hipStreamSynchronize(h->stream));
+  s->nsteps = nsteps;
   return CFD_OK;
 }
 
@@ -557,25 +576,29 @@ int cfd_adjoint_backward(cfd_handle* h, const double* cot_final, double* grad_in
   const size_t nd = (size_t)h->N * 4;
   AHIP(hipMemcpyAsync(s->LB, cot_final, s->state_bytes, hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_aos_to_soa4, dim3(nb), dim3(256), 0, h->stream, s->LB, s->LA, h->N);
-  hipLaunchKernelGGL(k_put_dt, dim3(1), dim3(1), 0, h->stream, h->scal, s->dt);
-  const int nck = (int)s->ckpt.size();
-  for (int m = nck - 1; m >= 0; m--) {
-    const int first = m * s->every;
-    const int len = std::min(s->every, s->nsteps - first);
-    // stage states of the segment: S_i at 3i, u1_i at 3i+1, u2_i at 3i+2
-    auto S = [&](int i) { return s->stage + (size_t)(3 * i) * nd; };
-    auto U1 = [&](int i) { return s->stage + (size_t)(3 * i + 1) * nd; };
-    auto U2 = [&](int i) { return s->stage + (size_t)(3 * i + 2) * nd; };
-    AHIP(hipMemcpyAsync(S(0), s->ckpt[m], s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
-    int rc;
-    for (int i = 0; i < len; i++) {
-      if ((rc = rk_stage(h, 1, S(i), S(i), U1(i), nullptr))) return rc;
-      if ((rc = rk_stage(h, 2, U1(i), S(i), U2(i), nullptr))) return rc;
-      if (i + 1 < len)
-        if ((rc = rk_stage(h, 3, U2(i), S(i), S(i + 1), nullptr))) return rc;
-    }
-    for (int i = len - 1; i >= 0; i--)
+  auto S = [&](int i) { return s->stage + (size_t)(3 * i) * nd; };
+  auto U1 = [&](int i) { return s->stage + (size_t)(3 * i + 1) * nd; };
+  auto U2 = [&](int i) { return s->stage + (size_t)(3 * i + 2) * nd; };
+  int rc;
+  if (s->every == 0) {
+    for (int i = s->nsteps - 1; i >= 0; i--)
       if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_put_dt, dim3(1), dim3(1), 0, h->stream, h->scal, s->dt);
+    const int nck = (int)s->ckpt.size();
+    for (int m = nck - 1; m >= 0; m--) {
+      const int first = m * s->every;
+      const int len = std::min(s->every, s->nsteps - first);
+      AHIP(hipMemcpyAsync(S(0), s->ckpt[m], s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+      for (int i = 0; i < len; i++) {
+        if ((rc = rk_stage(h, 1, S(i), S(i), U1(i), nullptr))) return rc;
+        if ((rc = rk_stage(h, 2, U1(i), S(i), U2(i), nullptr))) return rc;
+        if (i + 1 < len)
+          if ((rc = rk_stage(h, 3, U2(i), S(i), S(i + 1), nullptr))) return rc;
+      }
+      for (int i = len - 1; i >= 0; i--)
+        if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt))) return rc;
+    }
   }
   hipLaunchKernelGGL(k_soa_to_aos4, dim3(nb), dim3(256), 0, h->stream, s->LA, s->LB, h->N);
   AHIP(hipGetLastError());

@@ -299,7 +299,8 @@ class Solver:
         return out
 
     def adjoint_forward(self, dt, nsteps, checkpoint_every=20):
-        """nsteps fixed-dt RK3 steps, keeping checkpoints for adjoint_backward."""
+        """nsteps fixed-dt RK3 steps, keeping a checkpoint every checkpoint_every
+        steps for adjoint_backward (0: keep every stage state, no recomputation)."""
         self._check(self.lib.cfd_adjoint_forward(self.h, float(dt), int(nsteps), int(checkpoint_every)))
 
     def adjoint_backward(self, cot_final):

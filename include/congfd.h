@@ -204,7 +204,10 @@ int cfd_batch_synchronize(cfd_batch* b);
 int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad);
 
 /* Advances the current state by nsteps RK3 steps of the fixed dt, keeping the
- * state every checkpoint_every steps on the device for cfd_adjoint_backward. */
+ * state every checkpoint_every steps on the device for cfd_adjoint_backward
+ * (which then recomputes each segment's stage states).  checkpoint_every = 0
+ * keeps every stage state instead (3 nsteps + 1 states of device memory, no
+ * recomputation). */
 int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_every);
 
 /* Pulls the cotangent of the state after the last cfd_adjoint_forward back to

@@ -135,7 +135,7 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     case, U0 = _shock(n)
     dt = cases.cfl_dt_2d(U0, n, case["h"])
     ref = _gpu_gradient(case, U0, dt, nsteps, 1)[2]
-    for every in (7, 25, 40):
+    for every in (7, 25, 40, 0):  # 0: every stage state kept, no recomputation
         assert np.array_equal(_gpu_gradient(case, U0, dt, nsteps, every)[2], ref), every
 
 
@@ -156,14 +156,43 @@ def test_torch_harness_equals_direct_call():
     np.testing.assert_allclose(rho0.grad.numpy(), _grad_rho(case, gU), rtol=1e-12, atol=1e-14 * np.abs(gU).max())
 
 
-def test_full_workload_against_central_differences():
-    """512^2, 400 steps, checkpoint every 20: adjoint vs central FD of the GPU
-    forward on 8 seeded density directions."""
+def _full_workload():
     n, nsteps = 512, 400
     case, U0 = _shock(n)
     dt = cases.cfl_dt_2d(U0, n, case["h"])
     J, UT, gU = _gpu_gradient(case, U0, dt, nsteps, 20)
-    grho = _grad_rho(case, gU)
+    return n, nsteps, case, U0, dt, J, _grad_rho(case, gU)
+
+
+def test_full_workload_against_dual_oracle_golden():
+    """512^2, 400 steps, checkpoint every 20: dJ/drho0 . d against the
+    dual-number oracle's values (tests/golden/adjoint_dual_512.json, written by
+    tests/golden/make_adjoint_dual.py) on the same 32 seeded directions."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adjoint_dual_512.json")
+    ref = json.load(open(path))
+    n, nsteps, case, U0, dt, J, grho = _full_workload()
+    assert (ref["n"], ref["nsteps"]) == (n, nsteps) and ref["dt"] == dt
+    assert abs(J - ref["J"]) <= 1e-14 * abs(ref["J"])  # the forward is the oracle's, bit for bit
+    dirs = _directions(n, len(ref["jvp"]), ref["direction_seed"])
+    rel = [abs(float((grho * d).sum()) - j) / abs(j) for d, j in zip(dirs, ref["jvp"])]
+    _say(f"adjoint vs dual oracle, 512^2 x 400 steps, {len(rel)} directions: worst rel {max(rel):.2e}")
+    assert max(rel) < 1e-10
+
+
+def test_full_workload_against_central_differences():
+    """512^2, 400 steps: adjoint vs central FD of the GPU forward on 8 seeded
+    density directions.
+
+    The discrete map is only piecewise smooth: HLLC wave-region and
+    shock-factor switches move with the shock, and pressure equilibrium ties
+    the shock-factor predicate upstream.  Central differences therefore agree
+    with the active branch's derivative (which the dual-number oracle
+    reproduces to 1e-12) only down to a floor set by those switches and by FD
+    rounding noise: with a step of 1e-7 the measured disagreement is ~1e-6 to
+    ~1e-5 typically and up to ~3e-4 where dJ . d is small."""
+    n, nsteps, case, U0, dt, J, grho = _full_workload()
     s = _solver((n, n), (case["h"], case["h"]), [BC.PERIODIC] * 4)
 
     def Jof(rho):
@@ -172,19 +201,14 @@ def test_full_workload_against_central_differences():
             s.rk3_step(dt)
         return adjoint.window_pressure_objective(s.get_state(), case)[0]
 
-    # The map is only piecewise smooth (HLLC wave-region and shock-factor
-    # switches move with the shock; pressure equilibrium ties the shock-factor
-    # predicate upstream), so central differences converge to the active
-    # branch's derivative only as the step shrinks: the step is 1e-7 of the
-    # O(1) direction, where FD rounding noise is still ~1e-9 relative.
-    worst = 0.0
+    rels = []
     eps = 1e-7
     for d in _directions(n, 8, 99):
         fd = (Jof(case["rho0"] + eps * d) - Jof(case["rho0"] - eps * d)) / (2 * eps)
         adj = float((grho * d).sum())
-        rel = abs(adj - fd) / abs(fd)
-        _say(f"  FD {fd:.12e} adjoint {adj:.12e} rel {rel:.2e}")
-        worst = max(worst, rel)
+        rels.append(abs(adj - fd) / abs(fd))
+        _say(f"  FD {fd:.12e} adjoint {adj:.12e} rel {rels[-1]:.2e}")
     s.close()
-    _say(f"adjoint vs central FD, 512^2 x 400 steps: worst rel {worst:.2e}")
-    assert worst < 1e-6
+    _say(f"adjoint vs central FD, 512^2 x 400 steps: median {np.median(rels):.2e} worst {max(rels):.2e}")
+    assert np.median(rels) < 1e-4
+    assert max(rels) < 2e-3
