@@ -32,6 +32,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(dim):
+    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE x 2 + WRITE_SIZE x 1, copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_sweeps.json")
+    if not os.path.exists(path):
+        return None
+    key = "2d_vortex_2048" if dim == 2 else "3d_periodic_256"
+    entry = json.load(open(path)).get(key)
+    return entry["traffic_bytes_per_launch"] if entry else None
+
+
 def measure(sol, nv, dim, ncell, steps, warmup, cfl):
     tiles, segs = sol.autotune(reps=3)
     log(f"[bench] launch plan: tiles {tiles} segments {segs}")
@@ -51,7 +62,7 @@ def measure(sol, nv, dim, ncell, steps, warmup, cfl):
     bytes_per_launch = 24 * nv * ncell
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dim),
             "kernel": "sweep_kernel", "sweep_ms": [round(float(x), 4) for x in ms],
             "bytes_per_launch": bytes_per_launch, "launch_plan": {"tiles": tiles, "segments": segs}}
     return value, sec * 1e3 / steps, roof

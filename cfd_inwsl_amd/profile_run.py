@@ -6,8 +6,16 @@ group.
     python -m cfd_inwsl_amd.profile_run --case periodic3d --n 256 --steps 2
     python -m cfd_inwsl_amd.profile_run --case vortex2d --n 2048 --copy 5
     python -m cfd_inwsl_amd.profile_run --case adjoint2d --n 512 --steps 40
+    python -m cfd_inwsl_amd.profile_run --summarize gpurun_out/x/run_results.db [...]
+
+--autotune picks the launch plan by timing first (as bench.py does); the plan
+kept is printed, so its dispatches can be told apart by grid size.
+--summarize prints a markdown table per kernel and grid of the rocprofv3
+database(s): mean duration (kernel trace) or mean counter value per dispatch
+(--pmc runs).
 """
 import argparse
+import sqlite3
 import sys
 import time
 
@@ -28,15 +36,51 @@ def make(case, n):
     raise SystemExit(f"unknown case {case}")
 
 
+def summarize(paths):
+    for path in paths:
+        con = sqlite3.connect(path)
+        cur = con.cursor()
+        views = {r[0] for r in cur.execute("select name from sqlite_master where type='view'")}
+        print(f"### {path}\n")
+        rows = []
+        if "counters_collection" in views:
+            rows = list(cur.execute(
+                "select kernel_name, grid_size_x, grid_size_y, counter_name, count(*), avg(value) "
+                "from counters_collection group by kernel_name, grid_size_x, grid_size_y, counter_name "
+                "order by avg(value) desc"))
+        if rows:
+            print("| kernel | grid x | grid y | counter | dispatches | mean per dispatch |")
+            print("|---|---|---|---|---|---|")
+            for k, gx, gy, name, n, v in rows:
+                print(f"| `{k[:90]}` | {gx} | {gy} | {name} | {n} | {v:.1f} |")
+        else:
+            print("| kernel | grid x | grid y | dispatches | mean us | min us | max us |")
+            print("|---|---|---|---|---|---|---|")
+            for k, gx, gy, n, a, lo, hi in cur.execute(
+                    "select name, grid_x, grid_y, count(*), avg(duration), min(duration), max(duration) "
+                    "from kernels group by name, grid_x, grid_y order by sum(duration) desc"):
+                print(f"| `{k[:90]}` | {gx} | {gy} | {n} | {a / 1e3:.1f} | {lo / 1e3:.1f} | {hi / 1e3:.1f} |")
+        print()
+        con.close()
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
+        summarize(sys.argv[2:])
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--case", default="vortex2d")
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--copy", type=int, default=0, help="run the plain copy probe this many times instead")
+    ap.add_argument("--autotune", action="store_true", help="time the launch candidates first")
     a = ap.parse_args()
     s, cons = make(a.case, a.n)
     s.set_state(cons)
+    if a.autotune:
+        tiles, segs = s.autotune(reps=3)
+        print(f"[profile_run] autotuned plan: tiles {tiles} segments {segs}", flush=True)
+    print(f"[profile_run] launch plan: {s.launch_plan()}", flush=True)
     print(f"[profile_run] {a.case} n={a.n} ready", flush=True)
     t0 = time.perf_counter()
     if a.copy:
