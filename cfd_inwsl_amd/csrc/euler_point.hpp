@@ -29,6 +29,42 @@ template <> __device__ __forceinline__ float eps_tiny<float>() { return 1e-36f; 
 template <class T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
 template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
 
+// FP64 division and square root in the instruction sequences the compiler
+// emits for IEEE a / b and sqrt(x) on gfx950 (reciprocal or reciprocal square
+// root estimate, Newton-Raphson refinement, FMA correction), without their
+// range-scaling and special-operand steps (v_div_scale, the scaling of
+// v_div_fmas, v_div_fixup; the 2^-767 rescale and class test of sqrt).  Those
+// steps are identities for the solver's operands -- nonzero normal divisors,
+// dividends above 2^-969 in magnitude or zero, normal quotients, square-root
+// arguments above 2^-767 or zero -- so the results are the correctly rounded
+// ones, bit for bit (except that a zero dividend gives +0 where IEEE may give
+// -0, which no later operation of the path distinguishes).  8 instead of 11
+// and 13 instead of 19 instructions; FP32 keeps the compiler's sequences.
+template <class T> __device__ __forceinline__ T fdv(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ double fdv<double>(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = a * r;
+  const double t = __builtin_fma(-b, q, a);
+  return __builtin_fma(t, r, q);
+}
+template <class T> __device__ __forceinline__ T fsq(T x) { return sqrt(x); }
+template <> __device__ __forceinline__ double fsq<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return x == 0.0 ? x : g;
+}
+
 // ---- 5-point interpolation: value at q2 + h/2 ------------------------------
 // Smoothness indicators (zone_FD/src/include/interScheme.hpp:20-27).
 template <class T>
@@ -64,14 +100,14 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
     T u2 = T(0.375) * q2 + T(0.75) * q3 - T(0.125) * q4;
     T a0, a1, a2;
     if (SCH == SCH_JS) {
-      a0 = T(0.0625) / sq(T(1e-6) + b0);
-      a1 = T(0.625) / sq(T(1e-6) + b1);
-      a2 = T(0.3125) / sq(T(1e-6) + b2);
+      a0 = fdv(T(0.0625), sq(T(1e-6) + b0));
+      a1 = fdv(T(0.625), sq(T(1e-6) + b1));
+      a2 = fdv(T(0.3125), sq(T(1e-6) + b2));
     } else if (sizeof(T) == sizeof(double)) {
       T tau = fabs(b2 - b0);
-      a0 = T(0.0625) * (T(1.0) + sq(tau / (b0 + eps_tiny<T>())));
-      a1 = T(0.625) * (T(1.0) + sq(tau / (b1 + eps_tiny<T>())));
-      a2 = T(0.3125) * (T(1.0) + sq(tau / (b2 + eps_tiny<T>())));
+      a0 = T(0.0625) * (T(1.0) + sq(fdv(tau, b0 + eps_tiny<T>())));
+      a1 = T(0.625) * (T(1.0) + sq(fdv(tau, b1 + eps_tiny<T>())));
+      a2 = T(0.3125) * (T(1.0) + sq(fdv(tau, b2 + eps_tiny<T>())));
     } else {
       // FP32: (tau / beta)^2 overflows the float range once beta is ~19
       // orders below tau, and inf / inf would follow.  The weights are
@@ -93,15 +129,15 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
     }
     T s = a0 + a1 + a2;
     T r = a0 * u0 + a1 * u1 + a2 * u2;
-    return r / s;
+    return fdv(r, s);
   } else if (SCH == SCH_TENO) {
     // Teno5_Z (interScheme.hpp:384-452); ct = 1e-5 (CT4: 1e-4, CT7: 1e-7)
     T tau = fabs(b2 - b0);
     T t0, t1, t2;
     if (sizeof(T) == sizeof(double)) {
-      t0 = T(1.0) + tau / (b0 + eps_tiny<T>());
-      t1 = T(1.0) + tau / (b1 + eps_tiny<T>());
-      t2 = T(1.0) + tau / (b2 + eps_tiny<T>());
+      t0 = T(1.0) + fdv(tau, b0 + eps_tiny<T>());
+      t1 = T(1.0) + fdv(tau, b1 + eps_tiny<T>());
+      t2 = T(1.0) + fdv(tau, b2 + eps_tiny<T>());
     } else {
       // FP32: (1 + tau / beta)^6 overflows for tau / beta > ~2.6e6.  The
       // cut-off test g_k < ct * sum(g) is invariant to a common factor, so
@@ -145,23 +181,23 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
 // form appends w*w to the 2D sum.
 template <int D, class T> __device__ __forceinline__ void cons_to_prim(const T* c, T* p) {
   T r = c[0];
-  T u = c[1] / r;
+  T u = fdv(c[1], r);
   if (D == 1) {
-    T E = c[2] / r;
+    T E = fdv(c[2], r);
     T e = -u * u / 2 + E;
     T RT = (T(CFD_GAMMA) - T(1)) * e;
     p[0] = r; p[1] = u; p[2] = r * RT;
   } else if (D == 2) {
-    T v = c[2] / r;
-    T E = c[3] / r;
+    T v = fdv(c[2], r);
+    T E = fdv(c[3], r);
     T q2 = (u * u + v * v) / 2;
     T e = -q2 + E;
     T RT = (T(CFD_GAMMA) - T(1)) * e;
     p[0] = r; p[1] = u; p[2] = v; p[3] = r * RT;
   } else {
-    T v = c[2] / r;
-    T w = c[3] / r;
-    T E = c[4] / r;
+    T v = fdv(c[2], r);
+    T w = fdv(c[3], r);
+    T E = fdv(c[4], r);
     T q2 = (u * u + v * v + w * w) / 2;
     T e = -q2 + E;
     T RT = (T(CFD_GAMMA) - T(1)) * e;
@@ -174,14 +210,14 @@ template <int D, class T> __device__ __forceinline__ void prim_to_cons(const T* 
   T r = p[0];
   if (D == 1) {
     T ek = (p[1] * p[1]) / 2;
-    c[0] = r; c[1] = r * p[1]; c[2] = p[2] / (T(CFD_GAMMA) - T(1)) + r * ek;
+    c[0] = r; c[1] = r * p[1]; c[2] = fdv(p[2], T(CFD_GAMMA) - T(1)) + r * ek;
   } else if (D == 2) {
     T ek = (p[1] * p[1] + p[2] * p[2]) / 2;
-    c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = p[3] / (T(CFD_GAMMA) - T(1)) + r * ek;
+    c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = fdv(p[3], T(CFD_GAMMA) - T(1)) + r * ek;
   } else {
     T ek = (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]) / 2;
     c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = r * p[3];
-    c[4] = p[4] / (T(CFD_GAMMA) - T(1)) + r * ek;
+    c[4] = fdv(p[4], T(CFD_GAMMA) - T(1)) + r * ek;
   }
 }
 
@@ -190,16 +226,16 @@ template <int D, class T> __device__ __forceinline__ void prim_to_cons(const T* 
 template <int D, class T> __device__ __forceinline__ void eig_cons_to_prim(const T* c, T* p) {
   T rt = c[0];
   if (D == 1) {
-    T ut = c[1] / rt;
+    T ut = fdv(c[1], rt);
     T ekt = (ut * ut) / 2;
     p[0] = rt; p[1] = ut; p[2] = (T(CFD_GAMMA) - T(1)) * (c[2] - rt * ekt);
   } else if (D == 2) {
-    T ut = c[1] / rt, vt = c[2] / rt;
-    T rekt = (c[1] * c[1] + c[2] * c[2]) / rt / 2;
+    T ut = fdv(c[1], rt), vt = fdv(c[2], rt);
+    T rekt = fdv(c[1] * c[1] + c[2] * c[2], rt) / 2;
     p[0] = rt; p[1] = ut; p[2] = vt; p[3] = (T(CFD_GAMMA) - T(1)) * (c[3] - rekt);
   } else {
-    T ut = c[1] / rt, vt = c[2] / rt, wt = c[3] / rt;
-    T rekt = (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) / rt / 2;
+    T ut = fdv(c[1], rt), vt = fdv(c[2], rt), wt = fdv(c[3], rt);
+    T rekt = fdv(c[1] * c[1] + c[2] * c[2] + c[3] * c[3], rt) / 2;
     p[0] = rt; p[1] = ut; p[2] = vt; p[3] = wt; p[4] = (T(CFD_GAMMA) - T(1)) * (c[4] - rekt);
   }
 }
@@ -224,16 +260,16 @@ template <int D, int AX, class T> struct RoeSys {
     T HL, HR;
     T wl = T(0.0), wr = T(0.0);
     if (D == 2) {
-      HL = (ul * ul + vl * vl) / 2 + ppl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-      HR = (ur * ur + vr * vr) / 2 + ppr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
+      HL = (ul * ul + vl * vl) / 2 + fdv(fdv(ppl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+      HR = (ur * ur + vr * vr) / 2 + fdv(fdv(ppr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
     } else {
       wl = pl[3];
       wr = pr[3];
-      HL = (ul * ul + vl * vl + wl * wl) / 2 + ppl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-      HR = (ur * ur + vr * vr + wr * wr) / 2 + ppr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
+      HL = (ul * ul + vl * vl + wl * wl) / 2 + fdv(fdv(ppl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+      HR = (ur * ur + vr * vr + wr * wr) / 2 + fdv(fdv(ppr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
     }
-    T c1 = sqrt(rl), c2 = sqrt(rr);
-    T divisor = T(1.0) / (c1 + c2);
+    T c1 = fsq(rl), c2 = fsq(rr);
+    T divisor = fdv(T(1.0), c1 + c2);
     u = (c1 * ul + c2 * ur) * divisor;
     v = (c1 * vl + c2 * vr) * divisor;
     T ht = (c1 * HL + c2 * HR) * divisor;
@@ -247,7 +283,7 @@ template <int D, int AX, class T> struct RoeSys {
       ek = (u * u + v * v + w * w) / 2;
     }
     h = ht - ek;
-    c = sqrt((T(CFD_GAMMA) - T(1)) * h);
+    c = fsq((T(CFD_GAMMA) - T(1)) * h);
   }
 
   // Row k of the left eigenvector matrix (columns rho, rho u, rho v[, rho w], rho E).
@@ -265,17 +301,17 @@ template <int D, int AX, class T> struct RoeSys {
       if (D == 3) l[3] = w;
       l[NV - 1] = -T(1.0);
     } else if (k == 2) {
-      l[0] = (Vn / c + ek / h) / 2;
-      l[1] = (-nx / c - u / h) / 2;
-      l[2] = (-ny / c - v / h) / 2;
-      if (D == 3) l[3] = (-nz / c - w / h) / 2;
-      l[NV - 1] = T(1.0) / (T(2.0) * h);
+      l[0] = (fdv(Vn, c) + fdv(ek, h)) / 2;
+      l[1] = (fdv(-nx, c) - fdv(u, h)) / 2;
+      l[2] = (fdv(-ny, c) - fdv(v, h)) / 2;
+      if (D == 3) l[3] = (fdv(-nz, c) - fdv(w, h)) / 2;
+      l[NV - 1] = fdv(T(1.0), T(2.0) * h);
     } else if (k == 3) {
-      l[0] = (-Vn / c + ek / h) / 2;
-      l[1] = (nx / c - u / h) / 2;
-      l[2] = (ny / c - v / h) / 2;
-      if (D == 3) l[3] = (nz / c - w / h) / 2;
-      l[NV - 1] = T(1.0) / (T(2.0) * h);
+      l[0] = (fdv(-Vn, c) + fdv(ek, h)) / 2;
+      l[1] = (fdv(nx, c) - fdv(u, h)) / 2;
+      l[2] = (fdv(ny, c) - fdv(v, h)) / 2;
+      if (D == 3) l[3] = (fdv(nz, c) - fdv(w, h)) / 2;
+      l[NV - 1] = fdv(T(1.0), T(2.0) * h);
     } else {  // k == 4, 3D only: shear 2 along t2
       if (AX == 2) { l[0] = -v; l[1] = T(0.0); l[2] = T(1.0); l[3] = T(0.0); }
       else { l[0] = -w; l[1] = T(0.0); l[2] = T(0.0); l[3] = T(1.0); }
@@ -294,9 +330,9 @@ template <int D, int AX, class T> struct RoeSys {
         r[NV - 1] = ny * u - nx * v;
       }
     } else if (k == 1) {
-      r[0] = T(1.0) / h; r[1] = u / h; r[2] = v / h;
-      if (D == 3) r[3] = w / h;
-      r[NV - 1] = ek / h;
+      r[0] = fdv(T(1.0), h); r[1] = fdv(u, h); r[2] = fdv(v, h);
+      if (D == 3) r[3] = fdv(w, h);
+      r[NV - 1] = fdv(ek, h);
     } else if (k == 2) {
       r[0] = T(1.0); r[1] = u - nx * c; r[2] = v - ny * c;
       if (D == 3) r[3] = w - nz * c;
@@ -327,25 +363,25 @@ template <class T> struct RoeSys1 {
   __device__ __forceinline__ void build(const T* pl, const T* pr) {
     T rl = pl[0], ul = pl[1], ppl = pl[2];
     T rr = pr[0], ur = pr[1], ppr = pr[2];
-    T HL = (ul * ul) / 2 + ppl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-    T HR = (ur * ur) / 2 + ppr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-    T c1 = sqrt(rl), c2 = sqrt(rr);
-    T divisor = T(1.0) / (sqrt(rl) + sqrt(rr));
+    T HL = (ul * ul) / 2 + fdv(fdv(ppl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+    T HR = (ur * ur) / 2 + fdv(fdv(ppr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+    T c1 = fsq(rl), c2 = fsq(rr);
+    T divisor = fdv(T(1.0), fsq(rl) + fsq(rr));
     T u = (c1 * ul + c2 * ur) * divisor;
     T ht = (c1 * HL + c2 * HR) * divisor;
     T ek = (u * u) / 2;
     T h = ht - ek;
-    T c = sqrt((T(CFD_GAMMA) - T(1)) * h);
+    T c = fsq((T(CFD_GAMMA) - T(1)) * h);
     const T g1 = T(CFD_GAMMA) - T(1);
-    T f = T(0.5) * g1 / (c * c);
-    L[0] = (ht + c * (u - c) / g1) * f;
-    L[1] = (-u - c / g1) * f;
+    T f = fdv(T(0.5) * g1, c * c);
+    L[0] = (ht + fdv(c * (u - c), g1)) * f;
+    L[1] = (-u - fdv(c, g1)) * f;
     L[2] = T(1.0) * f;
-    L[3] = (-T(2.0) * ht + T(4.0) * c * c / g1) * f;
+    L[3] = (-T(2.0) * ht + fdv(T(4.0) * c * c, g1)) * f;
     L[4] = (T(2.0) * u) * f;
     L[5] = -T(2.0) * f;
-    L[6] = (ht - c * (u + c) / g1) * f;
-    L[7] = (-u + c / g1) * f;
+    L[6] = (ht - fdv(c * (u + c), g1)) * f;
+    L[7] = (-u + fdv(c, g1)) * f;
     L[8] = T(1.0) * f;
     R[0] = T(1.0); R[1] = T(1.0); R[2] = T(1.0);
     R[3] = u - c; R[4] = u; R[5] = u + c;
@@ -357,7 +393,7 @@ template <class T> struct RoeSys1 {
 // with q = rho (which 0) and q = gamma p (which 1).
 template <class T> __device__ __forceinline__ T lim_q(const T* W, int which) {
   if (which == 0) return W[0];
-  T u = W[1] / W[0];
+  T u = fdv(W[1], W[0]);
   T ekin = T(0.5) * u * u;
   T p = (T(CFD_GAMMA) - T(1)) * (W[2] - W[0] * ekin);
   return T(CFD_GAMMA) * p;
@@ -370,7 +406,7 @@ template <class T> __device__ __forceinline__ void positive_limit(const T* W, T*
     Wi[0] = W[0]; Wi[1] = W[1]; Wi[2] = W[2];
     return;
   }
-  T theta = (T(1e-10) - qw) / (qi - qw);
+  T theta = fdv(T(1e-10) - qw, qi - qw);
   for (int i = 0; i < 3; i++) Wi[i] = (1 - theta) * W[i] + theta * Wi[i];
 }
 
@@ -378,27 +414,27 @@ template <class T> __device__ __forceinline__ void positive_limit(const T* W, T*
 // HLLCFlux1D (zone_FD/src/include/fluxSchemes.hpp:74-125)
 template <class T> __device__ __forceinline__ void hllc1d(const T* L, const T* Rr, T* res) {
   T rl = L[0], ul = L[1], pl = L[2], rr = Rr[0], ur = Rr[1], pr = Rr[2];
-  T HL = pl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1)) + (ul * ul) / 2;
-  T HR = pr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1)) + (ur * ur) / 2;
-  T cl = sqrt(T(CFD_GAMMA) * (pl / rl));
-  T cr = sqrt(T(CFD_GAMMA) * (pr / rr));
+  T HL = fdv(fdv(pl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1)) + (ul * ul) / 2;
+  T HR = fdv(fdv(pr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1)) + (ur * ur) / 2;
+  T cl = fsq(T(CFD_GAMMA) * fdv(pl, rl));
+  T cr = fsq(T(CFD_GAMMA) * fdv(pr, rr));
   T SL = smin(ur - cr, ul - cl);
   T SR = smax(ul + cl, ur + cr);
-  T Sstar = ((pr - pl) + (rl * ul * (SL - ul) - rr * ur * (SR - ur))) / (rl * (SL - ul) - rr * (SR - ur));
+  T Sstar = fdv((pr - pl) + (rl * ul * (SL - ul) - rr * ur * (SR - ur)), rl * (SL - ul) - rr * (SR - ur));
   if (SL >= 0) {
     res[0] = rl * ul; res[1] = rl * ul * ul + pl; res[2] = rl * HL * ul;
   } else if (Sstar >= 0) {
     T pStar = pl + rl * (SL - ul) * (Sstar - ul);
-    T U[3] = {rl, rl * ul, pl / (T(CFD_GAMMA) - T(1)) + rl * ul * ul / 2};
+    T U[3] = {rl, rl * ul, fdv(pl, T(CFD_GAMMA) - T(1)) + rl * ul * ul / 2};
     T F[3] = {rl * ul, rl * ul * ul + pl, rl * HL * ul};
     T Dd[3] = {T(0.0), T(1.0), Sstar};
-    for (int i = 0; i < 3; i++) res[i] = (Sstar * (SL * U[i] - F[i]) + SL * pStar * Dd[i]) / (SL - Sstar);
+    for (int i = 0; i < 3; i++) res[i] = fdv(Sstar * (SL * U[i] - F[i]) + SL * pStar * Dd[i], SL - Sstar);
   } else if (SR >= 0) {
     T pStar = pr + rr * (SR - ur) * (Sstar - ur);
-    T U[3] = {rr, rr * ur, pr / (T(CFD_GAMMA) - T(1)) + rr * ur * ur / 2};
+    T U[3] = {rr, rr * ur, fdv(pr, T(CFD_GAMMA) - T(1)) + rr * ur * ur / 2};
     T F[3] = {rr * ur, rr * ur * ur + pr, rr * HR * ur};
     T Dd[3] = {T(0.0), T(1.0), Sstar};
-    for (int i = 0; i < 3; i++) res[i] = (Sstar * (SR * U[i] - F[i]) + SR * pStar * Dd[i]) / (SR - Sstar);
+    for (int i = 0; i < 3; i++) res[i] = fdv(Sstar * (SR * U[i] - F[i]) + SR * pStar * Dd[i], SR - Sstar);
   } else {
     res[0] = rr * ur; res[1] = rr * ur * ur + pr; res[2] = rr * HR * ur;
   }
@@ -407,22 +443,22 @@ template <class T> __device__ __forceinline__ void hllc1d(const T* L, const T* R
 // roeFlux1D2 (fluxSchemes.hpp:13-71)
 template <class T> __device__ __forceinline__ void roe1d(const T* L, const T* Rr, T* res) {
   T rl = L[0], ul = L[1], pl = L[2], rr = Rr[0], ur = Rr[1], pr = Rr[2];
-  T HL = pl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1)) + (ul * ul) / 2;
-  T HR = pr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1)) + (ur * ur) / 2;
-  T rBar = sqrt(rl * rr);
-  T srl = sqrt(rl), srr = sqrt(rr);
-  T uBar = (ul * srl + ur * srr) / (srl + srr);
-  T HBar = (HL * srl + HR * srr) / (srl + srr);
-  T cBar = sqrt((T(CFD_GAMMA) - T(1)) * (HBar - uBar * uBar / 2));
+  T HL = fdv(fdv(pl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1)) + (ul * ul) / 2;
+  T HR = fdv(fdv(pr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1)) + (ur * ur) / 2;
+  T rBar = fsq(rl * rr);
+  T srl = fsq(rl), srr = fsq(rr);
+  T uBar = fdv(ul * srl + ur * srr, srl + srr);
+  T HBar = fdv(HL * srl + HR * srr, srl + srr);
+  T cBar = fsq((T(CFD_GAMMA) - T(1)) * (HBar - uBar * uBar / 2));
   T cBar2 = (T(CFD_GAMMA) - T(1)) * (HBar - uBar * uBar / 2);
   T dr = rr - rl, du = ur - ul, dp = pr - pl;
   T lam[3] = {fabs(uBar), fabs(uBar + cBar), fabs(uBar - cBar)};
   T eps = T(0.1) * (fabs(uBar) + cBar);
   for (int i = 0; i < 3; i++)
-    if (lam[i] < eps) lam[i] = (lam[i] * lam[i] + eps * eps) / (T(2.0) * eps);
-  T a0 = lam[0] * (dr - dp / cBar2);
-  T a1 = lam[1] / (T(2.0) * cBar2) * (dp + rBar * cBar * du);
-  T a2 = lam[2] / (T(2.0) * cBar2) * (dp - rBar * cBar * du);
+    if (lam[i] < eps) lam[i] = fdv(lam[i] * lam[i] + eps * eps, T(2.0) * eps);
+  T a0 = lam[0] * (dr - fdv(dp, cBar2));
+  T a1 = fdv(lam[1], T(2.0) * cBar2) * (dp + rBar * cBar * du);
+  T a2 = fdv(lam[2], T(2.0) * cBar2) * (dp - rBar * cBar * du);
   T a3 = a0 + a1 + a2;
   T a4 = cBar * (a1 - a2);
   T FL[3] = {rl * ul, rl * ul * ul + pl, rl * HL * ul};
@@ -430,7 +466,7 @@ template <class T> __device__ __forceinline__ void roe1d(const T* L, const T* Rr
   for (int i = 0; i < 3; i++) res[i] = (FL[i] + FR[i]) / T(2.0);
   res[0] -= T(0.5) * a3;
   res[1] -= T(0.5) * (a3 * uBar + a4);
-  res[2] -= T(0.5) * (HBar * a3 + uBar * a4 - cBar2 * a0 / (T(CFD_GAMMA) - T(1)));
+  res[2] -= T(0.5) * (HBar * a3 + uBar * a4 - fdv(cBar2 * a0, T(CFD_GAMMA) - T(1)));
 }
 
 // HLLCFlux2D2 (fluxSchemes.hpp:202-327), pressure-based wave speeds, with the
@@ -442,21 +478,21 @@ template <int D, int AX, class T> __device__ __forceinline__ void hllc_md(const 
   T rl = Lp[0], rr = Rp[0], pl = Lp[E], pr = Rp[E];
   T HL, HR;
   if (D == 2) {
-    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2]) / 2 + pl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2]) / 2 + pr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
+    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2]) / 2 + fdv(fdv(pl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2]) / 2 + fdv(fdv(pr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
   } else {
-    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2] + Lp[3] * Lp[3]) / 2 + pl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2] + Rp[3] * Rp[3]) / 2 + pr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
+    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2] + Lp[3] * Lp[3]) / 2 + fdv(fdv(pl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2] + Rp[3] * Rp[3]) / 2 + fdv(fdv(pr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
   }
   T VnL = Lp[1 + AX], VnR = Rp[1 + AX];
-  T cl = sqrt(pl / rl * T(CFD_GAMMA));
-  T cr = sqrt(pr / rr * T(CFD_GAMMA));
+  T cl = fsq(fdv(pl, rl) * T(CFD_GAMMA));
+  T cr = fsq(fdv(pr, rr) * T(CFD_GAMMA));
   T rBar = (rl + rr) / 2, cBar = (cl + cr) / 2;
   T ps = smax(T(0.0), (pl + pr) / 2 - rBar * cBar * (VnR - VnL) / 2);
-  T ql = (ps <= pl) ? T(1.0) : sqrt(T(1.0) + (T(CFD_GAMMA) + T(1.0)) / (T(2.0) * T(CFD_GAMMA)) * (ps / pl - T(1.0)));
-  T qr = (ps <= pr) ? T(1.0) : sqrt(T(1.0) + (T(CFD_GAMMA) + T(1.0)) / (T(2.0) * T(CFD_GAMMA)) * (ps / pr - T(1.0)));
+  T ql = (ps <= pl) ? T(1.0) : fsq(T(1.0) + (T(CFD_GAMMA) + T(1.0)) / (T(2.0) * T(CFD_GAMMA)) * (fdv(ps, pl) - T(1.0)));
+  T qr = (ps <= pr) ? T(1.0) : fsq(T(1.0) + (T(CFD_GAMMA) + T(1.0)) / (T(2.0) * T(CFD_GAMMA)) * (fdv(ps, pr) - T(1.0)));
   T SL = VnL - cl * ql, SR = VnR + cr * qr;
-  T Sstar = ((pr - pl) + (rl * VnL * (SL - VnL) - rr * VnR * (SR - VnR))) / (rl * (SL - VnL) - rr * (SR - VnR));
+  T Sstar = fdv((pr - pl) + (rl * VnL * (SL - VnL) - rr * VnR * (SR - VnR)), rl * (SL - VnL) - rr * (SR - VnR));
 
   auto side = [&](const T* P, T r, T p, T H, T Vn, T* F) {
     F[0] = r * Vn;
@@ -466,7 +502,7 @@ template <int D, int AX, class T> __device__ __forceinline__ void hllc_md(const 
   auto star = [&](const T* P, T r, T p, T H, T Vn, T S, T* out) {
     T F[NV];
     side(P, r, p, H, Vn, F);
-    T Usf = (S - Vn) / (S - Sstar);
+    T Usf = fdv(S - Vn, S - Sstar);
     T UE = r * H - p;
     out[0] = F[0] + S * (Usf * r - r);
     for (int k = 0; k < D; k++) {
@@ -476,7 +512,7 @@ template <int D, int AX, class T> __device__ __forceinline__ void hllc_md(const 
       T Us = Usf * (Sstar * n[k] + P[1 + k] * other) * r;
       out[1 + k] = F[1 + k] + S * (Us - r * P[1 + k]);
     }
-    T UsE = Usf * (UE + (Sstar - Vn) * (r * Sstar + p / (S - Vn)));
+    T UsE = Usf * (UE + (Sstar - Vn) * (r * Sstar + fdv(p, S - Vn)));
     out[E] = F[E] + S * (UsE - UE);
   };
   if (SL > 0) {
@@ -503,31 +539,31 @@ template <int D, int AX, class T> __device__ __forceinline__ void roe_md(const T
   T rl = Lp[0], rr = Rp[0], pl = Lp[E], pr = Rp[E];
   T HL, HR;
   if (D == 2) {
-    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2]) / 2 + pl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2]) / 2 + pr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
+    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2]) / 2 + fdv(fdv(pl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2]) / 2 + fdv(fdv(pr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
   } else {
-    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2] + Lp[3] * Lp[3]) / 2 + pl / rl * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
-    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2] + Rp[3] * Rp[3]) / 2 + pr / rr * T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1));
+    HL = (Lp[1] * Lp[1] + Lp[2] * Lp[2] + Lp[3] * Lp[3]) / 2 + fdv(fdv(pl, rl) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
+    HR = (Rp[1] * Rp[1] + Rp[2] * Rp[2] + Rp[3] * Rp[3]) / 2 + fdv(fdv(pr, rr) * T(CFD_GAMMA), T(CFD_GAMMA) - T(1));
   }
   T VnL = Lp[1 + AX], VnR = Rp[1 + AX];
-  T c1 = sqrt(rl), c2 = sqrt(rr);
-  T divisor = T(1.0) / (c1 + c2);
-  T rhoAvg = sqrt(rl * rr);
+  T c1 = fsq(rl), c2 = fsq(rr);
+  T divisor = fdv(T(1.0), c1 + c2);
+  T rhoAvg = fsq(rl * rr);
   T va[3];
   for (int k = 0; k < D; k++) va[k] = (c1 * Lp[1 + k] + c2 * Rp[1 + k]) * divisor;
   T HAvg = (c1 * HL + c2 * HR) * divisor;
   T q2 = (D == 2) ? (va[0] * va[0] + va[1] * va[1]) / 2 : (va[0] * va[0] + va[1] * va[1] + va[2] * va[2]) / 2;
-  T cAvg = sqrt((T(CFD_GAMMA) - T(1)) * (HAvg - q2));
+  T cAvg = fsq((T(CFD_GAMMA) - T(1)) * (HAvg - q2));
   T VnAvg = va[AX];
   T lam0 = fabs(VnAvg - cAvg), lam1 = fabs(VnAvg), lam2 = fabs(VnAvg + cAvg);
   T eps = T(0.05) * (fabs(VnAvg) + cAvg);
-  if (lam0 < eps) lam0 = (lam0 * lam0 + eps * eps) / (T(2.0) * eps);
-  if (lam1 < eps) lam1 = (lam1 * lam1 + eps * eps) / (T(2.0) * eps);
-  if (lam2 < eps) lam2 = (lam2 * lam2 + eps * eps) / (T(2.0) * eps);
+  if (lam0 < eps) lam0 = fdv(lam0 * lam0 + eps * eps, T(2.0) * eps);
+  if (lam1 < eps) lam1 = fdv(lam1 * lam1 + eps * eps, T(2.0) * eps);
+  if (lam2 < eps) lam2 = fdv(lam2 * lam2 + eps * eps, T(2.0) * eps);
   T dP = pr - pl, dVn = VnR - VnL, dRho = rr - rl;
-  T cf1 = (dP - rhoAvg * cAvg * dVn) / (T(2.0) * cAvg * cAvg);
-  T cf = (dP + rhoAvg * cAvg * dVn) / (T(2.0) * cAvg * cAvg);
-  T cf2 = dRho - dP / (cAvg * cAvg);
+  T cf1 = fdv(dP - rhoAvg * cAvg * dVn, T(2.0) * cAvg * cAvg);
+  T cf = fdv(dP + rhoAvg * cAvg * dVn, T(2.0) * cAvg * cAvg);
+  T cf2 = dRho - fdv(dP, cAvg * cAvg);
   T shear = T(0.0);
   bool first = true;
   for (int k = 0; k < D; k++) {

@@ -292,7 +292,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
         for (int v = 0; v < NV; v++) {
           const T H0 = flux[fidx(v, f0, lp)], H1 = flux[fidx(v, f1, lp)];
           const T H2 = flux[fidx(v, f2, lp)], H3 = flux[fidx(v, f3, lp)];
-          const T term = (T(w1) * (fp[v] - fm[v]) + T(w0) * (H2 - H1) + T(w2) * (H3 - H0)) / hh;
+          const T term = fdv(T(w1) * (fp[v] - fm[v]) + T(w0) * (H2 - H1) + T(w2) * (H3 - H0), hh);
           const T rold = (a.flags & SW_READ_R) ? Rt[v * a.N + idx] : T(0.0);
           const T r = rold + term;
           if (a.flags & SW_WRITE_R) Rt[v * a.N + idx] = r;
