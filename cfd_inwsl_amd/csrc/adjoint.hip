@@ -20,7 +20,9 @@
 // k adds to cell k-3+j in phase j, so no two lanes meet and the summation
 // order is fixed: results are deterministic); finally each lane finalises one
 // cell whose six interfaces are done.  Ghost-cell cotangents go to a small
-// per-pencil buffer and a second kernel folds them onto their source cells.
+// per-pencil buffer; the x sweep's are folded onto their source cells by the
+// y sweep's cell phase (stream order makes them visible), the y sweep's by a
+// small kernel after it.
 //
 // RK3 (BlockSolver::RK3_SSP, zone_FD/src/src/blockSolver.cpp:43-87):
 //   u1 = u0 - dt R(u0); u2 = 3/4 u0 + 1/4 u1 - 1/4 dt R(u1);
@@ -76,7 +78,10 @@ struct VjpArgs {
   double* OUT;       // result, SoA (may alias an X of init)
   VjpInit init;
   int use_init;
-  double* GH;        // ghost cotangents [pencil][8][4]
+  double* GH;        // ghost cotangents of this direction [pencil][8][4]
+  const double* GHX; // y sweep: the x sweep's ghost cotangents, folded in here
+  int gx_src[8];     // y sweep: source column of each x ghost slot (-1: fixed state)
+  int gx_flip[8];    // y sweep: velocity component the x ghost negates (-1: none)
   double coef;
   long N;
   int n[2];
@@ -105,9 +110,8 @@ template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(c
     es.lrow(k, l);
     double ch[6];
     for (int j = 0; j < 6; j++) {
-      double p[4], c[4];
-      for (int v = 0; v < 4; v++) p[v] = w(j, v);
-      cfd::prim_to_cons<2>(p, c);
+      double c[4];
+      for (int v = 0; v < 4; v++) c[v] = w.cons(j, v);
       ch[j] = cfd::project<2>(c, l);
     }
     for (int v = 0; v < 4; v++) f.L[k][v] = l[v];
@@ -149,7 +153,7 @@ template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(c
 template <int AX, int CA, int CP>
 __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
-  __shared__ double wc[4 * RC * CP];  // primitive state
+  __shared__ double wc[8 * RC * CP];  // primitive state, then the conservative state rebuilt from it
   __shared__ double mc[4 * RC * CP];  // coef * cotangent
   __shared__ double ac[4 * RC * CP];  // accumulated primitive cotangent
   auto idx = [](int v, int slot, int p) -> int {
@@ -198,8 +202,11 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
           if (v == gs.flip) p[v] = -p[v];
       }
     }
+    double cc[4];
+    cfd::prim_to_cons<2>(p, cc);
     for (int v = 0; v < 4; v++) {
       wc[idx(v, s, lp)] = p[v];
+      wc[idx(4 + v, s, lp)] = cc[v];
       mc[idx(v, s, lp)] = m[v];
       ac[idx(v, s, lp)] = 0.0;
     }
@@ -208,11 +215,13 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   struct Cells {
     const double* wc;
     int k, lp;
-    __device__ double operator()(int j, int v) const {
+    __device__ double at(int j, int v) const {
       const int c = k - 3 + j;
       const int s = (c + 8 * RC) % RC;
       return wc[AX == 0 ? (v * CP + lp) * RC + s : (v * RC + s) * CP + lp];
     }
+    __device__ double operator()(int j, int v) const { return at(j, v); }
+    __device__ double cons(int j, int v) const { return at(j, 4 + v); }
   };
 
   if (pvalid && la < 5) load(kmin - 3 + la);
@@ -256,6 +265,16 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
           }
           cfdadj::node_flux_vjp<AX>(p, lf, lwc);
         }
+        if (AX == 1 && c >= 0 && c < n && (gp < 4 || gp >= nx - 4)) {
+          // x ghosts of row c whose source is this column (sources lie within
+          // four cells of the row ends)
+          for (int q = 0; q < 8; q++)
+            if (a.gx_src[q] == gp)
+              for (int v = 0; v < 4; v++) {
+                const double x = a.GHX[((long)c * 8 + q) * 4 + v];
+                lwc[v] += v == a.gx_flip[q] ? -x : x;
+              }
+        }
         if (c >= 0 && c < n) {
           const long i = base + (long)c * sa;
           double u[4], g[4] = {0.0, 0.0, 0.0, 0.0};
@@ -283,19 +302,17 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   }
 }
 
-// Ghost cotangents onto their source cells.  One thread per (pencil, ghost):
+// Ghost cotangents of the y sweep onto their source cells (the x sweep's are
+// folded into the y sweep's cell phase).  One thread per (pencil, ghost slot):
 // the first ghost (in slot order) of each source cell sums every ghost that
 // copies that cell, in slot order, and applies the transpose of consToPrim
-// once -- distinct threads write distinct cells and the sums are
-// deterministic.
-template <int AX> __global__ __launch_bounds__(256) void k_ghost_fold(VjpArgs a) {
+// once -- one writer per cell, deterministic sums.
+__global__ __launch_bounds__(256) void k_ghost_fold_y(VjpArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long gp = t >> 3;
   const int g = (int)(t & 7);
-  const int nx = a.n[0], ny = a.n[1];
-  const int n = a.n[AX];
-  const long npencil = AX == 0 ? ny : nx;
-  if (gp >= npencil) return;
+  const int nx = a.n[0], n = a.n[1];
+  if (gp >= nx) return;
   auto src_of = [&](int q) {
     return cfdadj::ghost_source(q < 4 ? a.bc_lo : a.bc_hi, q < 4 ? q - 4 : n + q - 4, n);
   };
@@ -314,8 +331,7 @@ template <int AX> __global__ __launch_bounds__(256) void k_ghost_fold(VjpArgs a)
       l[v] += v == o.flip ? -x : x;
     }
   }
-  const long sa = AX == 0 ? 1 : nx;
-  const long i = (AX == 0 ? gp * nx : gp) + (long)gs.src * sa;
+  const long i = gp + (long)gs.src * nx;
   double u[4], gr[4] = {0.0, 0.0, 0.0, 0.0};
   for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
   cfdadj::c2p_vjp(u, l, gr);
@@ -339,7 +355,7 @@ __global__ void k_put_dt(double* scal, double dt) { scal[0] = dt; }
 struct AdjointStore {
   size_t state_bytes = 0;
   double *LA = nullptr, *LB = nullptr, *LC = nullptr;  // cotangent buffers
-  double* GH = nullptr;                                // ghost cotangents
+  double* GH[2] = {nullptr, nullptr};                  // ghost cotangents per direction
   double* stage = nullptr;                             // segment stage states (3K+1)
   int stage_cap = 0;                                   // states the stage buffer holds
   std::vector<double*> ckpt;                           // checkpointed states
@@ -361,7 +377,8 @@ void cfd_adjoint_release(cfd_handle* h) {
   (void)hipFree(s->LA);
   (void)hipFree(s->LB);
   (void)hipFree(s->LC);
-  (void)hipFree(s->GH);
+  (void)hipFree(s->GH[0]);
+  (void)hipFree(s->GH[1]);
   (void)hipFree(s->stage);
   for (double* p : s->ckpt) (void)hipFree(p);
   delete s;
@@ -420,7 +437,7 @@ static int adj_ensure(cfd_handle* h) {
   AHIP(hipMalloc(&s->LA, s->state_bytes));
   AHIP(hipMalloc(&s->LB, s->state_bytes));
   AHIP(hipMalloc(&s->LC, s->state_bytes));
-  AHIP(hipMalloc(&s->GH, sizeof(double) * np * 8 * 4));
+  for (int ax = 0; ax < 2; ax++) AHIP(hipMalloc(&s->GH[ax], sizeof(double) * np * 8 * 4));
   for (int ax = 0; ax < 2; ax++) {
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
     const int ca = ax == 0 ? VjpTile<0>::CA : VjpTile<1>::CA;
@@ -441,7 +458,15 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     a.U = U;
     a.MU = MU;
     a.OUT = OUT;
-    a.GH = s->GH;
+    a.GH = s->GH[ax];
+    a.GHX = s->GH[0];
+    for (int q = 0; q < 8; q++) {
+      const int nx = h->cfg.n[0];
+      const cfdadj::GhostSrc o =
+          cfdadj::ghost_source(q < 4 ? h->cfg.bc[0] : h->cfg.bc[1], q < 4 ? q - 4 : nx + q - 4, nx);
+      a.gx_src[q] = o.fixed ? -1 : o.src;
+      a.gx_flip[q] = o.flip;
+    }
     a.coef = coef;
     a.use_init = ax == 0 && init != nullptr;
     if (a.use_init) a.init = *init;
@@ -459,17 +484,15 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
     const unsigned nseg = (unsigned)((n + a.seg_len - 1) / a.seg_len);
-    const dim3 fold((unsigned)((pencils * 8 + 255) / 256));
     if (ax == 0) {
       constexpr int CA = VjpTile<0>::CA, CP = VjpTile<0>::CP;
       const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
       hipLaunchKernelGGL((k_rhs_vjp<0, CA, CP>), grid, dim3(CA * CP), 0, h->stream, a);
-      hipLaunchKernelGGL(k_ghost_fold<0>, fold, dim3(256), 0, h->stream, a);
     } else {
       constexpr int CA = VjpTile<1>::CA, CP = VjpTile<1>::CP;
       const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
       hipLaunchKernelGGL((k_rhs_vjp<1, CA, CP>), grid, dim3(CA * CP), 0, h->stream, a);
-      hipLaunchKernelGGL(k_ghost_fold<1>, fold, dim3(256), 0, h->stream, a);
+      hipLaunchKernelGGL(k_ghost_fold_y, dim3((unsigned)((pencils * 8 + 255) / 256)), dim3(256), 0, h->stream, a);
     }
     AHIP(hipGetLastError());
   }

@@ -25,6 +25,14 @@
 #define CFD_HD
 #define CFD_UNROLL
 #endif
+// Reverse-pass code needs derivatives, not the forward's rounding: on the
+// device multiply-adds in it may contract to FMAs (the forward re-evaluation
+// that decides branches is compiled without contraction).
+#if defined(__clang__)
+#define CFD_CONTRACT _Pragma("clang fp contract(fast)")
+#else
+#define CFD_CONTRACT
+#endif
 
 namespace cfdadj {
 
@@ -51,6 +59,7 @@ CFD_HD inline double rcp(double x) {
 struct JS {
   double b[3], a[3], u[3], e[3], S, iS, out;
   CFD_HD void fwd(const double* q) {
+    CFD_CONTRACT
     double A0 = q[0] - 2.0 * q[1] + q[2], B0 = q[0] - 4.0 * q[1] + 3.0 * q[2];
     double A1 = q[1] - 2.0 * q[2] + q[3], B1 = q[1] - q[3];
     double A2 = q[2] - 2.0 * q[3] + q[4], B2 = 3.0 * q[2] - 4.0 * q[3] + q[4];
@@ -71,6 +80,7 @@ struct JS {
   }
   // adds lam * d(out)/dq to g
   CFD_HD void bwd(const double* q, double lam, double* g) const {
+    CFD_CONTRACT
     const double iS = lam * this->iS;
     double lu[3], lb[3];
     for (int k = 0; k < 3; k++) {
@@ -100,6 +110,7 @@ struct JS {
 
 // ---- primitive <-> conservative ---------------------------------------------------
 CFD_HD inline void p2c(const double* p, double* c) {
+  CFD_CONTRACT
   c[0] = p[0];
   c[1] = p[0] * p[1];
   c[2] = p[0] * p[2];
@@ -107,6 +118,7 @@ CFD_HD inline void p2c(const double* p, double* c) {
 }
 // g_p += (dc/dp)^T lc
 CFD_HD inline void p2c_vjp(const double* p, const double* lc, double* gp) {
+  CFD_CONTRACT
   double ek = 0.5 * (p[1] * p[1] + p[2] * p[2]);
   gp[0] += lc[0] + lc[1] * p[1] + lc[2] * p[2] + lc[3] * ek;
   gp[1] += lc[1] * p[0] + lc[3] * p[0] * p[1];
@@ -114,6 +126,7 @@ CFD_HD inline void p2c_vjp(const double* p, const double* lc, double* gp) {
   gp[3] += lc[3] * IG1;
 }
 CFD_HD inline void c2p(const double* c, double* p) {
+  CFD_CONTRACT
   double r = c[0], ir = rcp(r), u = c[1] * ir, v = c[2] * ir;
   p[0] = r;
   p[1] = u;
@@ -122,6 +135,7 @@ CFD_HD inline void c2p(const double* c, double* p) {
 }
 // g_c += (dp/dc)^T lp
 CFD_HD inline void c2p_vjp(const double* c, const double* lp, double* gc) {
+  CFD_CONTRACT
   double r = c[0], ir = rcp(r), u = c[1] * ir, v = c[2] * ir;
   // p1 = c1/r, p2 = c2/r, p3 = G1 (c3 - (c1^2 + c2^2)/(2r))
   gc[0] += lp[0] - lp[1] * u * ir - lp[2] * v * ir + lp[3] * G1 * 0.5 * (u * u + v * v);
@@ -140,6 +154,7 @@ template <int AX> CFD_HD inline void node_flux(const double* p, double* F) {
   F[3] = (0.5 * (u * u + v * v) * r + GG1 * pp) * Vn;
 }
 template <int AX> CFD_HD inline void node_flux_vjp(const double* p, const double* lF, double* gp) {
+  CFD_CONTRACT
   double r = p[0], u = p[1], v = p[2], pp = p[3];
   double Vn = AX == 0 ? u : v;
   double E = 0.5 * (u * u + v * v) * r + GG1 * pp;
@@ -170,6 +185,7 @@ CFD_HD inline int hb_region(int b) { return (b >> 3) & 7; }
 // forward evaluation, whose rounding can differ where a predicate is tied).
 template <int AX> CFD_HD inline void hllc(const double* L, const double* R, double* F, double* lL = nullptr,
                                           double* lR = nullptr, const double* lF = nullptr, int force = -1) {
+  CFD_CONTRACT
   const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
   const double rl = L[0], ul = L[1], vl = L[2], pl = L[3];
   const double rr = R[0], ur = R[1], vr = R[2], pr = R[3];
@@ -410,6 +426,7 @@ template <int AX> struct Roe2 {
   double c1, c2, D, HL, HR;
   double u, v, ht, Vn, ek, h, c, iD, ic, ih;
   CFD_HD void fwd(const double* pl, const double* pr) {
+    CFD_CONTRACT
     HL = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2]) + GG1 * pl[3] * rcp(pl[0]);
     HR = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]) + GG1 * pr[3] * rcp(pr[0]);
     c1 = sqrt(pl[0]);
@@ -428,6 +445,7 @@ template <int AX> struct Roe2 {
   }
   // left eigenvector matrix (rows = waves) and right (R[i][k])
   CFD_HD void mats(double L[4][4], double R[4][4]) const {
+    CFD_CONTRACT
     const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
     L[0][0] = -ny * u + nx * v; L[0][1] = ny; L[0][2] = -nx; L[0][3] = 0.0;
     L[1][0] = h - ek; L[1][1] = u; L[1][2] = v; L[1][3] = -1.0;
@@ -446,6 +464,7 @@ template <int AX> struct Roe2 {
   };
   // g = cotangent of row k of L
   CFD_HD void acc_Lrow(int k, const double* g, Acc& a) const {
+    CFD_CONTRACT
     const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
     const double ic2 = ic * ic, ih2 = ih * ih;
     if (k == 0) {
@@ -473,6 +492,7 @@ template <int AX> struct Roe2 {
   }
   // gR(i, k) = cotangent of R[i][k]
   template <class GR> CFD_HD void acc_R(const GR& gR, Acc& a) const {
+    CFD_CONTRACT
     const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
     const double ih2 = ih * ih;
     const double r30 = gR(3, 0), r01 = gR(0, 1), r11 = gR(1, 1), r21 = gR(2, 1), r31 = gR(3, 1);
@@ -492,6 +512,7 @@ template <int AX> struct Roe2 {
     a.lc += -Vn * r32 + Vn * r33;
   }
   CFD_HD void finish(const Acc& a0, const double* pl, const double* pr, double* lpl, double* lpr) const {
+    CFD_CONTRACT
     double lu = a0.lu, lv = a0.lv, lh = a0.lh, lek = a0.lek;
     // c = sqrt(G1 h); h = ht - ek; ek = (u^2+v^2)/2; Vn = u | v
     lh += a0.lc * 0.5 * G1 * ic;
@@ -686,7 +707,8 @@ template <int AX> CFD_HD inline int hllc_branch_bits(const double* Lp, const dou
 }
 
 // Forward record in this file's own arithmetic (host checks).  The six
-// cells are read through W(j, v); the interface lies between cells 2 and 3.
+// cells are read through w(j, v) (primitive) and w.cons(j, v) (the
+// conservative state rebuilt from it); the interface lies between cells 2 and 3.
 template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd& f) {
   double pl[4], pr[4];
   for (int v = 0; v < 4; v++) {
@@ -698,12 +720,8 @@ template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd
   es.mats(f.L, f.R);
   for (int k = 0; k < 4; k++) {
     double ch[6];
-    for (int j = 0; j < 6; j++) {
-      double p[4], c[4];
-      for (int v = 0; v < 4; v++) p[v] = w(j, v);
-      p2c(p, c);
-      ch[j] = f.L[k][0] * c[0] + f.L[k][1] * c[1] + f.L[k][2] * c[2] + f.L[k][3] * c[3];
-    }
+    for (int j = 0; j < 6; j++)
+      ch[j] = f.L[k][0] * w.cons(j, 0) + f.L[k][1] * w.cons(j, 1) + f.L[k][2] * w.cons(j, 2) + f.L[k][3] * w.cons(j, 3);
     double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
     double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
     JS a, b;
@@ -738,6 +756,7 @@ template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd
 // projection cotangents go straight through p2c's transpose wave by wave.
 template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF,
                                                              double lw[6][4]) {
+  CFD_CONTRACT
   double loL[4] = {0, 0, 0, 0}, loR[4] = {0, 0, 0, 0};
   {
     double Fd[4];
@@ -776,12 +795,8 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
   for (int k = 0; k < 4; k++) {
     double ch[6];
     CFD_UNROLL
-    for (int j = 0; j < 6; j++) {
-      double p[4], c[4];
-      for (int v = 0; v < 4; v++) p[v] = w(j, v);
-      p2c(p, c);
-      ch[j] = f.L[k][0] * c[0] + f.L[k][1] * c[1] + f.L[k][2] * c[2] + f.L[k][3] * c[3];
-    }
+    for (int j = 0; j < 6; j++)
+      ch[j] = f.L[k][0] * w.cons(j, 0) + f.L[k][1] * w.cons(j, 1) + f.L[k][2] * w.cons(j, 2) + f.L[k][3] * w.cons(j, 3);
     double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
     double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
     double gl[5] = {0, 0, 0, 0, 0}, gr[5] = {0, 0, 0, 0, 0};
@@ -802,11 +817,10 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
     double gLk[4] = {0, 0, 0, 0};
     CFD_UNROLL
     for (int j = 0; j < 6; j++) {
-      double p[4], c[4], lt[4];
-      for (int v = 0; v < 4; v++) p[v] = w(j, v);
-      p2c(p, c);
+      double p[4], lt[4];
       for (int v = 0; v < 4; v++) {
-        gLk[v] += lch[j] * c[v];
+        p[v] = w(j, v);
+        gLk[v] += lch[j] * w.cons(j, v);
         lt[v] = lch[j] * f.L[k][v];
       }
       p2c_vjp(p, lt, lw[j]);

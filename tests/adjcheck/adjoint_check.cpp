@@ -76,6 +76,11 @@ void adj_interface_vjp_lean(int ax, const double* w, const double* lF, double* l
   struct Acc {
     const double* w;
     double operator()(int j, int v) const { return w[4 * j + v]; }
+    double cons(int j, int v) const {
+      double c[4];
+      p2c(&w[4 * j], c);
+      return c[v];
+    }
   } acc{w};
   double lww[6][4] = {};
   if (ax == 0) interface_vjp_lean<0>(acc, lF, lww);
