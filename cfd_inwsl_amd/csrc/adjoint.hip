@@ -80,15 +80,17 @@ struct VjpArgs {
   int use_init;
   double* GH;        // ghost cotangents of this direction [pencil][8][4]
   const double* GHX; // y sweep: the x sweep's ghost cotangents, folded in here
-  int gx_src[8];     // y sweep: source column of each x ghost slot (-1: fixed state)
-  int gx_flip[8];    // y sweep: velocity component the x ghost negates (-1: none)
+  // y sweep: per x ghost slot q (4 bits at 4q) its source column as c < 4 ? c
+  // : c - nx + 8 (15: fixed state), and (2 bits at 2q) the velocity
+  // component it negates (0: none)
+  unsigned gx_src, gx_flip;
   double coef;
   long N;
   int n[2];
   int seg_len;
   double h;
   int bc_lo, bc_hi;
-  double prim_lo[4], prim_hi[4];
+  const double* bc_prim;  // Dirichlet ghost states of this direction: low face [0..3], high [4..7]
 };
 
 // Forward record of one interface in the forward sweep's own arithmetic
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       const bool lo = c < 0;
       const cfdadj::GhostSrc gs = cfdadj::ghost_source(lo ? a.bc_lo : a.bc_hi, c, n);
       if (gs.fixed) {
-        for (int v = 0; v < 4; v++) p[v] = lo ? a.prim_lo[v] : a.prim_hi[v];
+        for (int v = 0; v < 4; v++) p[v] = a.bc_prim[(lo ? 0 : 4) + v];
       } else {
         const long i = base + (long)gs.src * sa;
         double u[4];
@@ -212,14 +214,18 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     }
   };
   auto M = [&](int c, int v) -> double { return mc[idx(v, slot_of(c), lp)]; };
+  // the six cells of interface k: ring offsets computed once
   struct Cells {
     const double* wc;
-    int k, lp;
-    __device__ double at(int j, int v) const {
-      const int c = k - 3 + j;
-      const int s = (c + 8 * RC) % RC;
-      return wc[AX == 0 ? (v * CP + lp) * RC + s : (v * RC + s) * CP + lp];
+    int off[6];
+    __device__ Cells(const double* w, int k, int lp) : wc(w) {
+      int s = (k - 3 + 8 * RC) % RC;
+      for (int j = 0; j < 6; j++) {
+        off[j] = AX == 0 ? lp * RC + s : s * CP + lp;
+        s = s + 1 == RC ? 0 : s + 1;
+      }
     }
+    __device__ double at(int j, int v) const { return wc[off[j] + v * RC * CP]; }
     __device__ double operator()(int j, int v) const { return at(j, v); }
     __device__ double cons(int j, int v) const { return at(j, 4 + v); }
   };
@@ -239,7 +245,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     if (act) {
       double lG[4];
       for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) * ih;
-      Cells cells{wc, k, lp};
+      const Cells cells(wc, k, lp);
       cfdadj::IfaceFwd f;
       iface_forward_exact<AX>(cells, f);
       cfdadj::iface_backward<AX>(cells, f, lG, lw);
@@ -268,12 +274,15 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
         if (AX == 1 && c >= 0 && c < n && (gp < 4 || gp >= nx - 4)) {
           // x ghosts of row c whose source is this column (sources lie within
           // four cells of the row ends)
+          const unsigned col = gp < 4 ? (unsigned)gp : (unsigned)(gp - nx + 8);
           for (int q = 0; q < 8; q++)
-            if (a.gx_src[q] == gp)
+            if (((a.gx_src >> (4 * q)) & 15u) == col) {
+              const int flip = (int)((a.gx_flip >> (2 * q)) & 3u);
               for (int v = 0; v < 4; v++) {
                 const double x = a.GHX[((long)c * 8 + q) * 4 + v];
-                lwc[v] += v == a.gx_flip[q] ? -x : x;
+                lwc[v] += (flip != 0 && v == flip) ? -x : x;
               }
+            }
         }
         if (c >= 0 && c < n) {
           const long i = base + (long)c * sa;
@@ -356,6 +365,7 @@ struct AdjointStore {
   size_t state_bytes = 0;
   double *LA = nullptr, *LB = nullptr, *LC = nullptr;  // cotangent buffers
   double* GH[2] = {nullptr, nullptr};                  // ghost cotangents per direction
+  double* bcprim = nullptr;                            // Dirichlet ghost states [dir][side][4]
   double* stage = nullptr;                             // segment stage states (3K+1)
   int stage_cap = 0;                                   // states the stage buffer holds
   std::vector<double*> ckpt;                           // checkpointed states
@@ -378,6 +388,7 @@ void cfd_adjoint_release(cfd_handle* h) {
   (void)hipFree(s->LB);
   (void)hipFree(s->LC);
   (void)hipFree(s->GH[0]);
+  (void)hipFree(s->bcprim);
   (void)hipFree(s->GH[1]);
   (void)hipFree(s->stage);
   for (double* p : s->ckpt) (void)hipFree(p);
@@ -438,6 +449,13 @@ static int adj_ensure(cfd_handle* h) {
   AHIP(hipMalloc(&s->LB, s->state_bytes));
   AHIP(hipMalloc(&s->LC, s->state_bytes));
   for (int ax = 0; ax < 2; ax++) AHIP(hipMalloc(&s->GH[ax], sizeof(double) * np * 8 * 4));
+  {
+    double bp[16];
+    for (int f = 0; f < 4; f++)
+      for (int v = 0; v < 4; v++) bp[4 * f + v] = h->cfg.bc_prim[f][v];
+    AHIP(hipMalloc(&s->bcprim, sizeof(bp)));
+    AHIP(hipMemcpy(s->bcprim, bp, sizeof(bp), hipMemcpyHostToDevice));
+  }
   for (int ax = 0; ax < 2; ax++) {
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
     const int ca = ax == 0 ? VjpTile<0>::CA : VjpTile<1>::CA;
@@ -460,12 +478,15 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     a.OUT = OUT;
     a.GH = s->GH[ax];
     a.GHX = s->GH[0];
+    a.gx_src = 0u;
+    a.gx_flip = 0u;
     for (int q = 0; q < 8; q++) {
       const int nx = h->cfg.n[0];
       const cfdadj::GhostSrc o =
           cfdadj::ghost_source(q < 4 ? h->cfg.bc[0] : h->cfg.bc[1], q < 4 ? q - 4 : nx + q - 4, nx);
-      a.gx_src[q] = o.fixed ? -1 : o.src;
-      a.gx_flip[q] = o.flip;
+      const unsigned code = o.fixed ? 15u : (unsigned)(o.src < 4 ? o.src : o.src - nx + 8);
+      a.gx_src |= code << (4 * q);
+      a.gx_flip |= (unsigned)(o.flip > 0 ? o.flip : 0) << (2 * q);
     }
     a.coef = coef;
     a.use_init = ax == 0 && init != nullptr;
@@ -477,10 +498,7 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     a.h = h->cfg.h[ax];
     a.bc_lo = h->cfg.bc[2 * ax];
     a.bc_hi = h->cfg.bc[2 * ax + 1];
-    for (int v = 0; v < 4; v++) {
-      a.prim_lo[v] = h->cfg.bc_prim[2 * ax][v];
-      a.prim_hi[v] = h->cfg.bc_prim[2 * ax + 1][v];
-    }
+    a.bc_prim = s->bcprim + 8 * ax;
     const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
     const unsigned nseg = (unsigned)((n + a.seg_len - 1) / a.seg_len);
