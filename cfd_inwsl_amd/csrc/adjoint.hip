@@ -400,6 +400,11 @@ static int adj_supported(cfd_handle* h) {
   if (h->D != 2 || h->fp32 || !h->cw || h->flx != 0 || h->sch != cfd::SCH_JS)
     return cfd_fail(CFD_E_UNSUPP,
                     "the adjoint covers 2D FP64 characteristic-wise WCNS5 (WENO-JS) with HLLC flux");
+  if (h->cfg.gravity[0] != 0.0 || h->cfg.gravity[1] != 0.0 || h->cfg.gravity[2] != 0.0)
+    return cfd_fail(CFD_E_UNSUPP, "the adjoint has no source term");
+  for (int f = 0; f < 4; f++)
+    if (h->cfg.bc[f] == CFD_BC_DMR_TOP || h->cfg.bc[f] == CFD_BC_DMR_WALL)
+      return cfd_fail(CFD_E_UNSUPP, "the adjoint has no time- or position-dependent boundaries");
   return 0;
 }
 

@@ -321,6 +321,8 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
   int sch;
   double ct;
   if (c->precision != CFD_PRECISION_FP64) return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches run in FP64");
+  if (c->gravity[0] != 0.0 || c->gravity[1] != 0.0 || c->gravity[2] != 0.0)
+    return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches take no source term");
   if (cfd_select_scheme(c->inter_method, &sch, &ct))
     return cfd_set_error_from_batch(CFD_E_UNSUPP, "inter_method is not implemented");
   cfd_batch* b = new cfd_batch();
@@ -350,6 +352,10 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
       zd.seg_len[d] = zd.n[d];
     }
     for (int f = 0; f < 6; f++) {
+      if (zones[z].bc[f] == BC_DMR_TOP || zones[z].bc[f] == BC_DMR_WALL) {
+        delete b;
+        return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches take no time- or position-dependent boundaries");
+      }
       zd.bc[f] = zones[z].bc[f];
       for (int v = 0; v < 5; v++) zd.prim[f][v] = zones[z].bc_prim[f][v];
     }

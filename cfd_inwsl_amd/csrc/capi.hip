@@ -55,13 +55,24 @@ int cfd_select_scheme(int inter, int* sch, double* ct) {
 static bool bc_known(int b) {
   switch (b) {
   case BC_PERIODIC: case BC_SYMMETRY1D: case BC_DIRICHLET: case BC_OUTFLOW: case BC_SYMMETRYX:
-  case BC_SYMMETRYY: case BC_SYMMETRYZ: case BC_MIRROR_COPY: return true;
+  case BC_SYMMETRYY: case BC_SYMMETRYZ: case BC_MIRROR_COPY: case BC_DMR_TOP: case BC_DMR_WALL: return true;
   }
   return false;
 }
 
 // ---- small kernels ---------------------------------------------------------------
 __global__ void k_set_scalar(double* p, double v) { *p = v; }
+
+// Time bookkeeping of BlockSolver::RK3_SSP + solve (blockSolver.cpp:62,74,86,
+// 123): t + dt - dt/2 - dt/2 + dt, operation by operation.
+__device__ __forceinline__ double step_time(double t, double dt) { return ((t + dt) - dt / 2 - dt / 2) + dt; }
+
+// Start of a fixed-dt step: dt, the step's start time, the time after it.
+__global__ void k_begin_step(double* scal, double dt) {
+  scal[0] = dt;
+  scal[7] = scal[6];
+  scal[6] = step_time(scal[6], dt);
+}
 
 __global__ void k_aos_to_soa(const double* __restrict__ in, double* __restrict__ out, long N, int NV) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -149,13 +160,18 @@ __global__ __launch_bounds__(256) void k_abs_max(const E* __restrict__ x, long N
     atomicMax(reinterpret_cast<unsigned long long*>(scal) + 5, (unsigned long long)__double_as_longlong(red[0]));
 }
 
-// dt = cfl / ((l0 + l1) + l2); t += dt
-template <int D> __global__ void k_finish_dt(double* scal, double cfl) {
+// dt = cfl / ((l0 + l1) + l2); with `advance` (the step follows on the
+// device) the clocks move on by dt as well
+template <int D> __global__ void k_finish_dt(double* scal, double cfl, int advance) {
   double lam = 0;
   for (int d = 0; d < D; d++) lam += scal[1 + d];
   double dt = cfl / lam;
   scal[0] = dt;
-  scal[4] += dt;
+  if (advance) {
+    scal[4] = step_time(scal[4], dt);
+    scal[7] = scal[6];
+    scal[6] = step_time(scal[6], dt);
+  }
 }
 
 // Kahan-compensated partial sums per block and variable.
@@ -188,6 +204,9 @@ __global__ __launch_bounds__(256) void k_totals(const double* __restrict__ U, lo
 }
 
 // ---- helpers -----------------------------------------------------------------------
+static bool has_source(const cfd_handle* h) {
+  return h->cfg.gravity[0] != 0.0 || h->cfg.gravity[1] != 0.0 || h->cfg.gravity[2] != 0.0;
+}
 hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int stage, const double* U0,
                                void* Uout, int tile, int seg, void* Ushadow) {
   SweepArgs a;
@@ -213,6 +232,10 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
     a.prim_lo[v] = h->cfg.bc_prim[2 * ax][v];
     a.prim_hi[v] = h->cfg.bc_prim[2 * ax + 1][v];
   }
+  a.x0 = h->cfg.origin;
+  a.hx = h->cfg.h[0];
+  for (int d = 0; d < 3; d++) a.grav[d] = h->cfg.gravity[d];
+  if (ax == h->D - 1 && has_source(h)) a.flags |= SW_SOURCE;
   const long nb = plan_pencil_blocks(h->cfg.n, ax, tile);
   const int nseg = (h->cfg.n[ax] + seg - 1) / seg;
   return launch_sweep_dispatch(h->D, ax, h->sch, h->flx, h->cw, tile, h->fp32, &a, nullptr,
@@ -242,22 +265,22 @@ static int rk3(cfd_handle* h) {
   return 0;
 }
 
-static int wave_speed(cfd_handle* h, double cfl) {
+static int wave_speed(cfd_handle* h, double cfl, int advance) {
   hipLaunchKernelGGL(k_zero_lambda, dim3(1), dim3(1), 0, h->stream, h->scal);
   int blocks = (int)std::min<long>((h->N + 255) / 256, 2048);
   const double* hh = h->cfg.h;
   switch (h->D) {
   case 1:
     hipLaunchKernelGGL(k_wave_speed<1>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
-    hipLaunchKernelGGL(k_finish_dt<1>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl);
+    hipLaunchKernelGGL(k_finish_dt<1>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance);
     break;
   case 2:
     hipLaunchKernelGGL(k_wave_speed<2>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
-    hipLaunchKernelGGL(k_finish_dt<2>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl);
+    hipLaunchKernelGGL(k_finish_dt<2>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance);
     break;
   default:
     hipLaunchKernelGGL(k_wave_speed<3>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
-    hipLaunchKernelGGL(k_finish_dt<3>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl);
+    hipLaunchKernelGGL(k_finish_dt<3>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance);
     break;
   }
   HIPCHK(hipGetLastError());
@@ -292,6 +315,10 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
       for (int s = 0; s < 2; s++) {
         int b = cfg->bc[2 * d + s];
         if (!bc_known(b)) return cfd_fail(CFD_E_ARG, "unknown boundary code " + std::to_string(b));
+        if ((b == BC_DMR_TOP && !(cfg->dim == 2 && d == 1 && s == 1)) ||
+            (b == BC_DMR_WALL && !(cfg->dim == 2 && d == 1 && s == 0)))
+          return cfd_fail(CFD_E_ARG, "the double-Mach boundaries belong on the y faces of a 2D grid "
+                                     "(DoubleMachUp on y-high, the wall on y-low)");
         if (b != BC_PERIODIC && b != BC_OUTFLOW && b != BC_DIRICHLET && cfg->n[d] < 4)
           return cfd_fail(CFD_E_ARG, "mirror boundaries need at least 4 cells along the axis");
       }
@@ -410,7 +437,7 @@ int cfd_eval_rhs(cfd_handle* h, double* rhs) {
 
 int cfd_rk3_step(cfd_handle* h, double dt) {
   if (!h) return cfd_fail(CFD_E_ARG, "null handle");
-  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal, dt);
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, h->stream, h->scal, dt);
   HIPCHK(hipGetLastError());
   return rk3(h);
 }
@@ -420,7 +447,7 @@ int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out) {
   if (!(cfl > 0)) return cfd_fail(CFD_E_ARG, "cfl must be > 0");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 4, 0.0);
   for (int s = 0; s < nsteps; s++) {
-    int rc = wave_speed(h, cfl);
+    int rc = wave_speed(h, cfl, 1);
     if (rc) return rc;
     if ((rc = rk3(h))) return rc;
   }
@@ -455,7 +482,7 @@ int cfd_rhs_linf(cfd_handle* h, int var, double* out) {
 
 int cfd_cfl_dt(cfd_handle* h, double cfl, double* dt) {
   if (!h || !dt) return cfd_fail(CFD_E_ARG, "null argument");
-  int rc = wave_speed(h, cfl);
+  int rc = wave_speed(h, cfl, 0);  // the clock moves with the step that uses dt
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(dt, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -580,6 +607,21 @@ int cfd_set_launch_plan(cfd_handle* h, const int* tiles, const int* segs) {
 
 int cfd_synchronize(cfd_handle* h) {
   if (!h) return cfd_fail(CFD_E_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
+int cfd_set_time(cfd_handle* h, double t) {
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 6, t);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
+int cfd_get_time(cfd_handle* h, double* t) {
+  if (!h || !t) return cfd_fail(CFD_E_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(t, h->scal + 6, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return CFD_OK;
 }

@@ -113,6 +113,8 @@ std::string setupCase(const Info& info, std::vector<double>& cons, cfd_config& c
     for (int v = 0; v < 5; v++) cfg.bc_prim[f][v] = 0.0;
   }
   auto xc = [&](int i) { return cmin + (i + 0.5) * h; };
+  cfg.origin = cmin;
+  for (int d = 0; d < 3; d++) cfg.gravity[d] = 0.0;
 
   if (info.dim == 1) {
     const int n = ic[0];
@@ -200,8 +202,27 @@ std::string setupCase(const Info& info, std::vector<double>& cons, cfd_config& c
           p = std::pow(r, G);
           break;
         }
-        case 3: return "case 3 (RTI) needs the gravity source term, which this path does not implement";
-        case 4: return "case 4 (double Mach reflection) needs the time-dependent DoubleMachUp boundary";
+        case 3:  // Rayleigh-Taylor (:396-433): heavy fluid below y = 0.5
+          if (y <= 0.5) {
+            r = 2.0;
+            p = 2.0 * y + 1.0;
+          } else {
+            r = 1.0;
+            p = y + 3.0 / 2.0;
+          }
+          v = -0.025 * std::sqrt(G * p / r) * std::cos(8.0 * M_PI * x);
+          break;
+        case 4:  // double Mach reflection (:436-468): oblique Mach-10 shock from x = 1/6
+          if (y >= std::sqrt(3) * (x - 1.0 / 6.0)) {
+            r = 8.0;
+            u = 8.25 * std::cos(M_PI / 6);
+            v = -8.25 * std::sin(M_PI / 6);
+            p = 116.5;
+          } else {
+            r = 1.4;
+            p = 1.0;
+          }
+          break;
         default: return "2D case " + std::to_string(info.nCase) + " is not defined";
         }
         put(cons, {r, r * u, r * v, 1.0 / (G - 1) * p + r * (u * u + v * v) / 2});
@@ -209,6 +230,29 @@ std::string setupCase(const Info& info, std::vector<double>& cons, cfd_config& c
     if (info.nCase == 2) {
       set_bc(cfg, 0, CFD_BC_SYMMETRYX, CFD_BC_SYMMETRYX);
       set_bc(cfg, 1, CFD_BC_SYMMETRYY, CFD_BC_SYMMETRYY);
+    } else if (info.nCase == 3) {
+      // initBnds (:796-799, 831-862): symmetric sides, Dirichlet bottom and top
+      set_bc(cfg, 0, CFD_BC_SYMMETRYX, CFD_BC_SYMMETRYX);
+      set_bc(cfg, 1, CFD_BC_DIRICHLET, CFD_BC_DIRICHLET);
+      const double lo[4] = {2.0, 0, 0, 1.0}, hi[4] = {1.0, 0, 0, 2.5};
+      for (int v = 0; v < 4; v++) {
+        cfg.bc_prim[2][v] = lo[v];
+        cfg.bc_prim[3][v] = hi[v];
+      }
+      // SourceTerm GRAVITY, g = (0, 1); the reference leaves it switched off
+      // (zoneMain.cpp:32 is commented out) -- driver flag --gravity
+      if (info.gravity) cfg.gravity[1] = 1.0;
+    } else if (info.nCase == 4) {
+      // initDoubleMachBnds (:880-934): post-shock Dirichlet inflow, outflow,
+      // the wall (Dirichlet for x < 1/6, reflecting beyond), DoubleMachUp on top
+      const double post[4] = {8.0, 8.25 * std::cos(M_PI / 6), -8.25 * std::sin(M_PI / 6), 116.5};
+      set_bc(cfg, 0, CFD_BC_DIRICHLET, CFD_BC_OUTFLOW);
+      set_bc(cfg, 1, CFD_BC_DMR_WALL, CFD_BC_DMR_TOP);
+      for (int v = 0; v < 4; v++) {
+        cfg.bc_prim[0][v] = post[v];
+        cfg.bc_prim[2][v] = post[v];
+        cfg.bc_prim[3][v] = post[v];
+      }
     } else if (info.nCase == 6) {
       // PERIODIC1D as the reference wires it for 2D Euler (a mirror copy), or a true periodic halo
       int b = info.truePeriodic ? CFD_BC_PERIODIC : CFD_BC_MIRROR_COPY;

@@ -29,7 +29,7 @@ static void usage() {
                "usage: congfd [info.txt] [--flux hllc|roe] [--recon char|component] [--inter N]\n"
                "              [--case N] [--dim N] [--imax I J K] [--zone x0 x1 y0 y1 z0 z1]\n"
                "              [--endstep N] [--outputdt T] [--cfl C] [--true-periodic] [--device N]\n"
-               "              [--outdir DIR] [--no-output] [--fp32]\n");
+               "              [--outdir DIR] [--no-output] [--fp32] [--gravity]\n");
 }
 
 #define CHECK(call)                                                                      \
@@ -105,6 +105,7 @@ int main(int argc, char** argv) {
     else if (s == "--outdir") outdir = next();
     else if (s == "--no-output") output = false;
     else if (s == "--fp32") info.precision = CFD_PRECISION_FP32;
+    else if (s == "--gravity") info.gravity = true;
     else { usage(); return 2; }
   }
 
@@ -126,6 +127,7 @@ int main(int argc, char** argv) {
   cfd_handle* h = nullptr;
   CHECK(cfd_create(&cfg, &h));
   CHECK(cfd_set_state(h, cons.data()));
+  CHECK(cfd_set_time(h, info.t));
   CHECK(cfd_set_keep_rhs(h, 1));
   std::printf("initialized %s, %s, %s, %d cells\n", info.caseName().c_str(), info.schemeName().c_str(),
               info.gridInfo().c_str(), cfg.n[0] * cfg.n[1] * cfg.n[2]);
@@ -160,7 +162,9 @@ int main(int argc, char** argv) {
     CHECK(cfd_synchronize(h));
     auto s1 = std::chrono::high_resolution_clock::now();
     solverMs += std::chrono::duration_cast<std::chrono::milliseconds>(s1 - s0).count();
-    info.t += dt;  // BlockSolver::solve: RK3_SSP leaves t unchanged, then t += dt
+    // RK3_SSP's stage bookkeeping then BlockSolver::solve (blockSolver.cpp:62-123);
+    // the library advances its own copy the same way
+    info.t = ((info.t + dt) - dt / 2 - dt / 2) + dt;
     timesteps++;
     double linf;
     CHECK(cfd_rhs_linf(h, 0, &linf));

@@ -31,6 +31,7 @@ struct Info {
   bool truePeriodic = false;                          // use a true periodic halo where the reference mirrors
   int device = 0;
   int precision = CFD_PRECISION_FP64;
+  bool gravity = false;                               // RTI: SourceTerm GRAVITY (off in the reference)
 
   std::array<int, 3> icMax() const;   // cells per direction (info.cpp:115-128)
   double geth() const;                // one spacing for every direction (info.cpp:310-314, initializer.cpp:594-597)

@@ -33,10 +33,12 @@ enum BcCode {
   BC_SYMMETRYY = 11,
   BC_SYMMETRYZ = 101,
   BC_MIRROR_COPY = 102,
+  BC_DMR_TOP = 8,     // DoubleMachUp (oneDBnd.cpp:87-108), y-high face of 2D
+  BC_DMR_WALL = 103,  // double-Mach wall wiring (initializer.cpp:916-934), y-low face of 2D
 };
 
 // flags of a sweep launch
-enum SweepFlags { SW_READ_R = 1, SW_WRITE_R = 2, SW_RK = 4 };
+enum SweepFlags { SW_READ_R = 1, SW_WRITE_R = 2, SW_RK = 4, SW_SOURCE = 8 };
 
 // Buffers are typed by the kernel's compute type T (double or float): U, R
 // and the stage outputs hold T; U0 (the state at the start of the step) is
@@ -58,6 +60,12 @@ struct SweepArgs {
   double ct;         // TENO cut-off constant
   int bc_lo, bc_hi;
   double prim_lo[5], prim_hi[5];
+  // position- and time-dependent boundaries (2D y sweeps): cell centres at
+  // x0 + (i + 0.5) hx in every direction (Initializer::initUniformBlock,
+  // initializer.cpp:672-677); time from dtp: [6] now, [7] start of the step
+  double x0, hx;
+  // SW_SOURCE: gravity source on the last sweep (SourceTerm::calGravitySource)
+  double grav[3];
 };
 
 // Tile shapes: CA interfaces per chunk along the sweep axis x CP pencils.
@@ -119,11 +127,35 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
   auto load_cell = [&](int g) {
     T pr[NV];
     int flip = -1;
-    bool dir = false;
+    bool dir = false, exact = false;
     int src = g;
     if (g < 0 || g >= n) {
       const bool lo = g < 0;
-      const int bc = lo ? a.bc_lo : a.bc_hi;
+      int bc = lo ? a.bc_lo : a.bc_hi;
+      if (D == 2 && AX == 1 && (bc == BC_DMR_WALL || bc == BC_DMR_TOP)) {
+        // the pencil's cell-centre x (2D y sweep: pencil index = column)
+        const double x = a.x0 + ((int)gp + 0.5) * a.hx;
+        if (bc == BC_DMR_WALL) {
+          bc = x < 1.0 / 6.0 ? BC_DIRICHLET : BC_SYMMETRYY;
+        } else {
+          // ghost k = g - n at y = -dh/2 + k dh from the face, time of the stage
+          // (RK3_SSP: t, t + dt, t + dt - dt/2; blockSolver.cpp:62,74)
+          const double dh = (a.x0 + ((n - 1) + 0.5) * a.hx) - (a.x0 + ((n - 2) + 0.5) * a.hx);
+          const double y = -dh / 2 + (g - n) * dh;
+          double t = a.dtp[6];
+          if (a.stage >= 1) {
+            t = a.dtp[7];
+            if (a.stage >= 2) t = t + a.dtp[0];
+            if (a.stage == 3) t = t - a.dtp[0] / 2;
+          }
+          const double gt = 1.0 / 6.0 + sqrt(3.0) / 3.0 * (1.0 + 20 * t);
+          const bool post = y > sqrt(3.0) * (x - gt);
+          for (int v = 0; v < NV; v++)
+            pr[v] = T(post ? a.prim_hi[v] : (v == 0 ? 1.4 : (v == NV - 1 ? 1.0 : 0.0)));
+          dir = true;
+          exact = true;
+        }
+      }
       switch (bc) {
       case BC_PERIODIC: src = ((g % n) + n) % n; break;
       case BC_OUTFLOW: src = lo ? 0 : n - 1; break;
@@ -134,7 +166,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       case BC_SYMMETRYZ: src = lo ? -1 - g : 2 * n - 1 - g; flip = 3; break;
       default: src = lo ? -1 - g : 2 * n - 1 - g; break;  // mirror copy
       }
-      if (dir)
+      if (dir && !exact)
         for (int v = 0; v < NV; v++) pr[v] = T(lo ? a.prim_lo[v] : a.prim_hi[v]);
     }
     if (!dir) {
@@ -289,12 +321,28 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
         double dtv = 0.0;
         if (a.flags & SW_RK) dtv = *a.dtp;
         const T hh = T(a.h);
+        // gravity source of the cell (SourceTerm::calGravitySource,
+        // sourceTerm.cpp:3-20, for an acceleration vector), subtracted from
+        // the rhs after every direction's flux derivative
+        T src[NV];
+        const bool source = AX == D - 1 && (a.flags & SW_SOURCE);
+        if (source) {
+          const int si = (i + 8 * RC) % RC;
+          T pc[NV];
+          for (int v = 0; v < NV; v++) pc[v] = cell[cidx(v, si, lp)];
+          T ug = pc[1] * T(a.grav[0]);
+          for (int d = 1; d < D; d++) ug = ug + pc[1 + d] * T(a.grav[d]);
+          src[0] = T(0.0);
+          for (int d = 0; d < D; d++) src[1 + d] = pc[0] * T(a.grav[d]);
+          src[NV - 1] = pc[0] * ug;
+        }
         for (int v = 0; v < NV; v++) {
           const T H0 = flux[fidx(v, f0, lp)], H1 = flux[fidx(v, f1, lp)];
           const T H2 = flux[fidx(v, f2, lp)], H3 = flux[fidx(v, f3, lp)];
           const T term = fdv(T(w1) * (fp[v] - fm[v]) + T(w0) * (H2 - H1) + T(w2) * (H3 - H0), hh);
           const T rold = (a.flags & SW_READ_R) ? Rt[v * a.N + idx] : T(0.0);
-          const T r = rold + term;
+          T r = rold + term;
+          if (source && v > 0) r = r - src[v];
           if (a.flags & SW_WRITE_R) Rt[v * a.N + idx] = r;
           if (a.flags & SW_RK) {
             // stage combination in FP64 (BlockSolver::RK3_SSP, blockSolver.cpp:60,72,84)
@@ -398,6 +446,9 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
       a.prim_lo[v] = zd.prim[2 * AX][v];
       a.prim_hi[v] = zd.prim[2 * AX + 1][v];
     }
+    a.x0 = 0.0;  // batches take neither position-dependent boundaries nor sources
+    a.hx = 0.0;
+    a.grav[0] = a.grav[1] = a.grav[2] = 0.0;
     sweep_block<D, AX, SCH, FLX, CHARW, TILE, double>(a, __builtin_amdgcn_readfirstlane(w.bx),
                                               __builtin_amdgcn_readfirstlane(w.by));
   }

@@ -45,6 +45,8 @@ class BC(enum.IntEnum):
     SYMMETRYY = 11
     SYMMETRYZ = 101
     MIRROR_COPY = 102
+    DMR_TOP = 8       # DoubleMachUp
+    DMR_WALL = 103    # double-Mach y-low wiring: Dirichlet for x < 1/6, SYMMETRYY beyond
 
 
 DIFF_MND6 = 3
@@ -63,6 +65,8 @@ class cfd_config(ctypes.Structure):
         ("bc_prim", (ctypes.c_double * 5) * 6),
         ("device", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("origin", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
     ]
 
 
@@ -88,7 +92,7 @@ EXPORTS = [
     "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf", "cfd_autotune", "cfd_launch_plan", "cfd_set_launch_plan",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_state", "cfd_batch_get_state", "cfd_batch_steps_cfl",
     "cfd_batch_times", "cfd_batch_autotune", "cfd_batch_synchronize",
-    "cfd_rhs_vjp", "cfd_adjoint_forward", "cfd_adjoint_backward",
+    "cfd_rhs_vjp", "cfd_adjoint_forward", "cfd_adjoint_backward", "cfd_set_time", "cfd_get_time",
 ]
 
 
@@ -142,6 +146,8 @@ def load_library():
     lib.cfd_batch_autotune.argtypes = [P, ctypes.c_int]
     lib.cfd_batch_synchronize.argtypes = [P]
     lib.cfd_rhs_vjp.argtypes = [P, pd, pd]
+    lib.cfd_set_time.argtypes = [P, ctypes.c_double]
+    lib.cfd_get_time.argtypes = [P, pd]
     lib.cfd_adjoint_forward.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int]
     lib.cfd_adjoint_backward.argtypes = [P, pd, pd]
     _lib = lib
@@ -166,7 +172,8 @@ class Solver:
     """
 
     def __init__(self, dim, n, h, inter_method=InterMethod.WCNS5, flux_method=FluxMethod.ROE,
-                 recon=Recon.CHAR, bc=None, bc_prim=None, device=0, precision=PRECISION_FP64):
+                 recon=Recon.CHAR, bc=None, bc_prim=None, device=0, precision=PRECISION_FP64, origin=0.0,
+                 gravity=(0.0, 0.0, 0.0)):
         self.lib = load_library()
         self.dim = dim
         self.nv = dim + 2
@@ -192,6 +199,9 @@ class Solver:
                 cfg.bc_prim[i][v] = float(bc_prim[i][v]) if bc_prim is not None else 0.0
         cfg.device = device
         cfg.precision = int(precision)
+        cfg.origin = float(origin)
+        for i in range(3):
+            cfg.gravity[i] = float(gravity[i]) if i < len(gravity) else 0.0
         self.cfg = cfg
         handle = ctypes.c_void_p()
         self._check(self.lib.cfd_create(ctypes.byref(cfg), ctypes.byref(handle)))
@@ -289,6 +299,14 @@ class Solver:
         t = (ctypes.c_int * 3)(*(list(tiles) + [1] * (3 - len(tiles))))
         g = (ctypes.c_int * 3)(*(list(segs) + [1] * (3 - len(segs))))
         self._check(self.lib.cfd_set_launch_plan(self.h, t, g))
+
+    def set_time(self, t):
+        self._check(self.lib.cfd_set_time(self.h, float(t)))
+
+    def get_time(self):
+        t = ctypes.c_double(0)
+        self._check(self.lib.cfd_get_time(self.h, ctypes.byref(t)))
+        return t.value
 
     # ---- discrete adjoint (2D FP64, char-wise WCNS5 + HLLC) ----
     def rhs_vjp(self, cot):

@@ -59,6 +59,12 @@ extern "C" {
 #define CFD_BC_SYMMETRYZ 101     /* extension: mirror, w negated */
 #define CFD_BC_MIRROR_COPY 102   /* the reference's 2D Euler PERIODIC1D wiring
                                     (initializer.cpp:806-822): mirror, no flip */
+#define CFD_BC_DMR_TOP 8         /* DoubleMachUp (oneDBnd.cpp:87-108), 2D y-high face:
+                                    bc_prim = post-shock state where y > sqrt(3)(x - g(t)),
+                                    (1.4, 0, 0, 1) elsewhere; needs origin and the time */
+#define CFD_BC_DMR_WALL 103      /* 2D y-low face of the double-Mach wiring
+                                    (initializer.cpp:916-934): Dirichlet bc_prim for
+                                    cell centres x < 1/6, SYMMETRYY beyond */
 
 typedef struct cfd_config {
   int dim;              /* 1, 2 or 3 (Info::dim) */
@@ -76,6 +82,13 @@ typedef struct cfd_config {
                            CFD_PRECISION_FP32 (FP32 storage and arithmetic of the
                            sweeps; U^n and the RK3 stage combination stay FP64);
                            FP32 is available for dim 2 and 3, single zones */
+  double origin;        /* cell centres at origin + (i + 0.5) h[0] in every direction
+                           (Initializer::initUniformBlock, initializer.cpp:672-677);
+                           used by the double-Mach boundaries */
+  double gravity[3];    /* gravity source (SourceTerm::calGravitySource,
+                           sourceTerm.cpp:3-20): rhs(rho u_d) -= rho g_d,
+                           rhs(rho E) -= rho (u . g); all zero: no source (the
+                           reference's RTI case uses g = (0, 1) when enabled) */
 } cfd_config;
 
 #define CFD_PRECISION_FP64 0
@@ -214,6 +227,12 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
  * its initial state: grad_initial = (d U(t_end) / d U(0))^T cot_final.  Each
  * checkpoint segment's stage states are recomputed on the device. */
 int cfd_adjoint_backward(cfd_handle* h, const double* cot_final, double* grad_initial);
+
+/* Simulated time of the handle (Info::t): set it before time-dependent
+ * boundaries are used; each RK3 step advances it as BlockSolver::RK3_SSP +
+ * solve do (t + dt - dt/2 - dt/2 + dt, blockSolver.cpp:62-123). */
+int cfd_set_time(cfd_handle* h, double t);
+int cfd_get_time(cfd_handle* h, double* t);
 
 /* Device pointer of the current state (structure of arrays, nv planes of
  * n[0]*n[1]*n[2] doubles) and of the rhs buffer. */

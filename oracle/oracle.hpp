@@ -671,13 +671,34 @@ enum Bc {
   BC_SYMMETRYY = 11,     // mirror, flip v
   BC_SYMMETRYZ = 101,    // EXTENSION: mirror, flip w
   BC_MIRROR_COPY = 102,  // the reference's 2D Euler PERIODIC1D wiring: mirror without flip
+  BC_DMR_TOP = 8,        // DoubleMachUp: exact oblique-shock state from x and t (oneDBnd.cpp:87-108)
+  BC_DMR_WALL = 103,     // initDoubleMachBnds y-low wiring (initializer.cpp:916-934): Dirichlet
+                         // for cell centres x < 1/6, SYMMETRYY beyond
+};
+
+// Where a ghost layer sits, for the position- and time-dependent codes: the
+// pencil's cell-centre x, the cell-centre spacing across the face and the
+// current (stage) time.
+struct GhostCtx {
+  double x = 0.0, dh = 0.0, t = 0.0;
 };
 
 template <class T, int D>
 inline void fill_ghost(int bc, const double* dir_prim, const T* prim_pencil, int n, int stride_v, bool left,
-                       T* ghost /* 4 x NV */) {
+                       T* ghost /* 4 x NV */, const GhostCtx& ctx = GhostCtx()) {
   constexpr int NV = D + 2;
+  if (bc == BC_DMR_WALL) bc = ctx.x < 1.0 / 6.0 ? BC_DIRICHLET : BC_SYMMETRYY;
   for (int k = 0; k < 4; k++) {
+    if (bc == BC_DMR_TOP) {
+      // oneDBnd.cpp:90-105: ghost k at y = -dh/2 + k dh from the face
+      const double y = -ctx.dh / 2 + k * ctx.dh;
+      const double gt = 1.0 / 6.0 + std::sqrt(3.0) / 3.0 * (1.0 + 20 * ctx.t);
+      const bool post = y > std::sqrt(3.0) * (ctx.x - gt);
+      // pre-shock state: rho 1.4 at rest, p 1
+      for (int v = 0; v < NV; v++)
+        ghost[k * NV + v] = T(post ? dir_prim[v] : v == 0 ? 1.4 : v == NV - 1 ? 1.0 : 0.0);
+      continue;
+    }
     int src;
     int flip = -1;
     switch (bc) {
@@ -711,6 +732,13 @@ struct Grid {
   int charwise;
   int bc[6];
   double bc_prim[6][5];
+  // EXTENSION fields used by the position/time-dependent boundaries and the
+  // source term: cell centres are x0 + (i + 0.5) h[0] in every direction
+  // (Initializer::initUniformBlock, initializer.cpp:672-677), t is the
+  // current time, gravity the acceleration of SourceTerm::calGravitySource.
+  double x0 = 0.0;
+  double t = 0.0;
+  double gravity[3] = {0.0, 0.0, 0.0};
 };
 
 // One direction sweep: SpDistributor::rhsSolve body for direction `ax`
@@ -744,8 +772,12 @@ void sweep(const Grid& g, int ax, const T* prim, T* rhs, int nthreads) {
       for (int j = 0; j < n; j++)
         for (int v = 0; v < NV; v++) pp[(j + 4) * NV + v] = prim[(base + j * sa) * NV + v];
       T gl[4 * NV], gr[4 * NV];
-      fill_ghost<T, D>(g.bc[2 * ax], g.bc_prim[2 * ax], &pp[4 * NV], n, NV, true, gl);
-      fill_ghost<T, D>(g.bc[2 * ax + 1], g.bc_prim[2 * ax + 1], &pp[4 * NV], n, NV, false, gr);
+      GhostCtx ctx;
+      ctx.x = g.x0 + (i1 + 0.5) * g.h[0];  // the pencil's x (y sweeps of the 2D cases)
+      ctx.dh = (g.x0 + ((n - 1) + 0.5) * g.h[0]) - (g.x0 + ((n - 2) + 0.5) * g.h[0]);
+      ctx.t = g.t;
+      fill_ghost<T, D>(g.bc[2 * ax], g.bc_prim[2 * ax], &pp[4 * NV], n, NV, true, gl, ctx);
+      fill_ghost<T, D>(g.bc[2 * ax + 1], g.bc_prim[2 * ax + 1], &pp[4 * NV], n, NV, false, gr, ctx);
       for (int k = 0; k < 4; k++)
         for (int v = 0; v < NV; v++) {
           pp[(3 - k) * NV + v] = gl[k * NV + v];
@@ -788,22 +820,42 @@ template <class T, int D> void rhs_eval(const Grid& g, const T* cons, T* prim, T
     for (int v = 0; v < NV; v++) rhs[i * NV + v] = T(0.0);
   }
   for (int ax = 0; ax < D; ax++) sweep<T, D>(g, ax, prim, rhs, nthreads);
+  // SourceTerm::calGravitySource (zone_FD/src/src/sourceTerm.cpp:3-20), written
+  // for an acceleration vector: rhs(rho u_d) -= rho g_d, rhs(rho E) -= rho (u . g)
+  // (the reference's g = (0, 1) gives its rhs(i,2) -= r, rhs(i,3) -= r v).
+  if (g.gravity[0] != 0.0 || g.gravity[1] != 0.0 || g.gravity[2] != 0.0) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long i = 0; i < N; i++) {
+      const T* p = &prim[i * NV];
+      T ug = p[1] * g.gravity[0];
+      for (int d = 1; d < D; d++) ug = ug + p[1 + d] * g.gravity[d];
+      for (int d = 0; d < D; d++) rhs[i * NV + 1 + d] -= p[0] * g.gravity[d];
+      rhs[i * NV + NV - 1] -= p[0] * ug;
+    }
+  }
 }
 
-// BlockSolver::RK3_SSP (zone_FD/src/src/blockSolver.cpp:43-87).
-template <class T, int D> void rk3_step(const Grid& g, T* cons, double dt, int nthreads) {
+// BlockSolver::RK3_SSP (zone_FD/src/src/blockSolver.cpp:43-87) with its time
+// bookkeeping (stages at t, t + dt, t + dt - dt/2) and BlockSolver::solve's
+// final t += dt (:120-124); g0.t is the time at the start of the step.
+template <class T, int D> double rk3_step(const Grid& g0, T* cons, double dt, int nthreads) {
   constexpr int NV = D + 2;
-  const long N = (long)g.n[0] * g.n[1] * g.n[2] * NV;
+  const long N = (long)g0.n[0] * g0.n[1] * g0.n[2] * NV;
   std::vector<T> u0(cons, cons + N), prim(N), rhs(N);
+  Grid g = g0;
   rhs_eval<T, D>(g, cons, prim.data(), rhs.data(), nthreads);
 #pragma omp parallel for num_threads(nthreads) schedule(static)
   for (long i = 0; i < N; i++) cons[i] = u0[i] - dt * rhs[i];
+  g.t += dt;
   rhs_eval<T, D>(g, cons, prim.data(), rhs.data(), nthreads);
 #pragma omp parallel for num_threads(nthreads) schedule(static)
   for (long i = 0; i < N; i++) cons[i] = 0.75 * u0[i] - 0.25 * dt * rhs[i] + 0.25 * cons[i];
+  g.t -= dt / 2;
   rhs_eval<T, D>(g, cons, prim.data(), rhs.data(), nthreads);
 #pragma omp parallel for num_threads(nthreads) schedule(static)
   for (long i = 0; i < N; i++) cons[i] = 1.0 / 3.0 * u0[i] - 2.0 / 3.0 * dt * rhs[i] + 2.0 / 3.0 * cons[i];
+  g.t -= dt / 2;
+  return g.t + dt;
 }
 
 // BlockSolver::getTimeIntervalExplicit (zone_FD/src/src/blockSolver.cpp:318-348):

@@ -33,6 +33,9 @@ class OracleGrid(ctypes.Structure):
         ("charwise", ctypes.c_int),
         ("bc", ctypes.c_int * 6),
         ("bc_prim", (ctypes.c_double * 5) * 6),
+        ("x0", ctypes.c_double),
+        ("t", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
     ]
 
 
@@ -81,8 +84,12 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def make_grid(dim, n, h, scheme, flux, charwise, bc, bc_prim=None):
+def make_grid(dim, n, h, scheme, flux, charwise, bc, bc_prim=None, x0=0.0, t=0.0, gravity=(0.0, 0.0, 0.0)):
     g = OracleGrid()
+    g.x0 = float(x0)
+    g.t = float(t)
+    for i in range(3):
+        g.gravity[i] = float(gravity[i]) if i < len(gravity) else 0.0
     g.dim = dim
     nn = list(n) + [1] * (3 - len(n))
     hh = list(h) if hasattr(h, "__len__") else [h] * 3

@@ -19,6 +19,9 @@ struct oracle_grid {
   int charwise;  // 1 characteristic-wise, 0 component-wise
   int bc[6];     // oracle::Bc per face (xlo, xhi, ylo, yhi, zlo, zhi)
   double bc_prim[6][5];
+  double x0;          // cell centres at x0 + (i + 0.5) h[0]
+  double t;           // time at the start (boundaries that depend on time)
+  double gravity[3];  // acceleration of the gravity source (0: none)
 };
 
 static Grid to_grid(const oracle_grid* c) {
@@ -32,6 +35,9 @@ static Grid to_grid(const oracle_grid* c) {
     g.bc[i] = c->bc[i];
     for (int v = 0; v < 5; v++) g.bc_prim[i][v] = c->bc_prim[i][v];
   }
+  g.x0 = c->x0;
+  g.t = c->t;
+  for (int i = 0; i < 3; i++) g.gravity[i] = c->gravity[i];
   return g;
 }
 
@@ -49,20 +55,20 @@ int oracle_rhs(const oracle_grid* c, const double* cons, double* prim, double* r
 }
 
 // nsteps SSP-RK3 steps with a fixed dt (cfl <= 0) or the CFL dt of each step.
+// t_out: the time reached, advanced as BlockSolver::RK3_SSP + solve do it
+// (from the grid's start time).
 int oracle_rk3(const oracle_grid* c, double* cons, double dt, double cfl, int nsteps, int nthreads, double* t_out) {
   Grid g = to_grid(c);
-  double t = 0;
   for (int s = 0; s < nsteps; s++) {
     double d = dt;
     switch (g.dim) {
-    case 1: if (cfl > 0) d = cfl_dt<double, 1>(g, cons, cfl, nthreads); rk3_step<double, 1>(g, cons, d, nthreads); break;
-    case 2: if (cfl > 0) d = cfl_dt<double, 2>(g, cons, cfl, nthreads); rk3_step<double, 2>(g, cons, d, nthreads); break;
-    case 3: if (cfl > 0) d = cfl_dt<double, 3>(g, cons, cfl, nthreads); rk3_step<double, 3>(g, cons, d, nthreads); break;
+    case 1: if (cfl > 0) d = cfl_dt<double, 1>(g, cons, cfl, nthreads); g.t = rk3_step<double, 1>(g, cons, d, nthreads); break;
+    case 2: if (cfl > 0) d = cfl_dt<double, 2>(g, cons, cfl, nthreads); g.t = rk3_step<double, 2>(g, cons, d, nthreads); break;
+    case 3: if (cfl > 0) d = cfl_dt<double, 3>(g, cons, cfl, nthreads); g.t = rk3_step<double, 3>(g, cons, d, nthreads); break;
     default: return -1;
     }
-    t += d;
   }
-  if (t_out) *t_out = t;
+  if (t_out) *t_out = g.t;
   return 0;
 }
 
@@ -213,9 +219,9 @@ int oracle_dual_rk3(const oracle_grid* c, double* cons, double* dcons, double dt
   dual_pack(N, cons, dcons, u);
   for (int s = 0; s < nsteps; s++) {
     switch (g.dim) {
-    case 1: rk3_step<Dual, 1>(g, u.data(), dt, nthreads); break;
-    case 2: rk3_step<Dual, 2>(g, u.data(), dt, nthreads); break;
-    case 3: rk3_step<Dual, 3>(g, u.data(), dt, nthreads); break;
+    case 1: g.t = rk3_step<Dual, 1>(g, u.data(), dt, nthreads); break;
+    case 2: g.t = rk3_step<Dual, 2>(g, u.data(), dt, nthreads); break;
+    case 3: g.t = rk3_step<Dual, 3>(g, u.data(), dt, nthreads); break;
     default: return -1;
     }
   }

@@ -15,9 +15,9 @@ SCHEME_OF = {
 }
 
 
-def oracle_grid(dim, n, h, inter, flux, recon, bc, bc_prim=None):
+def oracle_grid(dim, n, h, inter, flux, recon, bc, bc_prim=None, x0=0.0, t=0.0, gravity=(0.0, 0.0, 0.0)):
     return O.make_grid(dim, n, h, SCHEME_OF[inter], int(flux), 1 if recon == Recon.CHAR else 0,
-                       [int(b) for b in bc], bc_prim)
+                       [int(b) for b in bc], bc_prim, x0=x0, t=t, gravity=gravity)
 
 
 def random_state(dim, n, seed, jumps=True, amp=0.2):

@@ -27,9 +27,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_matches_header_size():
-    # dim + n[3] | h[3] | 4 ints | bc[6] | bc_prim[6][5] | device, precision
+    # dim + n[3] | h[3] | 4 ints | bc[6] | bc_prim[6][5] | device, precision | origin | gravity[3]
     c = solver.cfd_config()
-    assert ctypes.sizeof(c) == 4 + 12 + 24 + 16 + 24 + 240 + 4 + 4
+    assert ctypes.sizeof(c) == 4 + 12 + 24 + 16 + 24 + 240 + 4 + 4 + 8 + 24
 
 
 def test_version_string():

@@ -152,7 +152,7 @@ def test_torch_harness_equals_direct_call():
     J.backward()
     s.close()
     Jd, _, gU = _gpu_gradient(case, U0, dt, nsteps, 5)
-    assert abs(float(J) - Jd) <= 1e-14 * abs(Jd)
+    assert abs(float(J.detach()) - Jd) <= 1e-14 * abs(Jd)
     np.testing.assert_allclose(rho0.grad.numpy(), _grad_rho(case, gU), rtol=1e-12, atol=1e-14 * np.abs(gU).max())
 
 

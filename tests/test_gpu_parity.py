@@ -111,6 +111,74 @@ def test_rk3_parity(dim, inter, flux):
     s.close()
 
 
+def _dmr_state(n, h, x0):
+    nx, ny = n
+    x = x0 + (np.arange(nx) + 0.5) * h
+    y = x0 + (np.arange(ny) + 0.5) * h
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    post = [8.0, 8.25 * np.cos(np.pi / 6), -8.25 * np.sin(np.pi / 6), 116.5]
+    isp = Y >= np.sqrt(3) * (X - 1.0 / 6.0)
+    prim = np.stack([np.where(isp, post[v], [1.4, 0.0, 0.0, 1.0][v]) for v in range(4)], -1)
+    return cases.prim_to_cons(prim).reshape(-1, 4), post
+
+
+def _stepped_parity(s, grid, cons, t0, what):
+    """Three fixed-dt steps and two device-CFL steps from absolute time t0,
+    against the oracle from the same time; the device clock must read the
+    oracle's time, advanced as the reference's RK3 step advances it."""
+    s.set_state(cons)
+    s.set_time(t0)
+    grid.t = t0
+    dt = 0.5 * O.cfl_dt(grid, cons, 1.0)
+    assert s.cfl_dt(1.0) == 2 * dt and s.get_time() == t0  # a dt query leaves the clock alone
+    for _ in range(3):
+        s.rk3_step(dt)
+    ref, t1 = O.rk3(grid, cons, dt=dt, nsteps=3)
+    _check(s.get_state(), ref, what)
+    assert s.get_time() == t1, (s.get_time(), t1)
+    s.rk3_steps_cfl(0.5, 2)
+    grid.t = t1
+    ref2, t2 = O.rk3(grid, ref, cfl=0.5, nsteps=2)
+    _check(s.get_state(), ref2, what + " cfl")
+    assert s.get_time() == t2
+    s.close()
+
+
+@pytest.mark.parametrize("flux", [FluxMethod.HLLC, FluxMethod.ROE])
+def test_rk3_parity_double_mach(flux):
+    """Double Mach reflection boundaries: the post-shock Dirichlet inflow, the
+    wall (Dirichlet for x < 1/6, reflecting beyond; cells placed from the
+    zone origin) and the DoubleMachUp top, whose ghost states follow the
+    shock with the stage time."""
+    n, h, x0 = (90, 24), 1.0 / 24, -0.05
+    cons, post = _dmr_state(n, h, x0)
+    bc = [BC.DIRICHLET, BC.OUTFLOW, BC.DMR_WALL, BC.DMR_TOP]
+    bc_prim = np.zeros((6, 5))
+    bc_prim[0, :4] = bc_prim[2, :4] = bc_prim[3, :4] = post
+    s = Solver(2, n, h, InterMethod.TCNS5, flux, Recon.CHAR, bc + [BC.OUTFLOW] * 2, bc_prim, origin=x0)
+    grid = oracle_grid(2, n, h, InterMethod.TCNS5, flux, Recon.CHAR, bc, bc_prim, x0=x0)
+    _stepped_parity(s, grid, cons, 0.013, f"double Mach {flux.name}")
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_rk3_parity_gravity(dim):
+    """The gravity source term on the last sweep, for any acceleration vector."""
+    n = {1: (70,), 2: (36, 40), 3: (14, 12, 16)}[dim]
+    gvec = [0.3, 1.0, -0.6][:dim]
+    cons = random_state(dim, n, seed=17, jumps=True)
+    bc = [BC.PERIODIC] * 2 + [BC.DIRICHLET] * 2 + [BC.SYMMETRYZ, BC.OUTFLOW]
+    bc_prim = np.zeros((6, 5))
+    bc_prim[2, :dim + 2] = [1.1] + [0.0] * dim + [1.3]
+    bc_prim[3, :dim + 2] = [0.9] + [0.1] * dim + [0.8]
+    s = Solver(dim, n, 0.04, InterMethod.WCNSZ5, FluxMethod.HLLC, Recon.CHAR, bc, bc_prim, gravity=gvec)
+    grid = oracle_grid(dim, n, 0.04, InterMethod.WCNSZ5, FluxMethod.HLLC, Recon.CHAR, bc, bc_prim,
+                       gravity=gvec + [0.0] * (3 - dim))
+    _say(f"gravity dim={dim}")
+    s.set_state(cons)
+    _check(s.rhs(), O.rhs(grid, cons)[0], f"gravity rhs dim={dim}")
+    _stepped_parity(s, grid, cons, 0.0, f"gravity rk3 dim={dim}")
+
+
 def test_cfl_dt_parity():
     for dim in (1, 2, 3):
         n = SIZES[dim]
