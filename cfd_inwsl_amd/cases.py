@@ -250,3 +250,41 @@ def sod_exact(x, t, left=(1.0, 0.0, 1.0), right=(0.125, 0.0, 0.1), x0=0.5):
                     r = rr * (c / cr) ** (2 / (g - 1))
                     out[i] = (r, u, pr * (c / cr) ** (2 * g / (g - 1)))
     return out
+
+
+def reference_1d(ncase, x0, x1, npoints, gamma=1.4):
+    """Initial conservative state of the reference's 1D Euler presets
+    (Initializer::solInit, zone_FD/src/src/initializer.cpp:79-242), on the
+    cell centres x0 + (i + 1/2) h, h = (x1 - x0) / (npoints - 1), with the
+    reference's own expressions (libm sin, the same operation order), so the
+    state is the reference's bit for bit.  Cases: 0 Sod, 1 Shu-Osher, 2 Lax,
+    3 Sedov, 4 Woodward-Colella, 5 double rarefaction.  Returns (cons, h)."""
+    n = npoints - 1
+    h = (x1 - x0) / (npoints - 1)
+    G = gamma
+    xc = [x0 + (i + 0.5) * h for i in range(n)]
+    out = []
+    for i, x in enumerate(xc):
+        if ncase == 0:
+            out.append([1, 0, 1.0 / (G - 1) * 1] if x < 0 else [0.125, 0, 1.0 / (G - 1) * 0.1])
+        elif ncase == 1:
+            if x <= 0.5:
+                out.append([3.857143, 3.857143 * 2.629369,
+                            1.0 / (G - 1.0) * (10.0 + 1.0 / 3.0) + 3.857143 * 2.629369 * 2.629369 / 2])
+            else:
+                out.append([1.0 + 0.2 * math.sin(5.0 * x), 0.0, 1.0 / (G - 1.0) * 1.0])
+        elif ncase == 2:
+            r, u, p = (0.445, 0.698, 3.528) if x <= 0 else (0.5, 0.0, 0.571)
+            out.append([r, r * u, 1.0 / (G - 1) * p + r * u * u / 2])
+        elif ncase == 3:
+            E = 3200000.0 / (xc[i] - (x0 + (i - 1 + 0.5) * h)) if abs(x) < 1e-10 else 1e-12
+            out.append([1, 0, 1 * E])
+        elif ncase == 4:
+            p = 1000 if x < 0.1 else (100 if x >= 0.9 else 0.01)
+            out.append([1, 0, 1.0 / (G - 1) * p])
+        elif ncase == 5:
+            u = -2 if x < 0 else 2
+            out.append([1, u, 1.0 / (G - 1) * 0.4 + u * u / 2])
+        else:
+            raise ValueError(f"1D case {ncase} is not defined")
+    return np.array(out, dtype=np.float64), h

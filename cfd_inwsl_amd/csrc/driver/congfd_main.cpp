@@ -8,10 +8,12 @@
 // writes the solution at every output time, and writes timeInfo.txt.  The
 // compute path is the C ABI of libcongfd (include/congfd.h).
 //
-// Solution files: libcgns is not available to this build, so each output is a
-// Tecplot ASCII point file with the reference's CGNS file name stem
-// (Info::filename) and the same cell-centred variables (Density, XVelocity,
-// [YVelocity, [ZVelocity,]] Pressure; CongFD1.0/src/src/info.cpp:252-280).
+// Solution files: CGNS with the reference's file names (Info::filename) and
+// tree (cgnsio.cpp:3-101: vertex coordinates, cell-centred Density, XVelocity,
+// [YVelocity, [ZVelocity,]] Pressure), written by cgns_writer.cpp without
+// libcgns; --format tecplot writes the same variables as a Tecplot ASCII
+// point file instead.  --initial-only writes the initial solution and stops
+// before any device call (it runs without a GPU).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "cgns_writer.hpp"
 #include "info.hpp"
 
 using namespace cfdhost;
@@ -29,7 +32,8 @@ static void usage() {
                "usage: congfd [info.txt] [--flux hllc|roe] [--recon char|component] [--inter N]\n"
                "              [--case N] [--dim N] [--imax I J K] [--zone x0 x1 y0 y1 z0 z1]\n"
                "              [--endstep N] [--outputdt T] [--cfl C] [--true-periodic] [--device N]\n"
-               "              [--outdir DIR] [--no-output] [--fp32] [--gravity]\n");
+               "              [--outdir DIR] [--no-output] [--fp32] [--gravity]\n"
+               "              [--format cgns|tecplot] [--initial-only]\n");
 }
 
 #define CHECK(call)                                                                      \
@@ -40,6 +44,37 @@ static void usage() {
       return 1;                                                                          \
     }                                                                                    \
   } while (0)
+
+static std::string write_cgns_file(const std::string& path, const Info& info, const cfd_config& cfg,
+                                   const std::vector<double>& cons) {
+  const int dim = info.dim, nv = dim + 2;
+  CgnsSolution s;
+  s.dim = dim;
+  s.h = info.geth();
+  for (int d = 0; d < dim; d++) {
+    s.vertices[d] = cfg.n[d] + 1;
+    s.lo[d] = info.calZone[2 * d];
+  }
+  const long N = (long)cfg.n[0] * cfg.n[1] * cfg.n[2];
+  s.fields.assign(nv, std::vector<double>(N));
+  for (long c = 0; c < N; c++) {
+    // Equation::consToPrimEuler1D/2D (zone_FD/src/src/equation.cpp:36-92), the
+    // primitive state the reference writes; 3D extends the 2D expression
+    const double* u = &cons[c * nv];
+    const double r = u[0];
+    double vel[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) vel[d] = u[1 + d] / r;
+    const double E = u[nv - 1] / r;
+    double e;
+    if (dim == 1) e = -vel[0] * vel[0] / 2 + E;
+    else if (dim == 2) e = -((vel[0] * vel[0] + vel[1] * vel[1]) / 2) + E;
+    else e = -((vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]) / 2) + E;
+    s.fields[0][c] = r;
+    for (int d = 0; d < dim; d++) s.fields[1 + d][c] = vel[d];
+    s.fields[nv - 1][c] = r * ((1.4 - 1) * e);
+  }
+  return write_cgns(path, s);
+}
 
 static void write_tecplot(const std::string& path, const Info& info, const cfd_config& cfg,
                           const std::vector<double>& cons) {
@@ -77,7 +112,7 @@ static void write_tecplot(const std::string& path, const Info& info, const cfd_c
 int main(int argc, char** argv) {
   Info info;
   std::string infoPath = "info.txt", outdir = ".";
-  bool output = true;
+  bool output = true, initialOnly = false, cgns = true;
   std::vector<std::string> args(argv + 1, argv + argc);
   // positional info file first, then overrides
   size_t a = 0;
@@ -106,6 +141,8 @@ int main(int argc, char** argv) {
     else if (s == "--no-output") output = false;
     else if (s == "--fp32") info.precision = CFD_PRECISION_FP32;
     else if (s == "--gravity") info.gravity = true;
+    else if (s == "--format") { std::string v = next(); cgns = v != "tecplot"; }
+    else if (s == "--initial-only") initialOnly = true;
     else { usage(); return 2; }
   }
 
@@ -124,6 +161,21 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "congfd: %s\n", why.c_str());
     return 2;
   }
+  auto base = [&]() { return outdir + "/" + info.filename(); };
+  auto write_solution = [&]() -> int {
+    if (cgns) {
+      std::string err = write_cgns_file(base() + ".cgns", info, cfg, cons);
+      if (!err.empty()) {
+        std::fprintf(stderr, "congfd: %s\n", err.c_str());
+        return 1;
+      }
+    } else {
+      write_tecplot(base() + ".dat", info, cfg, cons);
+    }
+    return 0;
+  };
+  if (initialOnly) return write_solution();
+
   cfd_handle* h = nullptr;
   CHECK(cfd_create(&cfg, &h));
   CHECK(cfd_set_state(h, cons.data()));
@@ -132,13 +184,11 @@ int main(int argc, char** argv) {
   std::printf("initialized %s, %s, %s, %d cells\n", info.caseName().c_str(), info.schemeName().c_str(),
               info.gridInfo().c_str(), cfg.n[0] * cfg.n[1] * cfg.n[2]);
 
-  auto base = [&]() { return outdir + "/" + info.filename(); };
   std::string logName = outdir + "/" + info.caseName() + " - " + info.schemeName() + " - " + info.gridInfo() + " - rholinf.txt";
   auto dump = [&]() -> int {
     if (!output) return 0;
     CHECK(cfd_get_state(h, cons.data()));
-    write_tecplot(base() + ".dat", info, cfg, cons);
-    return 0;
+    return write_solution();
   };
 
   // BlockSolver::stepsLoopCFL (blockSolver.cpp:170-233)

@@ -1,0 +1,121 @@
+"""Extract small fixtures from the solution files the reference ships.
+
+The reference keeps CGNS outputs of its own runs under zone_FD/cases and
+CongFD1.0/cases, with their per-step logs.  This script reads them with the
+package's CGNS reader (cfd_inwsl_amd/cgns.py) and writes:
+
+* reference_outputs.npz: cell-centred Density / XVelocity / Pressure of the
+  1D end states listed in CASES, the t=0 and t=0.2 states and the logged dt
+  sequence of the "Sod - congTCNS5CT5 - 101" run, and the 512x128 double-Mach
+  density at t=0.2 (float32: it is compared loosely);
+* reference_outputs.json: the run each array comes from (the reference file,
+  the driver's case number and info.txt values), and SHA-256 digests of the
+  fields and coordinates of initial-state files, which the driver's
+  --initial-only output must reproduce bit for bit.
+
+Run in a checkout that has the reference tree (default /root/reference):
+    python tests/golden/make_reference_outputs.py [REFERENCE_ROOT]
+"""
+import hashlib
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from cfd_inwsl_amd import cgns  # noqa: E402
+
+# 1D end states (TENO5 = Teno5_Z, info.txt inter 11); zone and iMax from the
+# files' own coordinates, outputDt the value the reference's run used (the
+# dt clamp at output times changes the step sequence).
+CASES = {
+    "sod": dict(file="zone_FD/cases/sod tube    t=0.2000/TENO.cgns", case=0, t_end=0.2, output_dt=0.01),
+    "shu_osher": dict(file="zone_FD/cases/Shu-Osher  t=1.8000/200/TENO.cgns", case=1, t_end=1.8, output_dt=1.8),
+    "lax": dict(file="zone_FD/cases/lax sod tube    t=0.1400/TENO.cgns", case=2, t_end=0.14, output_dt=0.01),
+    "double_rarefaction": dict(file="zone_FD/cases/ShuangXiShuBo   t=1.0000/TENO.cgns", case=5, t_end=1.0,
+                               output_dt=0.01),
+}
+INITIAL = {
+    "sod_101": dict(file="zone_FD/cases/Sod - whfzycTCNSNmyASF002_1 - 101 - t=0.0000.cgns", dim=1, case=0),
+    "sod_10000": dict(file="zone_FD/cases/EULERWCNS5t=0.0000.cgns", dim=1, case=0),
+    "lax_200": dict(file="CongFD1.0/cases/EULERWCNS5t=0.0000.cgns", dim=1, case=2),
+    "woodward_colella_40": dict(file="zone_FD/cases/Woodward_Colella - whfCOMPARE - 41 - t=0.0000.cgns", dim=1,
+                                case=4),
+    "woodward_colella_400": dict(file="zone_FD/cases/Woodward_Colella - whfCOMPARE - 401 - t=0.0000.cgns", dim=1,
+                                 case=4),
+}
+SOD_LOG = "zone_FD/cases/Sod - congTCNS5CT5 - 101"
+DMR = "zone_FD/cases/Double Mach    t=0.2000/512, 128 - TENO   t=0.2000.cgns"
+FIELDS = ("Density", "XVelocity", "YVelocity", "ZVelocity", "Pressure")
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def zone_of(sol):
+    x = sol["CoordinateX"]
+    xs = x if x.ndim == 1 else x[:, 0]
+    imax = [len(xs), 2, 2]
+    zone = [float(xs[0]), float(xs[-1]), 0.0, 1.0, 0.0, 0.0]
+    if "CoordinateY" in sol:
+        y = sol["CoordinateY"][0, :]
+        imax[1] = len(y)
+        zone[2:4] = [float(y[0]), float(y[-1])]
+    return zone, imax
+
+
+def tree_listing(node, path=""):
+    out = []
+    for c in node.children:
+        p = path + "/" + c.name
+        out.append([p, c.label, c.type, list(c.data.shape) if c.data is not None else None])
+        out += tree_listing(c, p)
+    return out
+
+
+def main(root):
+    arrays, meta = {}, {"cases": {}, "initial": {}}
+    for key, c in CASES.items():
+        sol = cgns.read_solution(os.path.join(root, c["file"]))
+        zone, imax = zone_of(sol)
+        for f in ("Density", "XVelocity", "Pressure"):
+            arrays[f"{key}/{f}"] = sol[f]
+        meta["cases"][key] = dict(c, dim=1, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5)
+    for key, c in INITIAL.items():
+        sol = cgns.read_solution(os.path.join(root, c["file"]))
+        zone, imax = zone_of(sol)
+        meta["initial"][key] = dict(c, zone=zone, imax=imax,
+                                    sha256={k: digest(v) for k, v in sol.items()
+                                            if k in FIELDS or k.startswith("Coordinate")})
+    # the logged Sod run: per-step "time = .. dt=..  rhoLinf = .." (two runs appended)
+    with open(os.path.join(root, SOD_LOG + ".txt")) as f:
+        log = [[float(v) for v in re.findall(r"=\s*([-+\d.eE]+)", line)] for line in f if line.strip()]
+    log = np.array(log)
+    restart = np.nonzero(np.diff(log[:, 0]) < 0)[0]
+    first = log[: restart[0] + 1] if len(restart) else log
+    arrays["sod_ct5/log_time_dt_rholinf"] = first
+    for t in ("0.0000", "0.2000"):
+        sol = cgns.read_solution(os.path.join(root, f"{SOD_LOG} - t={t}.cgns"))
+        for f in ("Density", "XVelocity", "Pressure"):
+            arrays[f"sod_ct5/t{t}/{f}"] = sol[f]
+    zone, imax = zone_of(sol)
+    meta["sod_ct5"] = dict(file=SOD_LOG, case=0, dim=1, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5,
+                           output_dt=0.1, t_end=0.2)
+    sol = cgns.read_solution(os.path.join(root, DMR))
+    zone, imax = zone_of(sol)
+    arrays["dmr/Density_f32"] = sol["Density"].astype(np.float32)
+    meta["dmr"] = dict(file=DMR, case=4, dim=2, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5, t_end=0.2)
+    tree = cgns.read_tree(os.path.join(root, CASES["sod"]["file"]))
+    meta["tree_1d"] = tree_listing(tree)
+    meta["root_attrs"] = {k: v for k, v in tree.attrs.items() if isinstance(v, str)}
+    np.savez_compressed(os.path.join(HERE, "reference_outputs.npz"), **arrays)
+    with open(os.path.join(HERE, "reference_outputs.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from cfd_inwsl_amd import BC, cases
+from cfd_inwsl_amd import BC, cases, cgns
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -37,12 +37,12 @@ def test_driver_usage_error(tmp_path):
     assert r.returncode == 2 and "usage" in r.stderr
 
 
-def _oracle_steps_cfl(grid, cons, cfl, output_dt, end_step):
+def _oracle_steps_cfl(grid, cons, cfl, output_dt, end_step, nthreads=None):
     """BlockSolver::stepsLoopCFL (zone_FD/src/src/blockSolver.cpp:170-233) over the
     oracle, the time advanced as RK3_SSP + solve do it."""
     t, out_t, step = grid.t, 0.0, 0
     while step < end_step:
-        dt = O.cfl_dt(grid, cons, cfl)
+        dt = O.cfl_dt(grid, cons, cfl, nthreads)
         if out_t + dt > output_dt:
             dt = output_dt - out_t
             out_t = 0.0
@@ -50,8 +50,15 @@ def _oracle_steps_cfl(grid, cons, cfl, output_dt, end_step):
         else:
             out_t += dt
         grid.t = t
-        cons, t = O.rk3(grid, cons, dt=dt, nsteps=1)
+        cons, t = O.rk3(grid, cons, dt=dt, nsteps=1, nthreads=nthreads)
     return cons, t
+
+
+def _last_output(d):
+    """The solution file of the latest output time ("... - t=<t>.cgns")."""
+    files = list(d.glob("*t=*.cgns"))
+    assert files, list(d.iterdir())
+    return max(files, key=lambda f: float(f.name.rsplit("t=", 1)[1][:-5]))
 
 
 def _read_tecplot(path, nvar):
@@ -68,7 +75,7 @@ def test_driver_sod_matches_oracle_and_exact(tmp_path):
     """Config S: Sod, 400 cells on [0,1] (reference frame [-0.5,0.5] split at 0),
     WCNS5-JS component-wise + HLLC + SSP-RK3, CFL 0.5, to t = 0.2."""
     _info_txt(tmp_path / "info.txt", 2, 1, 0.2, 0.5, 0, [-0.5, 0.5, 0, 0, 0, 0], [401, 2, 2], 1)
-    r = _run(["--recon", "component", "--flux", "hllc"], tmp_path)
+    r = _run(["--recon", "component", "--flux", "hllc", "--format", "tecplot"], tmp_path)
     print(r.stdout[-400:], r.stderr[-400:], flush=True)
     assert r.returncode == 0
     out = tmp_path / "Sod - WCNS5 - 401 - t=0.2000.dat"
@@ -132,8 +139,8 @@ def test_driver_rti_and_double_mach_match_oracle(tmp_path, ncase, extra):
     r = _run(["--flux", "hllc"] + extra, tmp_path)
     print(r.stdout[-300:], r.stderr[-300:], flush=True)
     assert r.returncode == 0
-    outs = sorted(tmp_path.glob("*t=*.dat"))
-    got = _read_tecplot(outs[-1], 5)  # x, y, rho, u, v, p
+    sol = cgns.read_solution(_last_output(tmp_path))
+    got_rho = sol["Density"].T.reshape(-1)  # (i, j) -> x fastest
     cons, h = _case_state(ncase, zone, imax)
     nx, ny = imax[0] - 1, imax[1] - 1
     post = [8.0, 8.25 * math.cos(math.pi / 6), -8.25 * math.sin(math.pi / 6), 116.5, 0.0]
@@ -149,4 +156,57 @@ def test_driver_rti_and_double_mach_match_oracle(tmp_path, ncase, extra):
                        gravity=(0.0, 1.0, 0.0) if "--gravity" in extra else (0.0, 0.0, 0.0))
     ref, t = _oracle_steps_cfl(grid, cons, 0.5, out_dt, 1)
     assert abs(t - out_dt) < 1e-12
-    assert np.array_equal(got[:, 2], ref[:, 0]), np.abs(got[:, 2] - ref[:, 0]).max()
+    assert np.array_equal(got_rho, ref[:, 0]), np.abs(got_rho - ref[:, 0]).max()
+
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _reference_fixtures():
+    import json
+    return (json.load(open(os.path.join(GOLD, "reference_outputs.json"))),
+            np.load(os.path.join(GOLD, "reference_outputs.npz")))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("key", ["sod", "shu_osher", "lax", "double_rarefaction"])
+def test_driver_reproduces_reference_solution_files(tmp_path, key):
+    """The reference's own TENO5 solution files (zone_FD/cases, Teno5_Z
+    char-wise + HLLC, CFL 0.5): the GPU through the driver's stepsLoopCFL
+    writes the same Density, XVelocity and Pressure, bit for bit."""
+    meta, arr = _reference_fixtures()
+    m = meta["cases"][key]
+    nout = int(round(m["t_end"] / m["output_dt"]))
+    _info_txt(tmp_path / "info.txt", 11, nout, m["output_dt"], m["cfl"], m["case"], m["zone"], m["imax"], 1)
+    r = _run(["--flux", "hllc"], tmp_path)
+    print(key, r.stdout[-200:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    sol = cgns.read_solution(_last_output(tmp_path))
+    for f in ("Density", "XVelocity", "Pressure"):
+        want = arr[f"{key}/{f}"]
+        assert np.array_equal(sol[f], want), (f, np.abs(sol[f] - want).max())
+
+
+@pytest.mark.gpu
+def test_driver_double_mach_near_reference_solution_file(tmp_path):
+    """Double Mach reflection, 512 x 128 cells to t = 0.2, against the
+    reference's "512, 128 - TENO t=0.2000" file.  That run's settings are not
+    recorded and no setting tried reproduces it exactly (the 1D TENO files
+    are reproduced bit for bit), so the comparison is loose: the same waves at
+    the same places, density within a mean 0.03 (measured 0.018; through
+    the oracle, a reflecting wall over x < 1/6 gives 0.34 and a DoubleMachUp
+    top frozen at t = 0 does not reach t = 0.2)."""
+    meta, arr = _reference_fixtures()
+    m = meta["dmr"]
+    _info_txt(tmp_path / "info.txt", 11, 1, m["t_end"], m["cfl"], m["case"], m["zone"], m["imax"], 2)
+    r = _run(["--flux", "hllc"], tmp_path)
+    print(r.stdout[-200:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    rho = cgns.read_solution(_last_output(tmp_path))["Density"]
+    ref = arr["dmr/Density_f32"].astype(np.float64)
+    d = np.abs(rho - ref)
+    print(f"double Mach vs reference file: mean |drho| {d.mean():.4f}, max {d.max():.3f}, "
+          f"equal to float32 {np.mean(rho.astype(np.float32) == arr['dmr/Density_f32']):.3f}", flush=True)
+    assert d.mean() < 0.03
+    # the undisturbed pre- and post-shock regions are the reference's exactly
+    assert np.mean(rho.astype(np.float32) == arr["dmr/Density_f32"]) > 0.25

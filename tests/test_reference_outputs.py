@@ -1,0 +1,186 @@
+"""The reference's own solution files as pins (CPU).
+
+tests/golden/reference_outputs.{npz,json} hold arrays and digests taken from
+the CGNS files and logs the reference ships (made by
+tests/golden/make_reference_outputs.py with the package's CGNS reader):
+
+* the oracle, driven through the reference's stepsLoopCFL, reproduces the
+  reference's TENO5 (Teno5_Z, char-wise, HLLC, CFL 0.5) end states of Sod,
+  Shu-Osher, Lax and the double rarefaction bit for bit, and the logged
+  "Sod - congTCNS5CT5" run's dt sequence to the log's 10 printed decimals;
+* the driver's --initial-only files equal the reference's t=0 files field for
+  field, bit for bit, and carry the reference's CGNS tree;
+* the CGNS writer's files read back through the reader (every HDF5 checksum
+  verified), and lookup3 gives its published test values.
+"""
+import hashlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cfd_inwsl_amd import cases, cgns
+from oracle import oracle as O
+from tests.test_driver import EXE, _info_txt, _oracle_steps_cfl
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+META = json.load(open(os.path.join(GOLD, "reference_outputs.json")))
+ARR = np.load(os.path.join(GOLD, "reference_outputs.npz"))
+
+
+def _prim(cons):
+    """Equation::consToPrimEuler1D (zone_FD/src/src/equation.cpp:36-58)."""
+    r = cons[:, 0]
+    u = cons[:, 1] / r
+    e = -u * u / 2 + cons[:, 2] / r
+    return r, u, r * ((1.4 - 1) * e)
+
+
+def _run_1d(meta, output_dt):
+    zone, imax = meta["zone"], meta["imax"]
+    cons, h = cases.reference_1d(meta["case"], zone[0], zone[1], imax[0])
+    bc = [2, 2] if meta["case"] == 4 else [7, 7]
+    g = O.make_grid(1, (imax[0] - 1,), h, O.S_TENO5, O.F_HLLC, 1, bc)
+    nout = int(round(meta["t_end"] / output_dt))
+    return _oracle_steps_cfl(g, cons, meta["cfl"], output_dt, nout, nthreads=1)
+
+
+@pytest.mark.parametrize("key", list(META["cases"]))
+def test_oracle_reproduces_reference_end_states(key):
+    m = META["cases"][key]
+    cons, t = _run_1d(m, m["output_dt"])
+    assert abs(t - m["t_end"]) < 1e-12
+    r, u, p = _prim(cons)
+    assert np.array_equal(r, ARR[f"{key}/Density"]), np.abs(r - ARR[f"{key}/Density"]).max()
+    assert np.array_equal(u, ARR[f"{key}/XVelocity"]), np.abs(u - ARR[f"{key}/XVelocity"]).max()
+    assert np.array_equal(p, ARR[f"{key}/Pressure"]), np.abs(p - ARR[f"{key}/Pressure"]).max()
+
+
+def test_oracle_output_dt_matters():
+    """The pin above is not trivially met: the same run with the output
+    interval of another case (a different dt clamp sequence) differs."""
+    m = META["cases"]["sod"]
+    r = _prim(_run_1d(m, 0.1)[0])[0]
+    assert np.abs(r - ARR["sod/Density"]).max() > 1e-4
+
+
+def test_oracle_follows_the_logged_sod_run():
+    """'Sod - congTCNS5CT5 - 101' (congTcns5_ZCT5, TENO5 with CT = 1e-5, the
+    Teno5_Z formulas): the per-step dt of the reference's log to its 10
+    printed decimals, and the t = 0.2 file to ~1e-12.  That file is not
+    bit-equal (its build's rounding differs somewhere in the last bits);
+    the TENO runs above are."""
+    m = META["sod_ct5"]
+    log = ARR["sod_ct5/log_time_dt_rholinf"]
+    cons, h = cases.reference_1d(0, m["zone"][0], m["zone"][1], m["imax"][0])
+    for a, f in zip(_prim(cons), ("Density", "XVelocity", "Pressure")):
+        assert np.array_equal(a, ARR[f"sod_ct5/t0.0000/{f}"]), f
+    g = O.make_grid(1, (100,), h, O.S_TENO5, O.F_HLLC, 1, [7, 7])
+    t, out_t, dts = 0.0, 0.0, []
+    while len(dts) < len(log):
+        dt = O.cfl_dt(g, cons, 0.5, nthreads=1)
+        if out_t + dt > m["output_dt"]:
+            dt = m["output_dt"] - out_t
+            out_t = 0.0
+        else:
+            out_t += dt
+        g.t = t
+        cons, t = O.rk3(g, cons, dt=dt, nsteps=1, nthreads=1)
+        dts.append(dt)
+    assert abs(t - 0.2) < 1e-12
+    assert np.abs(np.array(dts) - log[:, 1]).max() <= 5.1e-11
+    assert np.all(np.abs(np.round(np.cumsum(dts), 4) - log[:, 0]) <= 1e-12)
+    r, u, p = _prim(cons)
+    assert np.abs(r - ARR["sod_ct5/t0.2000/Density"]).max() < 5e-12
+    assert np.abs(p - ARR["sod_ct5/t0.2000/Pressure"]).max() < 5e-12
+
+
+def _initial_only(tmp_path, dim, ncase, zone, imax, extra=()):
+    _info_txt(tmp_path / "info.txt", 11, 1, 0.1, 0.5, ncase, zone, imax, dim)
+    r = subprocess.run([EXE, "--initial-only", *extra], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    files = sorted(tmp_path.glob("*t=0.0000.cgns"))
+    assert len(files) == 1, files
+    return files[0]
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("key", list(META["initial"]))
+def test_driver_initial_state_equals_reference_file(tmp_path, key):
+    m = META["initial"][key]
+    path = _initial_only(tmp_path, m["dim"], m["case"], m["zone"], m["imax"])
+    sol = cgns.read_solution(path)
+    for name, want in m["sha256"].items():
+        assert _digest(sol[name]) == want, name
+
+
+def test_driver_cgns_tree_is_the_reference_tree(tmp_path):
+    m = META["cases"]["sod"]
+    path = _initial_only(tmp_path, 1, 0, m["zone"], m["imax"])
+    tree = cgns.read_tree(path)
+
+    def listing(node, p=""):
+        out = []
+        for c in node.children:
+            q = p + "/" + c.name
+            out.append([q, c.label, c.type, list(c.data.shape) if c.data is not None else None])
+            out += listing(c, q)
+        return out
+
+    got = listing(tree)
+    want = META["tree_1d"]
+    # same nodes, labels, data types and shapes; the library-version value differs
+    assert [g[:3] for g in got] == [w[:3] for w in want]
+    assert [g[3] for g in got] == [w[3] for w in want]
+    assert {k: v for k, v in tree.attrs.items() if isinstance(v, str)} == META["root_attrs"]
+    assert tree["Base/Zone  1/flowSolution/GridLocation"].text() == "CellCenter"
+    assert tree["Base/Zone  1/ZoneType"].text() == "Structured"
+    assert tree.hidden[" format"].tobytes().rstrip(b"\0") == b"IEEE_LITTLE_32"
+
+
+def test_driver_cgns_2d_3d_round_trip(tmp_path):
+    """2D (RTI) and 3D files: coordinates lo + i h per direction, cell fields
+    in (i, j[, k]) order, equal to the initial state the case defines."""
+    p2 = _initial_only(tmp_path, 2, 3, [0, 0.25, 0, 1, 0, 0], [17, 65, 2])
+    s = cgns.read_solution(p2)
+    assert s["Density"].shape == (16, 64) and s["CoordinateX"].shape == (17, 65)
+    h = 0.25 / 16
+    assert np.array_equal(s["CoordinateX"][:, 0], 0 + np.arange(17) * h)
+    assert np.array_equal(s["CoordinateY"][0, :], 0 + np.arange(65) * h)
+    y = (np.arange(64) + 0.5) * h
+    assert np.array_equal(s["Density"][3], np.where(y <= 0.5, 2.0, 1.0))
+    assert np.all(s["XVelocity"] == 0)
+    d3 = tmp_path / "d3"
+    d3.mkdir()
+    p3 = _initial_only(d3, 3, 0, [0, 6.25, 1, 7.25, 2, 8.25], [9, 9, 9])
+    s3 = cgns.read_solution(p3)
+    assert s3["Density"].shape == (8, 8, 8) and s3["CoordinateZ"].shape == (9, 9, 9)
+    assert s3["cell_dim"] == 3 and list(s3["zone_size"][:, 0]) == [9, 9, 9]
+    assert np.array_equal(s3["CoordinateZ"][0, 0, :], 2 + np.arange(9) * 0.78125)
+    assert np.array_equal(s3["CoordinateY"][0, :, 0], 1 + np.arange(9) * 0.78125)
+    assert np.all(np.abs(s3["Density"] - 1) < 0.2) and s3["Density"].std() > 1e-3
+
+
+def test_lookup3_published_values():
+    # lookup3.c's own self-test values (hashlittle)
+    assert cgns.lookup3(b"") == 0xDEADBEEF
+    assert cgns.lookup3(b"", 0xDEADBEEF) == 0xBD5B7DDE
+    assert cgns.lookup3(b"Four score and seven years ago") == 0x17770551
+    assert cgns.lookup3(b"Four score and seven years ago", 1) == 0xCD628161
+
+
+def test_reader_rejects_a_corrupted_file(tmp_path):
+    m = META["cases"]["sod"]
+    path = _initial_only(tmp_path, 1, 0, m["zone"], m["imax"])
+    buf = bytearray(path.read_bytes())
+    i = buf.index(b"flowSolution")
+    buf[i] ^= 0x20
+    bad = tmp_path / "bad.cgns"
+    bad.write_bytes(bytes(buf))
+    with pytest.raises(cgns.FormatError, match="checksum"):
+        cgns.read_tree(bad)
