@@ -57,7 +57,7 @@ def measure(sol, nv, dim, ncell, steps, warmup, cfl):
     # dominant kernel: the direction sweep; algorithmic bytes per cell and
     # direction = 24*nv (direction-split streams, FP64): per-launch bytes
     # = 24*nv*ncell, averaged over the dim sweep launches of a stage
-    ms = sol.time_sweeps(reps=5)
+    ms = sol.time_sweeps(reps=20)
     mean_ms = float(np.mean(ms))
     bytes_per_launch = 24 * nv * ncell
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9

@@ -509,28 +509,36 @@ int cfd_time_sweeps(cfd_handle* h, int reps, double* ms_out) {
   if (!h || !ms_out || reps < 1) return cfd_fail(CFD_E_ARG, "bad argument");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal, 0.0);
   HIPCHK(hipGetLastError());
-  std::vector<hipEvent_t> ev(2 * h->D);
+  const int D = h->D;
+  auto sweep = [&](int ax) {
+    int flags = 0;
+    if (ax > 0) flags |= SW_READ_R;
+    if (ax < D - 1) flags |= SW_WRITE_R;
+    else flags |= SW_RK;
+    return launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, -1, -1, nullptr);
+  };
+  // one untimed pass, then every timed launch queued back to back (no host
+  // sync in between), so that each event pair brackets its kernel only and
+  // not the launch latency of an idle stream
+  for (int ax = 0; ax < D; ax++) HIPCHK(sweep(ax));
+  std::vector<hipEvent_t> ev(2 * D * reps);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  std::vector<double> acc(h->D, 0.0);
-  for (int r = 0; r < reps; r++) {
-    for (int ax = 0; ax < h->D; ax++) {
-      int flags = 0;
-      if (ax > 0) flags |= SW_READ_R;
-      if (ax < h->D - 1) flags |= SW_WRITE_R;
-      else flags |= SW_RK;
-      HIPCHK(hipEventRecord(ev[2 * ax], h->stream));
-      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, -1, -1, nullptr));
-      HIPCHK(hipEventRecord(ev[2 * ax + 1], h->stream));
+  for (int r = 0; r < reps; r++)
+    for (int ax = 0; ax < D; ax++) {
+      HIPCHK(hipEventRecord(ev[2 * (r * D + ax)], h->stream));
+      HIPCHK(sweep(ax));
+      HIPCHK(hipEventRecord(ev[2 * (r * D + ax) + 1], h->stream));
     }
-    HIPCHK(hipStreamSynchronize(h->stream));
-    for (int ax = 0; ax < h->D; ax++) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> acc(D, 0.0);
+  for (int r = 0; r < reps; r++)
+    for (int ax = 0; ax < D; ax++) {
       float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, ev[2 * ax], ev[2 * ax + 1]));
+      HIPCHK(hipEventElapsedTime(&ms, ev[2 * (r * D + ax)], ev[2 * (r * D + ax) + 1]));
       acc[ax] += ms;
     }
-  }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  for (int ax = 0; ax < h->D; ax++) ms_out[ax] = acc[ax] / reps;
+  for (int ax = 0; ax < D; ax++) ms_out[ax] = acc[ax] / reps;
   return CFD_OK;
 }
 

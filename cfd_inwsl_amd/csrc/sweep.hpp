@@ -81,9 +81,17 @@ __host__ __device__ constexpr int tile_cp(int t) { return t == 0 ? 8 : (t == 1 ?
 #ifndef CFD_SWEEP_WAVES_PER_EU
 #define CFD_SWEEP_WAVES_PER_EU 1
 #endif
+// Prefetch (template flag PF of sweep_block): the cell phase's global
+// operands (R, U^n, U^(k)) are loaded when the chunk starts and held in
+// registers across the interface phase.  At two waves per SIMD (the FP64
+// kernels) loads issued in the cell phase itself stall it (3D 256^3 y/z
+// sweeps 1.47/1.62 -> 1.27/1.31 ms); where the extra registers would cost
+// occupancy (FP32 kernels: 4 -> 3 waves; the persistent batch kernel: 2 -> 1)
+// the operands are loaded in the cell phase as before.
+template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T) == sizeof(double); };
 
 // One workgroup's share of a sweep: pencil group bx, segment by.
-template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T, bool PF>
 __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, const int by) {
   constexpr int NV = D + 2;
   const T* __restrict__ Ut = static_cast<const T*>(a.U);
@@ -124,9 +132,11 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
   const int c1 = min(c0 + a.seg_len, n);
 
   // ---- load one cell (position g along AX, may be a ghost) into the ring ----
-  auto load_cell = [&](int g) {
-    T pr[NV];
-    int flip = -1;
+  // fetch_cell: the cell's raw values (the conservative state read from HBM,
+  // or the primitive state of a synthesised ghost: raw_prim); commit_cell:
+  // primitive conversion, symmetry flip and the ring write.
+  auto fetch_cell = [&](int g, T* pr, bool& raw_prim, int& flip) {
+    flip = -1;
     bool dir = false, exact = false;
     int src = g;
     if (g < 0 || g >= n) {
@@ -169,11 +179,18 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       if (dir && !exact)
         for (int v = 0; v < NV; v++) pr[v] = T(lo ? a.prim_lo[v] : a.prim_hi[v]);
     }
+    raw_prim = dir;
     if (!dir) {
-      T cs[NV];
       const long idx = base + (long)src * sa;
-      for (int v = 0; v < NV; v++) cs[v] = Ut[v * a.N + idx];
-      cons_to_prim<D>(cs, pr);
+      for (int v = 0; v < NV; v++) pr[v] = Ut[v * a.N + idx];
+    }
+  };
+  auto commit_cell = [&](int g, const T* raw, bool raw_prim, int flip) {
+    T pr[NV];
+    if (raw_prim) {
+      for (int v = 0; v < NV; v++) pr[v] = raw[v];
+    } else {
+      cons_to_prim<D>(raw, pr);
       for (int v = 1; v < NV - 1; v++)
         if (v == flip) pr[v] = -pr[v];
     }
@@ -184,6 +201,13 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       prim_to_cons<D>(pr, cc);
       for (int v = 0; v < NV; v++) cell[cidx(NV + v, slot, lp)] = cc[v];
     }
+  };
+  auto load_cell = [&](int g) {
+    T raw[NV];
+    bool raw_prim;
+    int flip;
+    fetch_cell(g, raw, raw_prim, flip);
+    commit_cell(g, raw, raw_prim, flip);
   };
 
   // preload cells c0-4 .. c0
@@ -196,6 +220,20 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
     {
       const int g = m0 + 1 + la;
       if (pvalid && g <= c1 + 3) load_cell(g);
+    }
+    // operands of this chunk's cell phase (cell m0-3+la), in flight during phase 2
+    T pf_r[NV];
+    double pf_u0[NV], pf_uc[NV];
+    if (PF) {
+      const int i = m0 - 3 + la;
+      if (pvalid && i >= c0 && i < c1) {
+        const long idx = base + (long)i * sa;
+        for (int v = 0; v < NV; v++) {
+          pf_r[v] = (a.flags & SW_READ_R) ? Rt[v * a.N + idx] : T(0.0);
+          pf_u0[v] = (a.flags & SW_RK) ? a.U0[v * a.N + idx] : 0.0;
+          pf_uc[v] = ((a.flags & SW_RK) && a.stage != 1) ? (double)Ut[v * a.N + idx] : 0.0;
+        }
+      }
     }
     __syncthreads();
     // 2. interface m = m0 + la: between cells m-1 and m... (cells m-4 .. m+1)
@@ -340,19 +378,19 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           const T H0 = flux[fidx(v, f0, lp)], H1 = flux[fidx(v, f1, lp)];
           const T H2 = flux[fidx(v, f2, lp)], H3 = flux[fidx(v, f3, lp)];
           const T term = fdv(T(w1) * (fp[v] - fm[v]) + T(w0) * (H2 - H1) + T(w2) * (H3 - H0), hh);
-          const T rold = (a.flags & SW_READ_R) ? Rt[v * a.N + idx] : T(0.0);
+          const T rold = PF ? pf_r[v] : ((a.flags & SW_READ_R) ? Rt[v * a.N + idx] : T(0.0));
           T r = rold + term;
           if (source && v > 0) r = r - src[v];
           if (a.flags & SW_WRITE_R) Rt[v * a.N + idx] = r;
           if (a.flags & SW_RK) {
             // stage combination in FP64 (BlockSolver::RK3_SSP, blockSolver.cpp:60,72,84)
-            const double u0 = a.U0[v * a.N + idx];
+            const double u0 = PF ? pf_u0[v] : a.U0[v * a.N + idx];
             const double rd = (double)r;
             double un;
             if (a.stage == 1) {
               un = u0 - dtv * rd;
             } else {
-              const double uc = (double)Ut[v * a.N + idx];
+              const double uc = PF ? pf_uc[v] : (double)Ut[v * a.N + idx];
               if (a.stage == 2) un = 0.75 * u0 - 0.25 * dtv * rd + 0.25 * uc;
               else un = 1.0 / 3.0 * u0 - 2.0 / 3.0 * dtv * rd + 2.0 / 3.0 * uc;
             }
@@ -372,7 +410,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
 __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_kernel(SweepArgs a) {
-  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T>(a, blockIdx.x, blockIdx.y);
+  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value>(a, blockIdx.x, blockIdx.y);
 }
 
 // ---- batched zones ------------------------------------------------------------
@@ -449,7 +487,7 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     a.x0 = 0.0;  // batches take neither position-dependent boundaries nor sources
     a.hx = 0.0;
     a.grav[0] = a.grav[1] = a.grav[2] = 0.0;
-    sweep_block<D, AX, SCH, FLX, CHARW, TILE, double>(a, __builtin_amdgcn_readfirstlane(w.bx),
+    sweep_block<D, AX, SCH, FLX, CHARW, TILE, double, false>(a, __builtin_amdgcn_readfirstlane(w.bx),
                                               __builtin_amdgcn_readfirstlane(w.by));
   }
 }
