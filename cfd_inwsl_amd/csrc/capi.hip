@@ -99,6 +99,9 @@ __global__ void k_soa_to_aos_f(const float* __restrict__ in, double* __restrict_
     for (int v = 0; v < NV; v++) out[i * NV + v] = (double)in[v * N + i];
 }
 
+// the halt flag of cfd_steps_to_output (zero in every other call)
+__device__ __forceinline__ bool halted(const double* scal) { return scal[10] != 0.0; }
+
 __global__ void k_zero_lambda(double* scal) {
   unsigned long long* b = reinterpret_cast<unsigned long long*>(scal);
   b[1] = 0ull;
@@ -112,6 +115,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_wave_speed(const double* __restrict__ U, long N, double h0, double h1,
                                                      double h2, double* scal) {
   constexpr int NV = D + 2;
+  if (halted(scal)) return;
   double m[3] = {0.0, 0.0, 0.0};
   const double hh[3] = {h0, h1, h2};
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
@@ -144,6 +148,7 @@ __global__ __launch_bounds__(256) void k_wave_speed(const double* __restrict__ U
 // max |R[var]| into scal[5] (bits of a non-negative double)
 template <class E>
 __global__ __launch_bounds__(256) void k_abs_max(const E* __restrict__ x, long N, double* scal) {
+  if (halted(scal)) return;
   double m = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
     double a = fabs((double)x[i]);
@@ -172,6 +177,49 @@ template <int D> __global__ void k_finish_dt(double* scal, double cfl, int advan
     scal[7] = scal[6];
     scal[6] = step_time(scal[6], dt);
   }
+}
+
+// stepsLoopCFL's dt (blockSolver.cpp:178-190): the CFL dt clamped to the next
+// output time, the clocks advanced; once an output step has run, the next
+// queued step raises the halt flag instead.
+template <int D> __global__ void k_clamp_dt(double* scal, double cfl, double output_dt) {
+  if (halted(scal)) return;
+  if (scal[9] != 0.0) {
+    scal[10] = 1.0;
+    return;
+  }
+  double lam = 0;
+  for (int d = 0; d < D; d++) lam += scal[1 + d];
+  double dt = cfl / lam;
+  if (scal[8] + dt > output_dt) {
+    dt = output_dt - scal[8];
+    scal[8] = 0.0;
+    scal[9] = 1.0;
+  } else {
+    scal[8] += dt;
+  }
+  scal[0] = dt;
+  scal[4] = step_time(scal[4], dt);
+  scal[7] = scal[6];
+  scal[6] = step_time(scal[6], dt);
+}
+
+__global__ void k_loop_begin(double* scal, double output_t) {
+  scal[8] = output_t;
+  scal[9] = 0.0;
+  scal[10] = 0.0;
+  scal[11] = 0.0;
+}
+
+// the step's log record: time after the step, dt, max |rho rhs| of the last stage
+__global__ void k_log_step(double* scal, double* log) {
+  if (halted(scal)) return;
+  const int i = (int)scal[11];
+  log[3 * i] = scal[6];
+  log[3 * i + 1] = scal[0];
+  log[3 * i + 2] = scal[5];
+  scal[11] = i + 1;
+  scal[5] = 0.0;
 }
 
 // Kahan-compensated partial sums per block and variable.
@@ -234,6 +282,7 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
   }
   a.x0 = h->cfg.origin;
   a.hx = h->cfg.h[0];
+  a.halt = h->halt;
   for (int d = 0; d < 3; d++) a.grav[d] = h->cfg.gravity[d];
   if (ax == h->D - 1 && has_source(h)) a.flags |= SW_SOURCE;
   const long nb = plan_pencil_blocks(h->cfg.n, ax, tile);
@@ -359,10 +408,10 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   if (e == hipSuccess) e = hipMalloc(&h->Ub, bytes);
   if (e == hipSuccess) e = hipMalloc(&h->R, ebytes);
   if (e == hipSuccess && h->fp32) e = hipMalloc(&h->U0f, sizeof(float) * h->N * h->NV);
-  if (e == hipSuccess) e = hipMalloc(&h->scal, 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->scal, 16 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&h->partial, sizeof(double) * 2 * 5 * h->npartial);
   if (e == hipSuccess) e = hipStreamCreate(&h->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(h->scal, 0, 8 * sizeof(double), h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(h->scal, 0, 16 * sizeof(double), h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
     std::string m = std::string("allocation: ") + hipGetErrorString(e);
@@ -383,6 +432,7 @@ int cfd_destroy(cfd_handle* h) {
   if (h->U0f) (void)hipFree(h->U0f);
   cfd_adjoint_release(h);
   (void)hipFree(h->scal);
+  if (h->steplog) (void)hipFree(h->steplog);
   (void)hipFree(h->partial);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -486,6 +536,79 @@ int cfd_cfl_dt(cfd_handle* h, double cfl, double* dt) {
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(dt, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  return CFD_OK;
+}
+
+int cfd_steps_to_output(cfd_handle* h, double cfl, double output_dt, double* output_t, int max_steps,
+                        double* log, int* nsteps, int* reached) {
+  if (!h || !output_t || !log || !nsteps || !reached || max_steps < 1) return cfd_fail(CFD_E_ARG, "bad argument");
+  if (!(cfl > 0) || !(output_dt > 0)) return cfd_fail(CFD_E_ARG, "cfl and output_dt must be > 0");
+  if (h->steplog_cap < max_steps) {
+    if (h->steplog) (void)hipFree(h->steplog);
+    h->steplog = nullptr;
+    h->steplog_cap = 0;
+    HIPCHK(hipMalloc(&h->steplog, 3 * sizeof(double) * (size_t)max_steps));
+    h->steplog_cap = max_steps;
+  }
+  const int keep = h->keep_rhs;
+  h->keep_rhs = 1;  // rhoLinf is taken from the last stage's rhs
+  h->halt = h->scal + 10;
+  int rc = 0;
+  int done = 0;
+  double flags[3] = {0, 0, 0};  // [9] reached, [10] halt, [11] steps
+  hipLaunchKernelGGL(k_loop_begin, dim3(1), dim3(1), 0, h->stream, h->scal, *output_t);
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 5, 0.0);
+  const int group = 8;
+  const int blocks = (int)std::min<long>((h->N + 255) / 256, 2048);
+  while (rc == 0 && done < max_steps) {
+    const int k = std::min(group, max_steps - done);
+    for (int s = 0; s < k && rc == 0; s++) {
+      hipLaunchKernelGGL(k_zero_lambda, dim3(1), dim3(1), 0, h->stream, h->scal);
+      const double* hh = h->cfg.h;
+      switch (h->D) {
+      case 1:
+        hipLaunchKernelGGL(k_wave_speed<1>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
+        hipLaunchKernelGGL(k_clamp_dt<1>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, output_dt);
+        break;
+      case 2:
+        hipLaunchKernelGGL(k_wave_speed<2>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
+        hipLaunchKernelGGL(k_clamp_dt<2>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, output_dt);
+        break;
+      default:
+        hipLaunchKernelGGL(k_wave_speed<3>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
+        hipLaunchKernelGGL(k_clamp_dt<3>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, output_dt);
+        break;
+      }
+      if (hipGetLastError() != hipSuccess) rc = cfd_fail(CFD_E_HIP, "launch failed");
+      if (rc == 0) rc = rk3(h);
+      if (rc == 0) {
+        if (h->fp32)
+          hipLaunchKernelGGL(k_abs_max<float>, dim3(blocks), dim3(256), 0, h->stream, static_cast<const float*>(h->R), h->N, h->scal);
+        else
+          hipLaunchKernelGGL(k_abs_max<double>, dim3(blocks), dim3(256), 0, h->stream, static_cast<const double*>(h->R), h->N, h->scal);
+        hipLaunchKernelGGL(k_log_step, dim3(1), dim3(1), 0, h->stream, h->scal, h->steplog);
+        if (hipGetLastError() != hipSuccess) rc = cfd_fail(CFD_E_HIP, "launch failed");
+      }
+    }
+    done += k;
+    if (rc) break;
+    if (hipMemcpyAsync(flags, h->scal + 9, sizeof flags, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      rc = cfd_fail(CFD_E_HIP, "step loop failed");
+      break;
+    }
+    if (flags[0] != 0.0) break;
+  }
+  h->keep_rhs = keep;
+  h->halt = nullptr;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 10, 0.0);
+  const int n = (int)flags[2];
+  if (n > 0) HIPCHK(hipMemcpyAsync(log, h->steplog, 3 * sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(output_t, h->scal + 8, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *nsteps = n;
+  *reached = flags[0] != 0.0;
   return CFD_OK;
 }
 

@@ -180,7 +180,6 @@ int main(int argc, char** argv) {
   CHECK(cfd_create(&cfg, &h));
   CHECK(cfd_set_state(h, cons.data()));
   CHECK(cfd_set_time(h, info.t));
-  CHECK(cfd_set_keep_rhs(h, 1));
   std::printf("initialized %s, %s, %s, %d cells\n", info.caseName().c_str(), info.schemeName().c_str(),
               info.gridInfo().c_str(), cfg.n[0] * cfg.n[1] * cfg.n[2]);
 
@@ -191,40 +190,35 @@ int main(int argc, char** argv) {
     return write_solution();
   };
 
-  // BlockSolver::stepsLoopCFL (blockSolver.cpp:170-233)
+  // BlockSolver::stepsLoopCFL (blockSolver.cpp:170-233): the dt clamp, the
+  // RK3 steps, the clock and the per-step log values run on the device up to
+  // the next output time (cfd_steps_to_output); the host prints the log lines
+  // and writes the solution.
   const double eps = 1e-15;
   long timesteps = 0;
   long long solverMs = 0;
+  const int maxSteps = 256;
+  std::vector<double> steplog(3 * maxSteps);
   auto t0 = std::chrono::high_resolution_clock::now();
   while (info.step < info.endStep) {
     if (info.outputT < eps && dump()) return 1;
-    double dt;
-    CHECK(cfd_cfl_dt(h, info.CFL, &dt));
-    if (info.outputT + dt > info.outputDt) {
-      dt = info.outputDt - info.outputT;
-      info.outputT = 0.0;
-      info.step++;
-    } else {
-      info.outputT += dt;
-    }
+    int n = 0, reached = 0;
     auto s0 = std::chrono::high_resolution_clock::now();
-    CHECK(cfd_rk3_step(h, dt));
-    CHECK(cfd_synchronize(h));
+    CHECK(cfd_steps_to_output(h, info.CFL, info.outputDt, &info.outputT, maxSteps, steplog.data(), &n, &reached));
     auto s1 = std::chrono::high_resolution_clock::now();
     solverMs += std::chrono::duration_cast<std::chrono::milliseconds>(s1 - s0).count();
-    // RK3_SSP's stage bookkeeping then BlockSolver::solve (blockSolver.cpp:62-123);
-    // the library advances its own copy the same way
-    info.t = ((info.t + dt) - dt / 2 - dt / 2) + dt;
-    timesteps++;
-    double linf;
-    CHECK(cfd_rhs_linf(h, 0, &linf));
-    char line[160];
-    std::snprintf(line, sizeof(line), "time = %.4f dt=%.10f  rhoLinf = %.4f  \n", info.t, dt, linf);
-    std::fputs(line, stdout);
-    if (output) {
-      std::ofstream lf(logName, std::ios::app);
-      lf << line;
+    std::ofstream lf;
+    if (output) lf.open(logName, std::ios::app);
+    for (int i = 0; i < n; i++) {
+      info.t = steplog[3 * i];
+      char line[160];
+      std::snprintf(line, sizeof(line), "time = %.4f dt=%.10f  rhoLinf = %.4f  \n", info.t, steplog[3 * i + 1],
+                    steplog[3 * i + 2]);
+      std::fputs(line, stdout);
+      if (output) lf << line;
     }
+    timesteps += n;
+    if (reached) info.step++;
   }
   if (dump()) return 1;
   auto t1 = std::chrono::high_resolution_clock::now();

@@ -20,7 +20,14 @@ struct cfd_handle {
   float* U0f;            // FP32 shadow of U0 (FP32 precision only)
   bool fp32;
   size_t esz;            // bytes per element of Ua / Ub / R
-  double* scal;          // [0] dt, [1..3] max wave speed per direction (bits), [4] t
+  // device scalars: [0] dt, [1..3] max wave speed per direction (bits), [4] time
+  // elapsed in the current call, [5] max |rhs| (bits), [6] absolute time,
+  // [7] time at the start of the step; cfd_steps_to_output: [8] time since the
+  // last output, [9] output reached, [10] halt the queued steps, [11] steps done
+  double* scal;
+  const double* halt;    // scal + 10 while cfd_steps_to_output queues steps, else null
+  double* steplog;       // (t, dt, rhoLinf) per step of cfd_steps_to_output
+  int steplog_cap;
   double* partial;       // block partial sums for totals
   int npartial;
   int keep_rhs;

@@ -66,6 +66,9 @@ struct SweepArgs {
   double x0, hx;
   // SW_SOURCE: gravity source on the last sweep (SourceTerm::calGravitySource)
   double grav[3];
+  // when non-null and *halt != 0 the launch does nothing (queued steps after
+  // an output time in cfd_steps_to_output)
+  const double* halt;
 };
 
 // Tile shapes: CA interfaces per chunk along the sweep axis x CP pencils.
@@ -410,6 +413,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
 __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_kernel(SweepArgs a) {
+  if (a.halt && *a.halt != 0.0) return;
   sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value>(a, blockIdx.x, blockIdx.y);
 }
 
@@ -468,6 +472,7 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     a.U0 = b.U0 + zd.off;
     a.Uout = b.Uout + zd.off;
     a.Ushadow = nullptr;
+    a.halt = nullptr;
     a.dtp = b.dt + z;
     a.N = zd.N;
     a.n[0] = zd.n[0];

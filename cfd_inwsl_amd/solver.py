@@ -89,7 +89,7 @@ BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_ZONE = 0, 1, 2
 EXPORTS = [
     "cfd_last_error", "cfd_version", "cfd_create", "cfd_destroy", "cfd_set_state", "cfd_get_state",
     "cfd_eval_rhs", "cfd_rk3_step", "cfd_rk3_steps_cfl", "cfd_cfl_dt", "cfd_conserved_totals",
-    "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf", "cfd_autotune", "cfd_launch_plan", "cfd_set_launch_plan",
+    "cfd_steps_to_output", "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf", "cfd_autotune", "cfd_launch_plan", "cfd_set_launch_plan",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_state", "cfd_batch_get_state", "cfd_batch_steps_cfl",
     "cfd_batch_times", "cfd_batch_autotune", "cfd_batch_synchronize",
     "cfd_rhs_vjp", "cfd_adjoint_forward", "cfd_adjoint_backward", "cfd_set_time", "cfd_get_time",
@@ -130,6 +130,8 @@ def load_library():
     lib.cfd_synchronize.argtypes = [P]
     lib.cfd_device_buffers.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
     lib.cfd_time_sweeps.argtypes = [P, ctypes.c_int, pd]
+    lib.cfd_steps_to_output.argtypes = [P, ctypes.c_double, ctypes.c_double, pd, ctypes.c_int, pd,
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.cfd_copy_probe.argtypes = [P, ctypes.c_int]
     lib.cfd_set_keep_rhs.argtypes = [P, ctypes.c_int]
     lib.cfd_rhs_linf.argtypes = [P, ctypes.c_int, pd]
@@ -299,6 +301,16 @@ class Solver:
         t = (ctypes.c_int * 3)(*(list(tiles) + [1] * (3 - len(tiles))))
         g = (ctypes.c_int * 3)(*(list(segs) + [1] * (3 - len(segs))))
         self._check(self.lib.cfd_set_launch_plan(self.h, t, g))
+
+    def steps_to_output(self, cfl, output_dt, output_t=0.0, max_steps=64):
+        """stepsLoopCFL on the device up to the next output time (or max_steps):
+        returns (log [(t, dt, rhoLinf) per step], output_t, reached)."""
+        ot = ctypes.c_double(output_t)
+        log = np.empty((max_steps, 3))
+        n, reached = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.cfd_steps_to_output(self.h, float(cfl), float(output_dt), ctypes.byref(ot),
+                                                 int(max_steps), _pd(log), ctypes.byref(n), ctypes.byref(reached)))
+        return log[:n.value].copy(), ot.value, bool(reached.value)
 
     def set_time(self, t):
         self._check(self.lib.cfd_set_time(self.h, float(t)))

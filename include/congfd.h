@@ -122,6 +122,20 @@ int cfd_rk3_step(cfd_handle* h, double dt);
  * the elapsed simulated time; the call is asynchronous when t_out is NULL. */
 int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out);
 
+/* BlockSolver::stepsLoopCFL (blockSolver.cpp:170-233) on the device up to the
+ * next output time: each step takes the CFL dt, clamped as the reference
+ * clamps it (if output_t + dt > output_dt: dt = output_dt - output_t,
+ * output_t = 0, an output is due; else output_t += dt), advances RK3 and the
+ * clock, and records (t, dt, rhoLinf) -- the per-step log line of the
+ * reference.  Stops after the step that reaches the output time or after
+ * max_steps steps.  output_t (in/out) is the reference's Info::outputT; log
+ * (host, 3 * max_steps doubles) receives the records of *nsteps steps;
+ * *reached is 1 when an output time was reached.  Steps are queued in groups
+ * without host synchronisation; the steps of a group after the output step
+ * do nothing. */
+int cfd_steps_to_output(cfd_handle* h, double cfl, double output_dt, double* output_t, int max_steps,
+                        double* log, int* nsteps, int* reached);
+
 /* When keep != 0 the RK3 calls also store the rhs of each stage (the last
  * stage's rhs stays readable, as the reference's Equation::rhs does after
  * RK3_SSP); off by default, which saves one rhs write per step. */

@@ -237,3 +237,61 @@ def test_full_size_3d_freestream_and_conservation():
     r = s.rhs()
     s.close()
     assert np.all(r == 0.0)
+
+
+def _manual_rk3(grid, u0, dt):
+    """One RK3 step written out (the oracle's stage expressions), returning the
+    new state and the last stage's rhs (the reference logs its rho Linf)."""
+    r = O.rhs(grid, u0)[0]
+    u1 = u0 - dt * r
+    grid.t = grid.t + dt
+    r = O.rhs(grid, u1)[0]
+    u2 = 0.75 * u0 - 0.25 * dt * r + 0.25 * u1
+    grid.t = grid.t - dt / 2
+    r = O.rhs(grid, u2)[0]
+    u3 = 1.0 / 3.0 * u0 - 2.0 / 3.0 * dt * r + 2.0 / 3.0 * u2
+    grid.t = grid.t - dt / 2
+    return u3, r
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_steps_to_output_is_steps_loop_cfl(dim):
+    """cfd_steps_to_output: the CFL dt clamped at output times, the steps, the
+    clock and the (t, dt, rhoLinf) log of stepsLoopCFL, all on the device,
+    against the oracle driven through the same loop; a call stops after the
+    step that reaches the output time (the queued steps after it do nothing),
+    or after max_steps steps, and output_t carries over between calls."""
+    n = {1: (120,), 2: (40, 36)}[dim]
+    bc = [BC.OUTFLOW] * 6
+    cons = random_state(dim, n, seed=41, jumps=True)
+    s = Solver(dim, n, 0.02, InterMethod.TCNS5, FluxMethod.HLLC, Recon.CHAR, bc)
+    s.set_state(cons)
+    grid = oracle_grid(dim, n, 0.02, InterMethod.TCNS5, FluxMethod.HLLC, Recon.CHAR, bc)
+    out_dt = 0.0137
+    u, t, out_t = cons.copy(), 0.0, 0.0
+    log_all, calls = [], []
+    for max_steps in (5, 64, 64, 3, 64):
+        log, out_t_dev, reached = s.steps_to_output(0.5, out_dt, out_t, max_steps)
+        calls.append((len(log), reached))
+        for rec in log:
+            dt = O.cfl_dt(grid, u, 0.5)
+            hit = out_t + dt > out_dt
+            if hit:
+                dt = out_dt - out_t
+                out_t = 0.0
+            else:
+                out_t += dt
+            grid.t = t
+            u, r = _manual_rk3(grid, u, dt)
+            t = ((t + dt) - dt / 2 - dt / 2) + dt
+            assert rec[1] == dt and rec[0] == t, (rec, dt, t)
+            assert rec[2] == np.abs(r[:, 0]).max()
+        assert out_t_dev == out_t
+        assert reached == (out_t == 0.0)
+        assert reached or len(log) == max_steps
+        log_all += list(log)
+    _say(f"steps_to_output dim={dim}: calls (steps, reached) {calls}")
+    assert any(c[1] for c in calls) and any(not c[1] for c in calls)
+    _check(s.get_state(), u, f"steps_to_output dim={dim}")
+    assert s.get_time() == t
+    s.close()
