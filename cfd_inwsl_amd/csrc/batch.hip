@@ -24,7 +24,7 @@ using namespace cfd;
 
 // error slot and scheme table of the single-zone entry points (capi.hip)
 int cfd_set_error_from_batch(int code, const char* msg);
-int cfd_select_scheme(int inter, int* sch, double* ct);
+int cfd_select_scheme(int inter, int dim, int* sch, double* ct);
 
 #define BCHK(expr)                                                                                 \
   do {                                                                                             \
@@ -323,7 +323,7 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
   if (c->precision != CFD_PRECISION_FP64) return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches run in FP64");
   if (c->gravity[0] != 0.0 || c->gravity[1] != 0.0 || c->gravity[2] != 0.0)
     return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches take no source term");
-  if (cfd_select_scheme(c->inter_method, &sch, &ct))
+  if (cfd_select_scheme(c->inter_method, c->dim, &sch, &ct))
     return cfd_set_error_from_batch(CFD_E_UNSUPP, "inter_method is not implemented");
   cfd_batch* b = new cfd_batch();
   b->common = *c;

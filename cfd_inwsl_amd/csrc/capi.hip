@@ -36,7 +36,7 @@ int cfd_set_error_from_batch(int code, const char* msg) { return cfd_fail(code, 
 // SolverType::initReconer for EULER (zone_FD/src/src/solverType.cpp:217-594).
 // Values the reference routes to its default branch map to weno5_JSchen as
 // there; research variants it lists explicitly are not implemented here.
-int cfd_select_scheme(int inter, int* sch, double* ct) {
+int cfd_select_scheme(int inter, int dim, int* sch, double* ct) {
   *ct = 1e-5;
   switch (inter) {
   case 2: *sch = SCH_JS; return 0;             // WCNS5
@@ -44,7 +44,17 @@ int cfd_select_scheme(int inter, int* sch, double* ct) {
   case 11: case 35: *sch = SCH_TENO; return 0; // TCNS5, new_TCNS5
   case 12: *sch = SCH_TENO; *ct = 1e-4; return 0;  // TCNS5CT4
   case 13: *sch = SCH_TENO; *ct = 1e-7; return 0;  // TCNS5CT7
-  case 8: case 37: *sch = SCH_TENOS; return 0;     // WCNS5CONGZ, new_WCNS5CONGZ
+  case 22: *sch = SCH_TENO; return 0;              // congTCNS5CT5 (congTcns5_ZCT5: Teno5_Z's arithmetic)
+  case 23: *sch = SCH_TENO; *ct = 1e-10; return 0; // congTCNS5CT10
+  case 36: *sch = SCH_TENOA; return 0;             // new_WHFTCNSA, "TENO-A" (new_whf_TCNS_A)
+  case 8: case 37: *sch = SCH_TENOS; *ct = 0.15704178024750198; return 0;  // WCNS5CONGZ, new_WCNS5CONGZ
+  case 9: *sch = SCH_TENOS; *ct = 0.23050581003334941; return 0;           // WCNS5CONGZCT4
+  case 10:  // WCNS5CONGZCT7: Teno5_CongZCT7 in 2D (solverType.cpp:435-437); the 1D
+            // Euler path wires Teno5_CongAA instead (:248-250), which is not built
+    if (dim == 1) return -1;
+    *sch = SCH_TENOS;
+    *ct = 0.072892337360747711;
+    return 0;
   case 0: case 1: case 4: case 5: case 6: case 7: case 14: case 16:
     *sch = SCH_JS;  // reference default branch
     return 0;
@@ -353,7 +363,7 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
     return cfd_fail(CFD_E_ARG, "recon must be CFD_RECON_CHAR or CFD_RECON_COMPONENT");
   int sch;
   double ct;
-  if (cfd_select_scheme(cfg->inter_method, &sch, &ct))
+  if (cfd_select_scheme(cfg->inter_method, cfg->dim, &sch, &ct))
     return cfd_fail(CFD_E_UNSUPP, "inter_method " + std::to_string(cfg->inter_method) + " is not implemented");
   for (int d = 0; d < 3; d++) {
     int want_active = d < cfg->dim;

@@ -17,7 +17,7 @@ namespace cfd {
 // zone_FD/src/include/macro.hpp:28
 #define CFD_GAMMA 1.4
 
-enum SchemeId { SCH_JS = 0, SCH_Z = 1, SCH_TENO = 2, SCH_TENOS = 3 };
+enum SchemeId { SCH_JS = 0, SCH_Z = 1, SCH_TENO = 2, SCH_TENOS = 3, SCH_TENOA = 4 };
 enum FluxId { FLX_HLLC = 0, FLX_ROE = 1 };
 
 template <class T> __device__ __forceinline__ T sq(T a) { return a * a; }
@@ -161,12 +161,55 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
     T CT = ct * (g0 + g1 + g2);
     unsigned flag = (g0 < CT ? 1u : 0u) + (g1 < CT ? 2u : 0u) + (g2 < CT ? 4u : 0u);
     return teno_pick(flag, q0, q1, q2, q3, q4);
+  } else if (SCH == SCH_TENOA) {
+    // new_whf_TCNS_A, TENO-A (interScheme.hpp:3863-3983): Teno5_Z's cut-off with
+    // the adaptive constant 10^-beta_A, beta_A = floor(10 - 5 (1 - g)),
+    // g = s^4 (5 - 4 s), s = min(1, 4.1666667 eta_min)
+    T tau = fabs(b2 - b0);
+    const T epsA = T(2.7551 * 1e-7);
+    const T d0 = q0 - q1, d1 = q1 - q2, d2 = q2 - q3, d3 = q3 - q4;
+    const T e0 = fdv(fabs(T(2.0) * d0 * d1) + epsA, d0 * d0 + d1 * d1 + epsA);
+    const T e1 = fdv(fabs(T(2.0) * d1 * d2) + epsA, d1 * d1 + d2 * d2 + epsA);
+    const T e2 = fdv(fabs(T(2.0) * d2 * d3) + epsA, d2 * d2 + d3 * d3 + epsA);
+    const T emin = smin(smin(e0, e1), e2);
+    const T mn = smin(T(1.0), T(4.1666667) * emin);
+    const T gm = mn * mn * mn * mn * (T(5.0) - T(4) * mn);
+    const int betaA = (int)floor(T(10.0) - T(5.0) * (T(1.0) - gm));
+    T t0, t1, t2;
+    if (sizeof(T) == sizeof(double)) {
+      t0 = T(1.0) + fdv(tau, b0 + eps_tiny<T>());
+      t1 = T(1.0) + fdv(tau, b1 + eps_tiny<T>());
+      t2 = T(1.0) + fdv(tau, b2 + eps_tiny<T>());
+    } else {
+      // FP32: the scaled terms of the TENO branch (the test is scale invariant)
+      T s0 = b0 + eps_tiny<T>(), s1 = b1 + eps_tiny<T>(), s2 = b2 + eps_tiny<T>();
+      T sm = smin(smin(s0, s1), s2);
+      T base = T(1.0);
+      T num = tau;
+      if (tau > sm) {
+        base = sm / tau;
+        num = sm;
+      }
+      t0 = base + num / s0;
+      t1 = base + num / s1;
+      t2 = base + num / s2;
+    }
+    t0 = t0 * t0;
+    t1 = t1 * t1;
+    t2 = t2 * t2;
+    T g0 = t0 * t0 * t0, g1 = t1 * t1 * t1, g2 = t2 * t2 * t2;
+    const T cta = betaA <= 5 ? T(1e-5) : betaA == 6 ? T(1e-6) : betaA == 7 ? T(1e-7)
+                : betaA == 8 ? T(1e-8) : betaA == 9 ? T(1e-9) : T(1e-10);
+    T rr = cta * (g0 + g1 + g2);
+    unsigned flag = (g0 < rr ? 1u : 0u) + (g1 < rr ? 2u : 0u) + (g2 < rr ? 4u : 0u);
+    return teno_pick(flag, q0, q1, q2, q3, q4);
   } else {
-    // Teno5_CongZ, TENO-S (interScheme.hpp:494-585)
+    // Teno5_CongZ, TENO-S (interScheme.hpp:494-585); ct = its constant CT
+    // (0.15704178024750198; Teno5_CongZCT4 / CT7, :586 / :661, differ in it only)
     T bm = b0;
     if (b1 < bm) bm = b1;
     if (b2 < bm) bm = b2;
-    const T CT = T(0.15704178024750198);
+    const T CT = ct;
     const T CT_1 = 1 - CT;
     T tau = fabs(b2 - b0);
     T rr = CT * tau - CT_1 * bm;

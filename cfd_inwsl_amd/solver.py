@@ -16,12 +16,17 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 class InterMethod(enum.IntEnum):
     """InterMethod (zone_FD/src/include/macro.hpp:37-77), implemented subset."""
-    WCNS5 = 2        # weno5_JSchen
-    WCNSZ5 = 3       # weno5_Z
-    WCNS5CONGZ = 8   # Teno5_CongZ (TENO-S)
-    TCNS5 = 11       # Teno5_Z (TENO5)
+    WCNS5 = 2            # weno5_JSchen
+    WCNSZ5 = 3           # weno5_Z
+    WCNS5CONGZ = 8       # Teno5_CongZ (TENO-S)
+    WCNS5CONGZCT4 = 9    # Teno5_CongZCT4
+    WCNS5CONGZCT7 = 10   # Teno5_CongZCT7 (2D/3D; the reference's 1D path wires another scheme)
+    TCNS5 = 11           # Teno5_Z (TENO5)
     TCNS5CT4 = 12
     TCNS5CT7 = 13
+    CONGTCNS5CT5 = 22    # congTcns5_ZCT5 (Teno5_Z's arithmetic)
+    CONGTCNS5CT10 = 23   # congTcns5_ZCT10
+    NEW_WHFTCNSA = 36    # new_whf_TCNS_A ("TENO-A")
 
 
 class FluxMethod(enum.IntEnum):

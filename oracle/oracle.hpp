@@ -26,7 +26,10 @@ namespace oracle {
 constexpr double GAMMA = 1.4;
 
 // Interpolation schemes (zone_FD/src/include/interScheme.hpp).
-enum Scheme { S_JS = 0, S_Z = 1, S_TENO5 = 2, S_TENO5_CT4 = 3, S_TENO5_CT7 = 4, S_TENOS = 5, S_LINEAR = 6 };
+enum Scheme {
+  S_JS = 0, S_Z = 1, S_TENO5 = 2, S_TENO5_CT4 = 3, S_TENO5_CT7 = 4, S_TENOS = 5, S_LINEAR = 6,
+  S_TENOS_CT4 = 7, S_TENOS_CT7 = 8, S_TENO5_CT10 = 9, S_TENOA = 10
+};
 enum Flux { F_HLLC = 0, F_ROE = 1 };
 
 template <class T> inline T sq(const T& a) { return a * a; }
@@ -111,15 +114,15 @@ template <class T> inline T teno5(const T q[5], double ct) {
   return teno_select(flag, q);
 }
 
-// Teno5_CongZ, "TENO-S" (interScheme.hpp:494-585)
-template <class T> inline T tenos(const T q[5]) {
+// Teno5_CongZ, "TENO-S" (interScheme.hpp:494-585); the CT4 / CT7 variants
+// Teno5_CongZCT4 / CT7 (:586, :661) differ only in the constant CT.
+template <class T> inline T tenos(const T q[5], const double CT = 0.15704178024750198) {
   using std::abs;
   T b[3];
   betas(q, b);
   int m = 0;  // std::min_element: first minimal index
   if (b[1] < b[m]) m = 1;
   if (b[2] < b[m]) m = 2;
-  const double CT = 0.15704178024750198;
   const double CT_1 = 1 - CT;
   T tau = abs(b[2] - b[0]);
   T rr = CT * tau - CT_1 * b[m];
@@ -128,6 +131,44 @@ template <class T> inline T tenos(const T q[5]) {
   if (ll < rr * b[0]) flag += 1;
   if (ll < rr * b[1]) flag += 2;
   if (ll < rr * b[2]) flag += 4;
+  return teno_select(flag, q);
+}
+
+// new_whf_TCNS_A, "TENO-A" (interScheme.hpp:3863-3983; InterMethod
+// new_WHFTCNSA, named "TENO-A" in info.cpp): Teno5_Z's cut-off with the
+// adaptive constant CT = 10^-beta_A, beta_A = floor(10 - 5 (1 - g)),
+// g = s^4 (5 - 4 s), s = min(1, 4.1666667 eta_min), eta the smoothness ratios
+// of adjacent differences.
+template <class T> inline T tenoa(const T q[5]) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  const double epsA = 2.7551 * 1e-7;
+  const T d0 = q[0] - q[1], d1 = q[1] - q[2], d2 = q[2] - q[3], d3 = q[3] - q[4];
+  const T e0 = (abs(2.0 * d0 * d1) + epsA) / (d0 * d0 + d1 * d1 + epsA);
+  const T e1 = (abs(2.0 * d1 * d2) + epsA) / (d1 * d1 + d2 * d2 + epsA);
+  const T e2 = (abs(2.0 * d2 * d3) + epsA) / (d2 * d2 + d3 * d3 + epsA);
+  T emin = e0;
+  if (e1 < emin) emin = e1;
+  if (e2 < emin) emin = e2;
+  const T x = 4.1666667 * emin;
+  const T mn = x < 1.0 ? x : T(1.0);
+  const T gm = mn * mn * mn * mn * (5.0 - 4 * mn);
+  const int betaA = (int)std::floor((double)(10.0 - 5.0 * (1.0 - gm)));
+  T g[3];
+  for (int i = 0; i < 3; i++) {
+    T t = 1.0 + tau / (b[i] + 1e-40);
+    t = t * t;
+    g[i] = t * t * t;
+  }
+  const double cts[6] = {1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10};
+  const double CTA = cts[betaA < 5 ? 0 : (betaA > 10 ? 5 : betaA - 5)];
+  T rr = CTA * (g[0] + g[1] + g[2]);
+  unsigned flag = 0;
+  if (g[0] < rr) flag += 1;
+  if (g[1] < rr) flag += 2;
+  if (g[2] < rr) flag += 4;
   return teno_select(flag, q);
 }
 
@@ -142,6 +183,10 @@ template <class T> inline T inter5(int s, const T q[5]) {
   case S_TENO5_CT4: return teno5(q, 1e-4);
   case S_TENO5_CT7: return teno5(q, 1e-7);
   case S_TENOS: return tenos(q);
+  case S_TENOS_CT4: return tenos(q, 0.23050581003334941);
+  case S_TENOS_CT7: return tenos(q, 0.072892337360747711);
+  case S_TENO5_CT10: return teno5(q, 1e-10);
+  case S_TENOA: return tenoa(q);
   default: return linear5(q);
   }
 }

@@ -12,6 +12,11 @@ SCHEME_OF = {
     InterMethod.TCNS5CT4: O.S_TENO5_CT4,
     InterMethod.TCNS5CT7: O.S_TENO5_CT7,
     InterMethod.WCNS5CONGZ: O.S_TENOS,
+    InterMethod.WCNS5CONGZCT4: O.S_TENOS_CT4,
+    InterMethod.WCNS5CONGZCT7: O.S_TENOS_CT7,
+    InterMethod.CONGTCNS5CT5: O.S_TENO5,
+    InterMethod.CONGTCNS5CT10: O.S_TENO5_CT10,
+    InterMethod.NEW_WHFTCNSA: O.S_TENOA,
 }
 
 

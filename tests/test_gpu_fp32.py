@@ -92,3 +92,19 @@ def test_fp32_steps_against_fp64(dim):
     drift = np.abs(t1 - t0) / np.abs(cons).sum(0)
     _say("fp32 totals drift", drift)
     assert np.all(drift < 1e-6)
+
+
+def test_fp32_teno_a_rhs_against_oracle():
+    """TENO-A's FP32 kernels (scaled gamma terms, the adaptive constant from
+    FP32 difference ratios) on the smooth vortex against the FP64 oracle."""
+    cons, h = cases.vortex(128)
+    bc = [BC.PERIODIC] * 4
+    s = Solver(2, (128, 128), h, InterMethod.NEW_WHFTCNSA, FluxMethod.HLLC, Recon.CHAR, bc, precision=PRECISION_FP32)
+    s.set_state(cons)
+    g = s.rhs()
+    s.close()
+    ref, _ = O.rhs(oracle_grid(2, (128, 128), h, InterMethod.NEW_WHFTCNSA, FluxMethod.HLLC, Recon.CHAR, bc), cons)
+    l1, linf = _errs(g, ref)
+    _say("fp32 TENO-A rhs vortex 128^2: L1", l1, "Linf", linf)
+    assert np.all(linf < 1e-4)
+    assert np.all(l1 < 1e-5)

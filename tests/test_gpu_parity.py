@@ -41,6 +41,32 @@ COMBOS = list(itertools.product(
 ))
 
 
+RESEARCH = [InterMethod.WCNS5CONGZCT4, InterMethod.WCNS5CONGZCT7, InterMethod.CONGTCNS5CT5,
+            InterMethod.CONGTCNS5CT10, InterMethod.NEW_WHFTCNSA]
+
+
+@pytest.mark.parametrize("dim,inter", [(d, i) for d in (1, 2, 3) for i in RESEARCH
+                                       if not (d == 1 and i == InterMethod.WCNS5CONGZCT7)])
+def test_rhs_parity_research_variants(dim, inter):
+    """The research TENO variants (runtime cut-off constants of TENO5 / TENO-S,
+    and TENO-A's own kernels) against the oracle, characteristic-wise HLLC
+    and component-wise Roe, FP64."""
+    n = SIZES[dim]
+    cons = random_state(dim, n, seed=29, jumps=True)
+    for flux, recon in ((FluxMethod.HLLC, Recon.CHAR), (FluxMethod.ROE, Recon.COMPONENT)):
+        s = Solver(dim, n, 0.05, inter, flux, recon, [BC.PERIODIC] * 6)
+        s.set_state(cons)
+        g = s.rhs()
+        s.close()
+        ref, _ = O.rhs(oracle_grid(dim, n, 0.05, inter, flux, recon, [BC.PERIODIC] * 6), cons)
+        _check(g, ref, f"research rhs dim={dim} inter={inter.name} {flux.name} {recon.name}")
+
+
+def test_research_variant_wired_to_another_scheme_in_1d_is_refused():
+    with pytest.raises(Exception, match="not implemented"):
+        Solver(1, (40,), 0.05, InterMethod.WCNS5CONGZCT7, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 2)
+
+
 @pytest.mark.parametrize("dim,inter,flux,recon", COMBOS)
 def test_rhs_parity(dim, inter, flux, recon):
     n = SIZES[dim]

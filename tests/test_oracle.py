@@ -15,7 +15,8 @@ import pytest
 from oracle import oracle as O
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-SCHEMES = [O.S_JS, O.S_Z, O.S_TENO5, O.S_TENO5_CT4, O.S_TENO5_CT7, O.S_TENOS, O.S_LINEAR]
+SCHEMES = [O.S_JS, O.S_Z, O.S_TENO5, O.S_TENO5_CT4, O.S_TENO5_CT7, O.S_TENOS, O.S_LINEAR,
+           O.S_TENOS_CT4, O.S_TENOS_CT7, O.S_TENO5_CT10, O.S_TENOA]
 
 
 # ---------------------------------------------------------------- accuracy
@@ -81,7 +82,7 @@ def test_inter5_exact_for_quadratics(scheme):
         assert abs(O.inter5(scheme, q) - want) < 1e-12 * (1 + abs(want) + abs(c) * 10)
 
 
-@pytest.mark.parametrize("scheme", [O.S_LINEAR, O.S_TENO5])
+@pytest.mark.parametrize("scheme", [O.S_LINEAR, O.S_TENO5, O.S_TENO5_CT10, O.S_TENOA])
 def test_inter5_fifth_order_exact_for_quartics(scheme):
     rng = np.random.default_rng(2)
     for _ in range(20):
@@ -102,6 +103,73 @@ def test_teno_step_selects_smooth_candidate():
     # all three stencils smooth -> linear 5th-order weights
     q = [1.0, 2.0, 3.0, 4.0, 5.0]
     assert O.inter5(O.S_TENO5, q) == O.inter5(O.S_LINEAR, q) == 3.5
+
+
+# the eight stencils of the TENO family by flag (interScheme.hpp:421-449):
+# 0 five-point linear, 1 and 4 four-point, 2/3 u2, 5 u1, 6 u0, 7 q2
+def _stencils(q):
+    q0, q1, q2, q3, q4 = q
+    u0 = 3 / 8 * q0 - 5 / 4 * q1 + 15 / 8 * q2
+    u1 = -1 / 8 * q1 + 3 / 4 * q2 + 3 / 8 * q3
+    u2 = 3 / 8 * q2 + 3 / 4 * q3 - 1 / 8 * q4
+    return [3 / 128 * q0 - 5 / 32 * q1 + 45 / 64 * q2 + 15 / 32 * q3 - 5 / 128 * q4,
+            -1 / 16 * q1 + 9 / 16 * q2 + 9 / 16 * q3 - 1 / 16 * q4, u2, u2,
+            1 / 16 * q0 - 5 / 16 * q1 + 15 / 16 * q2 + 5 / 16 * q3, u1, u0, q2]
+
+
+def _cut(scheme, q):
+    """Set of stencils a TENO-family scheme cut at q (bits of its flag), read
+    off from which of the eight stencil values it returned."""
+    v = O.inter5(scheme, q)
+    st = _stencils(q)
+    flags = [0, 1, 2, 4, 5, 6, 7]  # flag 3 returns flag 2's stencil: not told apart
+    vals = [st[f] for f in flags]
+    hits = [f for f, x in zip(flags, vals) if abs(x - v) <= 1e-12 * (1 + abs(v))]
+    if len({round(x, 9) for x in vals}) < len(vals) or len(hits) != 1 or hits[0] == 2:
+        return None  # ambiguous
+    f = hits[0]
+    return {0: set(), 1: {0}, 2: {1}, 3: {0, 1}, 4: {2}, 5: {0, 2}, 6: {1, 2}, 7: {0, 1, 2}}[f]
+
+
+def test_teno_cutoff_constants_order_the_cut_stencils():
+    """The cut-off constant orders the stencils cut, for every data: TENO5
+    cuts stencil k when g_k < CT sum(g), TENO-S when ll < (CT tau - (1-CT)
+    beta_min) beta_k -- both monotone in CT.  So CT 1e-10 cuts a subset of
+    CT 1e-7, of CT 1e-5, of CT 1e-4 (TENO5), and TENO-S CT7 (0.0729) a subset
+    of CT5 (0.157), of CT4 (0.2305).  A variant wired to the wrong constant
+    breaks the chain."""
+    rng = np.random.default_rng(12)
+    chains = [[O.S_TENO5_CT10, O.S_TENO5_CT7, O.S_TENO5, O.S_TENO5_CT4],
+              [O.S_TENOS_CT7, O.S_TENOS, O.S_TENOS_CT4]]
+    strict = [0, 0]
+    for _ in range(3000):
+        k = rng.uniform(0.2, 3.0)
+        q = np.sin(k * np.arange(5.0) + rng.uniform(0, 6.3)) + rng.normal(scale=rng.choice([0, 1e-3, 0.3]), size=5)
+        for c, chain in enumerate(chains):
+            sets = [_cut(s, q) for s in chain]
+            if any(x is None for x in sets):
+                continue
+            for a, b in zip(sets, sets[1:]):
+                assert a <= b, (chain, q, sets)
+            strict[c] += sets[0] != sets[-1]
+    assert min(strict) > 50  # the constants do separate the variants
+
+
+def test_teno_a_adapts_its_constant():
+    """TENO-A (new_whf_TCNS_A) is TENO5 with CT = 10^-beta_A: on a step the
+    difference ratios are ~eps_A, so beta_A = 5 (CT 1e-5, TENO5); on resolved
+    oscillations they are ~1, so beta_A = 10 (CT 1e-10)."""
+    step = [0.0, 0.0, 0.0, 1.0, 1.0]
+    assert O.inter5(O.S_TENOA, step) == O.inter5(O.S_TENO5, step) == 0.0
+    q = np.sin(1.7488 * np.arange(5.0) + 0.7692)  # where CT 1e-5 and 1e-10 cut differently
+    assert O.inter5(O.S_TENO5, q) != O.inter5(O.S_TENO5_CT10, q)
+    assert O.inter5(O.S_TENOA, q) == O.inter5(O.S_TENO5_CT10, q)
+    # and its output is always one of the TENO5 family's
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        q = rng.normal(size=5)
+        assert O.inter5(O.S_TENOA, q) in {O.inter5(s, q) for s in (O.S_TENO5, O.S_TENO5_CT7, O.S_TENO5_CT10)} or \
+            _cut(O.S_TENOA, q) is not None
 
 
 def test_weno_weights_js_known_value():
