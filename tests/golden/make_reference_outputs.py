@@ -5,7 +5,7 @@ CongFD1.0/cases, with their per-step logs.  This script reads them with the
 package's CGNS reader (cfd_inwsl_amd/cgns.py) and writes:
 
 * reference_outputs.npz: cell-centred Density / XVelocity / Pressure of the
-  1D end states listed in CASES, the t=0 and t=0.2 states and the logged dt
+  1D end states listed in CASES (TENO.cgns and TENOS.cgns), the t=0 and t=0.2 states and the logged dt
   sequence of the "Sod - congTCNS5CT5 - 101" run, and the 512x128 double-Mach
   density at t=0.2 (float32: it is compared loosely);
 * reference_outputs.json: the run each array comes from (the reference file,
@@ -85,6 +85,10 @@ def main(root):
         for f in ("Density", "XVelocity", "Pressure"):
             arrays[f"{key}/{f}"] = sol[f]
         meta["cases"][key] = dict(c, dim=1, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5)
+        # the same run with TENO-S (Teno5_CongZ, InterMethod WCNS5CONGZ = 8)
+        sol = cgns.read_solution(os.path.join(root, c["file"].replace("TENO.cgns", "TENOS.cgns")))
+        for f in ("Density", "XVelocity", "Pressure"):
+            arrays[f"{key}_tenos/{f}"] = sol[f]
     for key, c in INITIAL.items():
         sol = cgns.read_solution(os.path.join(root, c["file"]))
         zone, imax = zone_of(sol)

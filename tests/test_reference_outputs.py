@@ -38,11 +38,11 @@ def _prim(cons):
     return r, u, r * ((1.4 - 1) * e)
 
 
-def _run_1d(meta, output_dt):
+def _run_1d(meta, output_dt, scheme=O.S_TENO5):
     zone, imax = meta["zone"], meta["imax"]
     cons, h = cases.reference_1d(meta["case"], zone[0], zone[1], imax[0])
     bc = [2, 2] if meta["case"] == 4 else [7, 7]
-    g = O.make_grid(1, (imax[0] - 1,), h, O.S_TENO5, O.F_HLLC, 1, bc)
+    g = O.make_grid(1, (imax[0] - 1,), h, scheme, O.F_HLLC, 1, bc)
     nout = int(round(meta["t_end"] / output_dt))
     return _oracle_steps_cfl(g, cons, meta["cfl"], output_dt, nout, nthreads=1)
 
@@ -56,6 +56,22 @@ def test_oracle_reproduces_reference_end_states(key):
     assert np.array_equal(r, ARR[f"{key}/Density"]), np.abs(r - ARR[f"{key}/Density"]).max()
     assert np.array_equal(u, ARR[f"{key}/XVelocity"]), np.abs(u - ARR[f"{key}/XVelocity"]).max()
     assert np.array_equal(p, ARR[f"{key}/Pressure"]), np.abs(p - ARR[f"{key}/Pressure"]).max()
+
+
+@pytest.mark.parametrize("key", list(META["cases"]))
+def test_oracle_reproduces_reference_tenos_end_states(key):
+    """The same runs with TENO-S (TENOS.cgns): Sod, Lax and the double
+    rarefaction bit for bit; Shu-Osher within 1e-5 (4.9e-6 measured: that
+    file was not reproduced exactly by any output interval tried)."""
+    m = META["cases"][key]
+    cons, t = _run_1d(m, m["output_dt"], O.S_TENOS)
+    r, u, p = _prim(cons)
+    want = ARR[f"{key}_tenos/Density"]
+    if key == "shu_osher":
+        assert np.abs(r - want).max() < 1e-5
+    else:
+        for a, f in zip((r, u, p), ("Density", "XVelocity", "Pressure")):
+            assert np.array_equal(a, ARR[f"{key}_tenos/{f}"]), (f, np.abs(a - ARR[f"{key}_tenos/{f}"]).max())
 
 
 def test_oracle_output_dt_matters():
