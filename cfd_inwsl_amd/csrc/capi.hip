@@ -118,6 +118,31 @@ __global__ void k_zero_lambda(double* scal) {
   b[2] = 0ull;
   b[3] = 0ull;
 }
+// maxima of the per-workgroup partials written by the last sweep of stage 3
+// (SW_WAVE) into scal[1..D]; nothing for the queued steps after an output time
+template <int D> __global__ __launch_bounds__(256) void k_wave_parts(const double* __restrict__ part, long nb,
+                                                                     double* scal) {
+  if (halted(scal)) return;
+  __shared__ double red[3][256];
+  for (int d = 0; d < D; d++) {
+    double m = 0.0;
+    for (long i = threadIdx.x; i < nb; i += 256) {
+      const double o = part[d * nb + i];
+      m = m < o ? o : m;
+    }
+    red[d][threadIdx.x] = m;
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int d = 0; d < D; d++) {
+        const double o = red[d][threadIdx.x + s];
+        if (red[d][threadIdx.x] < o) red[d][threadIdx.x] = o;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < D) scal[1 + threadIdx.x] = red[threadIdx.x][0];
+}
 
 // max over cells of (sqrt(gamma p / rho) + |u_d|) / h_d per direction
 // (BlockSolver::getTimeIntervalExplicit, zone_FD/src/src/blockSolver.cpp:325-338).
@@ -132,9 +157,9 @@ __global__ __launch_bounds__(256) void k_wave_speed(const double* __restrict__ U
     double c[NV], p[NV];
     for (int v = 0; v < NV; v++) c[v] = U[v * N + i];
     cons_to_prim<D>(c, p);
-    double cs = sqrt(CFD_GAMMA * p[NV - 1] / p[0]);
+    double cs = fsq(fdv(CFD_GAMMA * p[NV - 1], p[0]));
     for (int d = 0; d < D; d++) {
-      double l = (cs + fabs(p[1 + d])) / hh[d];
+      double l = fdv(cs + fabs(p[1 + d]), hh[d]);
       m[d] = m[d] < l ? l : m[d];
     }
   }
@@ -293,10 +318,25 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
   a.x0 = h->cfg.origin;
   a.hx = h->cfg.h[0];
   a.halt = h->halt;
+  for (int d = 0; d < 3; d++) a.hw[d] = h->cfg.h[d];
+  a.wave = nullptr;
   for (int d = 0; d < 3; d++) a.grav[d] = h->cfg.gravity[d];
   if (ax == h->D - 1 && has_source(h)) a.flags |= SW_SOURCE;
   const long nb = plan_pencil_blocks(h->cfg.n, ax, tile);
   const int nseg = (h->cfg.n[ax] + seg - 1) / seg;
+  if (flags & SW_WAVE) {
+    const long need = (long)h->D * nb * nseg;
+    if (need > h->wpart_cap) {  // once per plan change; the step loops are not captured
+      if (h->wpart) (void)hipFree(h->wpart);
+      h->wpart = nullptr;
+      h->wpart_cap = 0;
+      hipError_t e = hipMalloc(&h->wpart, need * sizeof(double));
+      if (e != hipSuccess) return e;
+      h->wpart_cap = need;
+    }
+    h->wpart_blocks = nb * nseg;
+    a.wave = h->wpart;
+  }
   return launch_sweep_dispatch(h->D, ax, h->sch, h->flx, h->cw, tile, h->fp32, &a, nullptr,
                                dim3((unsigned)nb, (unsigned)nseg), h->stream);
 }
@@ -308,6 +348,7 @@ int rk_stage(cfd_handle* h, int stage, const void* Ucur, const double* U0, void*
     if (ax > 0) flags |= SW_READ_R;
     if (ax < h->D - 1 || h->keep_rhs) flags |= SW_WRITE_R;
     if (ax == h->D - 1) flags |= SW_RK;
+    if (ax == h->D - 1 && stage == 3 && h->fuse_wave) flags |= SW_WAVE;
     HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, U0, Uout, -1, -1, Ushadow));
   }
   return 0;
@@ -321,6 +362,27 @@ static int rk3(cfd_handle* h) {
   if ((rc = rk_stage(h, 1, sweep_state(h), h->U0, h->Ua, nullptr))) return rc;
   if ((rc = rk_stage(h, 2, h->Ua, h->U0, h->Ub, nullptr))) return rc;
   if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->U0, h->fp32 ? (void*)h->U0f : nullptr))) return rc;
+  return 0;
+}
+
+// maxima from the partials of the previous step's last sweep (SW_WAVE)
+static void wave_from_parts(cfd_handle* h) {
+  switch (h->D) {
+  case 1: hipLaunchKernelGGL(k_wave_parts<1>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, h->scal); break;
+  case 2: hipLaunchKernelGGL(k_wave_parts<2>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, h->scal); break;
+  default: hipLaunchKernelGGL(k_wave_parts<3>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, h->scal); break;
+  }
+}
+
+// dt from the previous step's partial maxima (fused into its last sweep)
+static int finish_dt(cfd_handle* h, double cfl, int advance) {
+  wave_from_parts(h);
+  switch (h->D) {
+  case 1: hipLaunchKernelGGL(k_finish_dt<1>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance); break;
+  case 2: hipLaunchKernelGGL(k_finish_dt<2>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance); break;
+  default: hipLaunchKernelGGL(k_finish_dt<3>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance); break;
+  }
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
@@ -443,6 +505,7 @@ int cfd_destroy(cfd_handle* h) {
   cfd_adjoint_release(h);
   (void)hipFree(h->scal);
   if (h->steplog) (void)hipFree(h->steplog);
+  if (h->wpart) (void)hipFree(h->wpart);
   (void)hipFree(h->partial);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -506,11 +569,18 @@ int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out) {
   if (!h) return cfd_fail(CFD_E_ARG, "null handle");
   if (!(cfl > 0)) return cfd_fail(CFD_E_ARG, "cfl must be > 0");
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 4, 0.0);
+  // the first step reads the CFL maxima of the state in a pass of its own; each
+  // step's last sweep accumulates those of its result for the next one
+  h->fuse_wave = true;
   for (int s = 0; s < nsteps; s++) {
-    int rc = wave_speed(h, cfl, 1);
-    if (rc) return rc;
-    if ((rc = rk3(h))) return rc;
+    int rc = s == 0 ? wave_speed(h, cfl, 1) : finish_dt(h, cfl, 1);
+    if (rc == 0) rc = rk3(h);
+    if (rc) {
+      h->fuse_wave = false;
+      return rc;
+    }
   }
+  h->fuse_wave = false;
   if (t_out) {
     HIPCHK(hipMemcpyAsync(t_out, h->scal + 4, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -570,22 +640,25 @@ int cfd_steps_to_output(cfd_handle* h, double cfl, double output_dt, double* out
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 5, 0.0);
   const int group = 8;
   const int blocks = (int)std::min<long>((h->N + 255) / 256, 2048);
+  h->fuse_wave = true;  // as in cfd_rk3_steps_cfl: maxima of each step's result in its last sweep
   while (rc == 0 && done < max_steps) {
     const int k = std::min(group, max_steps - done);
     for (int s = 0; s < k && rc == 0; s++) {
-      hipLaunchKernelGGL(k_zero_lambda, dim3(1), dim3(1), 0, h->stream, h->scal);
       const double* hh = h->cfg.h;
+      const bool first = done + s == 0;
+      if (first) hipLaunchKernelGGL(k_zero_lambda, dim3(1), dim3(1), 0, h->stream, h->scal);
+      else wave_from_parts(h);
       switch (h->D) {
       case 1:
-        hipLaunchKernelGGL(k_wave_speed<1>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
+        if (first) hipLaunchKernelGGL(k_wave_speed<1>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
         hipLaunchKernelGGL(k_clamp_dt<1>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, output_dt);
         break;
       case 2:
-        hipLaunchKernelGGL(k_wave_speed<2>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
+        if (first) hipLaunchKernelGGL(k_wave_speed<2>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
         hipLaunchKernelGGL(k_clamp_dt<2>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, output_dt);
         break;
       default:
-        hipLaunchKernelGGL(k_wave_speed<3>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
+        if (first) hipLaunchKernelGGL(k_wave_speed<3>, dim3(blocks), dim3(256), 0, h->stream, h->U0, h->N, hh[0], hh[1], hh[2], h->scal);
         hipLaunchKernelGGL(k_clamp_dt<3>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, output_dt);
         break;
       }
@@ -611,6 +684,7 @@ int cfd_steps_to_output(cfd_handle* h, double cfl, double output_dt, double* out
   }
   h->keep_rhs = keep;
   h->halt = nullptr;
+  h->fuse_wave = false;
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 10, 0.0);
   const int n = (int)flags[2];

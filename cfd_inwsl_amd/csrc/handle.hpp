@@ -28,6 +28,10 @@ struct cfd_handle {
   const double* halt;    // scal + 10 while cfd_steps_to_output queues steps, else null
   double* steplog;       // (t, dt, rhoLinf) per step of cfd_steps_to_output
   int steplog_cap;
+  bool fuse_wave;        // stage 3's last sweep writes the next step's CFL maxima per workgroup
+  double* wpart;         // those maxima: D x wpart_blocks
+  long wpart_cap;        // doubles allocated
+  long wpart_blocks;     // workgroups of the last launch that wrote them
   double* partial;       // block partial sums for totals
   int npartial;
   int keep_rhs;

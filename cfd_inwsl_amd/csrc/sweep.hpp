@@ -38,7 +38,10 @@ enum BcCode {
 };
 
 // flags of a sweep launch
-enum SweepFlags { SW_READ_R = 1, SW_WRITE_R = 2, SW_RK = 4, SW_SOURCE = 8 };
+// SW_WAVE (last sweep of stage 3): the CFL maxima of the new state are
+// written per workgroup to wave[d * nblocks + block], so the next step needs
+// no separate pass over the state (a small kernel reduces the partials)
+enum SweepFlags { SW_READ_R = 1, SW_WRITE_R = 2, SW_RK = 4, SW_SOURCE = 8, SW_WAVE = 16 };
 
 // Buffers are typed by the kernel's compute type T (double or float): U, R
 // and the stage outputs hold T; U0 (the state at the start of the step) is
@@ -69,6 +72,9 @@ struct SweepArgs {
   // when non-null and *halt != 0 the launch does nothing (queued steps after
   // an output time in cfd_steps_to_output)
   const double* halt;
+  // SW_WAVE: spacing per direction and the per-workgroup maxima (D x blocks)
+  double hw[3];
+  double* wave;
 };
 
 // Tile shapes: CA interfaces per chunk along the sweep axis x CP pencils.
@@ -218,6 +224,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
   const int m_end = c1 + 3;  // interfaces [c0, c1+3)
+  double wmax[3] = {0.0, 0.0, 0.0};  // SW_WAVE: this lane's CFL maxima
   for (int m0 = c0; m0 < m_end; m0 += CA) {
     // 1. load cells m0+1 .. m0+CA (up to c1+3)
     {
@@ -377,6 +384,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           for (int d = 0; d < D; d++) src[1 + d] = pc[0] * T(a.grav[d]);
           src[NV - 1] = pc[0] * ug;
         }
+        double unv[NV];
         for (int v = 0; v < NV; v++) {
           const T H0 = flux[fidx(v, f0, lp)], H1 = flux[fidx(v, f1, lp)];
           const T H2 = flux[fidx(v, f2, lp)], H3 = flux[fidx(v, f3, lp)];
@@ -403,11 +411,42 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
               static_cast<double*>(a.Uout)[v * a.N + idx] = un;
               static_cast<T*>(a.Ushadow)[v * a.N + idx] = (T)un;
             }
+            unv[v] = un;
+          }
+        }
+        if (a.flags & SW_WAVE) {
+          // BlockSolver::getTimeIntervalExplicit's maxima (blockSolver.cpp:325-338)
+          // of the new state, the expressions of k_wave_speed
+          double p[NV];
+          cons_to_prim<D>(unv, p);
+          const double cs = fsq(fdv(CFD_GAMMA * p[NV - 1], p[0]));
+          for (int d = 0; d < D; d++) {
+            const double l = fdv(cs + fabs(p[1 + d]), a.hw[d]);
+            wmax[d] = wmax[d] < l ? l : wmax[d];
           }
         }
       }
     }
     __syncthreads();
+  }
+  if (a.flags & SW_WAVE) {
+    // workgroup maxima: wave reductions, then the four waves through LDS
+    __shared__ double wred[4][3];
+    for (int d = 0; d < D; d++) {
+      double m = wmax[d];
+      for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = m < o ? o : m;
+      }
+      if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6][d] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < D) {
+      double m = wred[0][threadIdx.x];
+      for (int w = 1; w < 4; w++) m = m < wred[w][threadIdx.x] ? wred[w][threadIdx.x] : m;
+      const long nb = (long)gridDim.x * gridDim.y;
+      a.wave[threadIdx.x * nb + (long)blockIdx.y * gridDim.x + blockIdx.x] = m;
+    }
   }
 }
 
@@ -473,6 +512,7 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     a.Uout = b.Uout + zd.off;
     a.Ushadow = nullptr;
     a.halt = nullptr;
+    a.wave = nullptr;
     a.dtp = b.dt + z;
     a.N = zd.N;
     a.n[0] = zd.n[0];
