@@ -98,6 +98,15 @@ __host__ __device__ constexpr int tile_cp(int t) { return t == 0 ? 8 : (t == 1 ?
 // occupancy (FP32 kernels: 4 -> 3 waves; the persistent batch kernel: 2 -> 1)
 // the operands are loaded in the cell phase as before.
 template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T) == sizeof(double); };
+// Waves per SIMD the compiler must leave room for (launch bounds).  The 2D
+// FP64 Roe x sweep fits 168 VGPRs without spills and runs faster at three
+// waves (0.256 -> 0.237 ms at 2048^2, same-box A/B); its y sweep spills
+// there and loses more than it gains, and so do the HLLC and 3D kernels.
+// The three-wave kernel drops the prefetch (its cell phase has nothing to
+// prefetch: the first sweep only writes R).
+template <int D, int AX, int FLX, class T> struct SweepOcc {
+  static constexpr int waves = (D == 2 && AX == 0 && FLX == FLX_ROE && sizeof(T) == sizeof(double)) ? 3 : 1;
+};
 
 // One workgroup's share of a sweep: pencil group bx, segment by.
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T, bool PF>
@@ -451,9 +460,10 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 }
 
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
-__global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(256, (SweepOcc<D, AX, FLX, T>::waves)) void sweep_kernel(SweepArgs a) {
   if (a.halt && *a.halt != 0.0) return;
-  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value>(a, blockIdx.x, blockIdx.y);
+  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value && SweepOcc<D, AX, FLX, T>::waves == 1>(
+      a, blockIdx.x, blockIdx.y);
 }
 
 // ---- batched zones ------------------------------------------------------------
