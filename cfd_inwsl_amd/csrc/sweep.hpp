@@ -233,7 +233,11 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
   const int m_end = c1 + 3;  // interfaces [c0, c1+3)
-  double wmax[3] = {0.0, 0.0, 0.0};  // SW_WAVE: this lane's CFL maxima
+  // SW_WAVE: this lane's CFL maxima, kept in LDS (not in registers across the
+  // interface phase)
+  __shared__ double wmax[3][256];
+  if (a.flags & SW_WAVE)
+    for (int d = 0; d < D; d++) wmax[d][threadIdx.x] = 0.0;
   for (int m0 = c0; m0 < m_end; m0 += CA) {
     // 1. load cells m0+1 .. m0+CA (up to c1+3)
     {
@@ -431,7 +435,8 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           const double cs = fsq(fdv(CFD_GAMMA * p[NV - 1], p[0]));
           for (int d = 0; d < D; d++) {
             const double l = fdv(cs + fabs(p[1 + d]), a.hw[d]);
-            wmax[d] = wmax[d] < l ? l : wmax[d];
+            const double m = wmax[d][threadIdx.x];
+            wmax[d][threadIdx.x] = m < l ? l : m;
           }
         }
       }
@@ -442,7 +447,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
     // workgroup maxima: wave reductions, then the four waves through LDS
     __shared__ double wred[4][3];
     for (int d = 0; d < D; d++) {
-      double m = wmax[d];
+      double m = wmax[d][threadIdx.x];
       for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(m, off, 64);
         m = m < o ? o : m;
