@@ -276,10 +276,7 @@ static int batch_step_per_zone(cfd_batch* b, double cfl) {
         a.ct = b->ct;
         a.bc_lo = zd.bc[2 * ax];
         a.bc_hi = zd.bc[2 * ax + 1];
-        for (int v = 0; v < 5; v++) {
-          a.prim_lo[v] = zd.prim[2 * ax][v];
-          a.prim_hi[v] = zd.prim[2 * ax + 1][v];
-        }
+        a.aux = &b->dz[z].aux[ax];
         long nb = plan_pencil_blocks(zd.n, ax, b->tile[ax]);
         int nseg = (zd.n[ax] + zd.seg_len[ax] - 1) / zd.seg_len[ax];
         BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], false, &a, nullptr,
@@ -357,7 +354,10 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
         return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches take no time- or position-dependent boundaries");
       }
       zd.bc[f] = zones[z].bc[f];
-      for (int v = 0; v < 5; v++) zd.prim[f][v] = zones[z].bc_prim[f][v];
+      for (int v = 0; v < 5; v++) {
+        if (f % 2 == 0) zd.aux[f / 2].prim_lo[v] = zones[z].bc_prim[f][v];
+        else zd.aux[f / 2].prim_hi[v] = zones[z].bc_prim[f][v];
+      }
     }
     zd.N = (long)zd.n[0] * zd.n[1] * zd.n[2];
     zd.off = off;

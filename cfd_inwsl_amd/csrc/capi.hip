@@ -118,10 +118,12 @@ __global__ void k_zero_lambda(double* scal) {
   b[2] = 0ull;
   b[3] = 0ull;
 }
-// maxima of the per-workgroup partials written by the last sweep of stage 3
-// (SW_WAVE) into scal[1..D]; nothing for the queued steps after an output time
+// maxima of the per-workgroup partials of c + |u_d| written by the last sweep
+// of stage 3 (SW_WAVE), divided by h_d, into scal[1..D] -- the maxima of the
+// per-cell quotients bit for bit, the division being monotone; nothing for the
+// queued steps after an output time
 template <int D> __global__ __launch_bounds__(256) void k_wave_parts(const double* __restrict__ part, long nb,
-                                                                     double* scal) {
+                                                                     double h0, double h1, double h2, double* scal) {
   if (halted(scal)) return;
   __shared__ double red[3][256];
   for (int d = 0; d < D; d++) {
@@ -141,7 +143,10 @@ template <int D> __global__ __launch_bounds__(256) void k_wave_parts(const doubl
       }
     __syncthreads();
   }
-  if (threadIdx.x < D) scal[1 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < D) {
+    const double hh[3] = {h0, h1, h2};
+    scal[1 + threadIdx.x] = fdv(red[threadIdx.x][0], hh[threadIdx.x]);
+  }
 }
 
 // max over cells of (sqrt(gamma p / rho) + |u_d|) / h_d per direction
@@ -311,16 +316,9 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
   a.ct = h->ct;
   a.bc_lo = h->cfg.bc[2 * ax];
   a.bc_hi = h->cfg.bc[2 * ax + 1];
-  for (int v = 0; v < 5; v++) {
-    a.prim_lo[v] = h->cfg.bc_prim[2 * ax][v];
-    a.prim_hi[v] = h->cfg.bc_prim[2 * ax + 1][v];
-  }
-  a.x0 = h->cfg.origin;
-  a.hx = h->cfg.h[0];
+  a.aux = h->aux + ax;
   a.halt = h->halt;
-  for (int d = 0; d < 3; d++) a.hw[d] = h->cfg.h[d];
   a.wave = nullptr;
-  for (int d = 0; d < 3; d++) a.grav[d] = h->cfg.gravity[d];
   if (ax == h->D - 1 && has_source(h)) a.flags |= SW_SOURCE;
   const long nb = plan_pencil_blocks(h->cfg.n, ax, tile);
   const int nseg = (h->cfg.n[ax] + seg - 1) / seg;
@@ -367,10 +365,11 @@ static int rk3(cfd_handle* h) {
 
 // maxima from the partials of the previous step's last sweep (SW_WAVE)
 static void wave_from_parts(cfd_handle* h) {
+  const double* hh = h->cfg.h;
   switch (h->D) {
-  case 1: hipLaunchKernelGGL(k_wave_parts<1>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, h->scal); break;
-  case 2: hipLaunchKernelGGL(k_wave_parts<2>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, h->scal); break;
-  default: hipLaunchKernelGGL(k_wave_parts<3>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, h->scal); break;
+  case 1: hipLaunchKernelGGL(k_wave_parts<1>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
+  case 2: hipLaunchKernelGGL(k_wave_parts<2>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
+  default: hipLaunchKernelGGL(k_wave_parts<3>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
   }
 }
 
@@ -484,6 +483,21 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   if (e == hipSuccess) e = hipMalloc(&h->partial, sizeof(double) * 2 * 5 * h->npartial);
   if (e == hipSuccess) e = hipStreamCreate(&h->stream);
   if (e == hipSuccess) e = hipMemsetAsync(h->scal, 0, 16 * sizeof(double), h->stream);
+  if (e == hipSuccess) {
+    SweepAux ha[3];
+    std::memset(ha, 0, sizeof ha);
+    for (int ax = 0; ax < 3; ax++) {
+      for (int v = 0; v < 5; v++) {
+        ha[ax].prim_lo[v] = cfg->bc_prim[2 * ax][v];
+        ha[ax].prim_hi[v] = cfg->bc_prim[2 * ax + 1][v];
+      }
+      ha[ax].x0 = cfg->origin;
+      ha[ax].hx = cfg->h[0];
+      for (int d = 0; d < 3; d++) ha[ax].grav[d] = cfg->gravity[d];
+    }
+    e = hipMalloc(&h->aux, sizeof ha);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->aux, ha, sizeof ha, hipMemcpyHostToDevice, h->stream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
     std::string m = std::string("allocation: ") + hipGetErrorString(e);
@@ -506,6 +520,7 @@ int cfd_destroy(cfd_handle* h) {
   (void)hipFree(h->scal);
   if (h->steplog) (void)hipFree(h->steplog);
   if (h->wpart) (void)hipFree(h->wpart);
+  if (h->aux) (void)hipFree(h->aux);
   (void)hipFree(h->partial);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

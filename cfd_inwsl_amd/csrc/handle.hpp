@@ -7,6 +7,9 @@
 #include "../../include/congfd.h"
 
 struct AdjointStore;
+namespace cfd {
+struct SweepAux;
+}
 
 struct cfd_handle {
   cfd_config cfg;
@@ -28,6 +31,7 @@ struct cfd_handle {
   const double* halt;    // scal + 10 while cfd_steps_to_output queues steps, else null
   double* steplog;       // (t, dt, rhoLinf) per step of cfd_steps_to_output
   int steplog_cap;
+  cfd::SweepAux* aux;    // device: per direction, the sweeps' rarely used arguments
   bool fuse_wave;        // stage 3's last sweep writes the next step's CFL maxima per workgroup
   double* wpart;         // those maxima: D x wpart_blocks
   long wpart_cap;        // doubles allocated

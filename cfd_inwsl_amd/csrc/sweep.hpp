@@ -47,6 +47,18 @@ enum SweepFlags { SW_READ_R = 1, SW_WRITE_R = 2, SW_RK = 4, SW_SOURCE = 8, SW_WA
 // and the stage outputs hold T; U0 (the state at the start of the step) is
 // always FP64.  With T = float the stage-3 output is the FP64 state U0 plus
 // its FP32 shadow `Ushadow` read by the next step's sweeps.
+// Arguments used only at boundaries, by sources or by the fused CFL maxima,
+// kept in device memory behind one pointer so that they do not occupy scalar
+// registers for the whole kernel (the FP64 sweeps are at the SGPR limit).
+struct SweepAux {
+  double prim_lo[5], prim_hi[5];  // Dirichlet states of the two faces (primitive)
+  // position- and time-dependent boundaries (2D y sweeps): cell centres at
+  // x0 + (i + 0.5) hx in every direction (Initializer::initUniformBlock,
+  // initializer.cpp:672-677)
+  double x0, hx;
+  double grav[3];  // SW_SOURCE: gravity (SourceTerm::calGravitySource)
+};
+
 struct SweepArgs {
   const void* U;     // state whose RHS is evaluated (SoA planes of N), T
   void* R;           // RHS accumulator (SoA), T
@@ -62,18 +74,11 @@ struct SweepArgs {
   double h;          // spacing along AX
   double ct;         // TENO cut-off constant
   int bc_lo, bc_hi;
-  double prim_lo[5], prim_hi[5];
-  // position- and time-dependent boundaries (2D y sweeps): cell centres at
-  // x0 + (i + 0.5) hx in every direction (Initializer::initUniformBlock,
-  // initializer.cpp:672-677); time from dtp: [6] now, [7] start of the step
-  double x0, hx;
-  // SW_SOURCE: gravity source on the last sweep (SourceTerm::calGravitySource)
-  double grav[3];
+  const SweepAux* aux;  // boundary states, origin, gravity, spacings (time: dtp[6], dtp[7])
   // when non-null and *halt != 0 the launch does nothing (queued steps after
   // an output time in cfd_steps_to_output)
   const double* halt;
-  // SW_WAVE: spacing per direction and the per-workgroup maxima (D x blocks)
-  double hw[3];
+  // SW_WAVE: the per-workgroup maxima of c + |u_d| (D x blocks)
   double* wave;
 };
 
@@ -162,13 +167,14 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       int bc = lo ? a.bc_lo : a.bc_hi;
       if (D == 2 && AX == 1 && (bc == BC_DMR_WALL || bc == BC_DMR_TOP)) {
         // the pencil's cell-centre x (2D y sweep: pencil index = column)
-        const double x = a.x0 + ((int)gp + 0.5) * a.hx;
+        const double x0 = a.aux->x0, hx = a.aux->hx;
+        const double x = x0 + ((int)gp + 0.5) * hx;
         if (bc == BC_DMR_WALL) {
           bc = x < 1.0 / 6.0 ? BC_DIRICHLET : BC_SYMMETRYY;
         } else {
           // ghost k = g - n at y = -dh/2 + k dh from the face, time of the stage
           // (RK3_SSP: t, t + dt, t + dt - dt/2; blockSolver.cpp:62,74)
-          const double dh = (a.x0 + ((n - 1) + 0.5) * a.hx) - (a.x0 + ((n - 2) + 0.5) * a.hx);
+          const double dh = (x0 + ((n - 1) + 0.5) * hx) - (x0 + ((n - 2) + 0.5) * hx);
           const double y = -dh / 2 + (g - n) * dh;
           double t = a.dtp[6];
           if (a.stage >= 1) {
@@ -179,7 +185,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           const double gt = 1.0 / 6.0 + sqrt(3.0) / 3.0 * (1.0 + 20 * t);
           const bool post = y > sqrt(3.0) * (x - gt);
           for (int v = 0; v < NV; v++)
-            pr[v] = T(post ? a.prim_hi[v] : (v == 0 ? 1.4 : (v == NV - 1 ? 1.0 : 0.0)));
+            pr[v] = T(post ? a.aux->prim_hi[v] : (v == 0 ? 1.4 : (v == NV - 1 ? 1.0 : 0.0)));
           dir = true;
           exact = true;
         }
@@ -195,7 +201,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       default: src = lo ? -1 - g : 2 * n - 1 - g; break;  // mirror copy
       }
       if (dir && !exact)
-        for (int v = 0; v < NV; v++) pr[v] = T(lo ? a.prim_lo[v] : a.prim_hi[v]);
+        for (int v = 0; v < NV; v++) pr[v] = T(lo ? a.aux->prim_lo[v] : a.aux->prim_hi[v]);
     }
     raw_prim = dir;
     if (!dir) {
@@ -391,10 +397,11 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           const int si = (i + 8 * RC) % RC;
           T pc[NV];
           for (int v = 0; v < NV; v++) pc[v] = cell[cidx(v, si, lp)];
-          T ug = pc[1] * T(a.grav[0]);
-          for (int d = 1; d < D; d++) ug = ug + pc[1 + d] * T(a.grav[d]);
+          const double* grav = a.aux->grav;
+          T ug = pc[1] * T(grav[0]);
+          for (int d = 1; d < D; d++) ug = ug + pc[1 + d] * T(grav[d]);
           src[0] = T(0.0);
-          for (int d = 0; d < D; d++) src[1 + d] = pc[0] * T(a.grav[d]);
+          for (int d = 0; d < D; d++) src[1 + d] = pc[0] * T(grav[d]);
           src[NV - 1] = pc[0] * ug;
         }
         double unv[NV];
@@ -432,9 +439,12 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           // of the new state, the expressions of k_wave_speed
           double p[NV];
           cons_to_prim<D>(unv, p);
+          // max_i (c + |u_d|)_i / h_d = (max_i (c + |u_d|)_i) / h_d exactly (a
+          // correctly rounded division by a positive constant is monotone), so
+          // the division is left to the reduction of the partials
           const double cs = fsq(fdv(CFD_GAMMA * p[NV - 1], p[0]));
           for (int d = 0; d < D; d++) {
-            const double l = fdv(cs + fabs(p[1 + d]), a.hw[d]);
+            const double l = cs + fabs(p[1 + d]);
             const double m = wmax[d][threadIdx.x];
             wmax[d][threadIdx.x] = m < l ? l : m;
           }
@@ -482,7 +492,7 @@ struct ZoneDesc {
   int seg_len[3];    // segment length per direction
   double h[3];
   int bc[6];
-  double prim[6][5];
+  SweepAux aux[3];   // per direction: the faces' Dirichlet states (origin, gravity, spacings zero)
 };
 
 struct WorkItem {
@@ -540,13 +550,7 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     a.ct = b.ct;
     a.bc_lo = zd.bc[2 * AX];
     a.bc_hi = zd.bc[2 * AX + 1];
-    for (int v = 0; v < 5; v++) {
-      a.prim_lo[v] = zd.prim[2 * AX][v];
-      a.prim_hi[v] = zd.prim[2 * AX + 1][v];
-    }
-    a.x0 = 0.0;  // batches take neither position-dependent boundaries nor sources
-    a.hx = 0.0;
-    a.grav[0] = a.grav[1] = a.grav[2] = 0.0;
+    a.aux = &zd.aux[AX];  // batches take neither position-dependent boundaries nor sources
     sweep_block<D, AX, SCH, FLX, CHARW, TILE, double, false>(a, __builtin_amdgcn_readfirstlane(w.bx),
                                               __builtin_amdgcn_readfirstlane(w.by));
   }
