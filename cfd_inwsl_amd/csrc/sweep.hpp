@@ -103,14 +103,22 @@ __host__ __device__ constexpr int tile_cp(int t) { return t == 0 ? 8 : (t == 1 ?
 // occupancy (FP32 kernels: 4 -> 3 waves; the persistent batch kernel: 2 -> 1)
 // the operands are loaded in the cell phase as before.
 template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T) == sizeof(double); };
-// Waves per SIMD the compiler must leave room for (launch bounds).  The 2D
-// FP64 Roe x sweep fits 168 VGPRs without spills and runs faster at three
-// waves (0.256 -> 0.237 ms at 2048^2, same-box A/B); its y sweep spills
-// there and loses more than it gains, and so do the HLLC and 3D kernels.
-// The three-wave kernel drops the prefetch (its cell phase has nothing to
-// prefetch: the first sweep only writes R).
-template <int D, int AX, int FLX, class T> struct SweepOcc {
-  static constexpr int waves = (D == 2 && AX == 0 && FLX == FLX_ROE && sizeof(T) == sizeof(double)) ? 3 : 1;
+// Waves per SIMD the compiler must leave room for (launch bounds), from
+// same-box A/B runs:
+// * the 2D FP64 Roe x sweep fits 168 VGPRs without spills and runs faster at
+//   three waves (0.256 -> 0.237 ms at 2048^2); its y sweep spills there and
+//   loses more than it gains, and so do the HLLC kernels.  The three-wave
+//   kernel drops the prefetch (the first sweep has nothing to prefetch);
+// * the 3D FP64 WENO-JS / WENO-Z / TENO-A kernels would otherwise take more
+//   than 256 registers and run one wave per SIMD; bounded to two they spill
+//   a few registers and run 17-41 % faster at 256^3 (without the prefetch,
+//   which bounded kernels do not take).  TENO5 and TENO-S already fit two
+//   waves and lose 12-15 % when bounded.
+template <int D, int AX, int SCH, int FLX, class T> struct SweepOcc {
+  static constexpr bool dbl = sizeof(T) == sizeof(double);
+  static constexpr int waves = (D == 2 && AX == 0 && FLX == FLX_ROE && dbl)                               ? 3
+                               : (D == 3 && dbl && (SCH == SCH_JS || SCH == SCH_Z || SCH == SCH_TENOA)) ? 2
+                                                                                                          : 1;
 };
 
 // One workgroup's share of a sweep: pencil group bx, segment by.
@@ -475,9 +483,9 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 }
 
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
-__global__ __launch_bounds__(256, (SweepOcc<D, AX, FLX, T>::waves)) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(256, (SweepOcc<D, AX, SCH, FLX, T>::waves)) void sweep_kernel(SweepArgs a) {
   if (a.halt && *a.halt != 0.0) return;
-  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value && SweepOcc<D, AX, FLX, T>::waves == 1>(
+  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value && SweepOcc<D, AX, SCH, FLX, T>::waves == 1>(
       a, blockIdx.x, blockIdx.y);
 }
 
