@@ -181,31 +181,44 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
   const double ih = 1.0 / a.h;
 
-  auto load = [&](int c) {
+  // Global loads of one ring cell (interior: state and cotangent; ghost: the
+  // state of its source cell), issued one chunk before the cell enters the
+  // ring so that their latency hides behind a chunk of interface work.
+  struct Fetch {
+    double u[4], m[4];
+  };
+  auto fetch = [&](int c, Fetch& f) {
+    const bool in = c >= 0 && c < n;
+    const cfdadj::GhostSrc gs = cfdadj::ghost_source(c < 0 ? a.bc_lo : a.bc_hi, c, n);
+    const long i = base + (long)(in ? c : gs.fixed ? 0 : gs.src) * sa;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      f.u[v] = a.U[v * a.N + i];
+      f.m[v] = in ? a.MU[v * a.N + i] : 0.0;
+    }
+  };
+  // primitive state, coef * cotangent and a zero accumulator into the ring
+  auto commit = [&](int c, const Fetch& f) {
     const int s = slot_of(c);
     double p[4], m[4] = {0.0, 0.0, 0.0, 0.0};
     if (c >= 0 && c < n) {
-      const long i = base + (long)c * sa;
-      double u[4];
-      for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
-      cfd::cons_to_prim<2>(u, p);
-      for (int v = 0; v < 4; v++) m[v] = a.coef * a.MU[v * a.N + i];
+      cfd::cons_to_prim<2>(f.u, p);
+#pragma unroll
+      for (int v = 0; v < 4; v++) m[v] = a.coef * f.m[v];
     } else {
-      const bool lo = c < 0;
-      const cfdadj::GhostSrc gs = cfdadj::ghost_source(lo ? a.bc_lo : a.bc_hi, c, n);
+      const cfdadj::GhostSrc gs = cfdadj::ghost_source(c < 0 ? a.bc_lo : a.bc_hi, c, n);
       if (gs.fixed) {
-        for (int v = 0; v < 4; v++) p[v] = a.bc_prim[(lo ? 0 : 4) + v];
+#pragma unroll
+        for (int v = 0; v < 4; v++) p[v] = a.bc_prim[(c < 0 ? 0 : 4) + v];
       } else {
-        const long i = base + (long)gs.src * sa;
-        double u[4];
-        for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
-        cfd::cons_to_prim<2>(u, p);
-        for (int v = 1; v < 3; v++)
-          if (v == gs.flip) p[v] = -p[v];
+        cfd::cons_to_prim<2>(f.u, p);
+        p[1] = gs.flip == 1 ? -p[1] : p[1];  // selects, not an index into p
+        p[2] = gs.flip == 2 ? -p[2] : p[2];
       }
     }
     double cc[4];
     cfd::prim_to_cons<2>(p, cc);
+#pragma unroll
     for (int v = 0; v < 4; v++) {
       wc[idx(v, s, lp)] = p[v];
       wc[idx(4 + v, s, lp)] = cc[v];
@@ -230,12 +243,38 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     __device__ double cons(int j, int v) const { return at(j, 4 + v); }
   };
 
-  if (pvalid && la < 5) load(kmin - 3 + la);
+  if (pvalid && la < 5) {
+    Fetch f;
+    fetch(kmin - 3 + la, f);
+    commit(kmin - 3 + la, f);
+  }
+  Fetch nf;
+  if (pvalid && kmin + 2 + la <= kmax + 2) fetch(kmin + 2 + la, nf);
   for (int k0 = kmin; k0 - 3 < e1; k0 += CA) {
-    {
-      const int c = k0 + 2 + la;
-      if (pvalid && c <= kmax + 2) load(c);
+    if (pvalid && k0 + 2 + la <= kmax + 2) commit(k0 + 2 + la, nf);
+    // this chunk's finalised cell: its state and base terms, loaded now
+    const int cf = k0 - 3 + la;
+    const bool fin = pvalid && cf >= e0 && cf < e1;
+    const bool fin_in = fin && cf >= 0 && cf < n;
+    const long fi = base + (long)cf * sa;
+    double fu[4], fx[3][4];
+    if (fin_in) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) fu[v] = a.U[v * a.N + fi];
+      if (a.use_init) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+          const double* X = a.init.X[t];
+#pragma unroll
+          for (int v = 0; v < 4; v++) fx[t][v] = X ? X[v * a.N + fi] : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++) fx[0][v] = a.OUT[v * a.N + fi];
+      }
     }
+    // the next chunk's new ring cell
+    if (pvalid && k0 + CA - 3 < e1 && k0 + CA + 2 + la <= kmax + 2) fetch(k0 + CA + 2 + la, nf);
     __syncthreads();
     const int k = k0 + la;
     const bool act = pvalid && k <= kmax;
@@ -257,54 +296,50 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       }
       __syncthreads();
     }
-    {
-      const int c = k0 - 3 + la;
-      if (pvalid && c >= e0 && c < e1) {
-        const int s = slot_of(c);
-        double lwc[4];
-        for (int v = 0; v < 4; v++) lwc[v] = ac[idx(v, s, lp)];
-        if (c >= -1 && c <= n) {  // node fluxes exist on cells -1 .. n
-          double p[4], lf[4];
-          for (int v = 0; v < 4; v++) {
-            p[v] = wc[idx(v, s, lp)];
-            lf[v] = w1 * (M(c - 1, v) - M(c + 1, v)) * ih;
-          }
-          cfdadj::node_flux_vjp<AX>(p, lf, lwc);
+    if (fin) {
+      const int c = cf;
+      const int s = slot_of(c);
+      double lwc[4];
+      for (int v = 0; v < 4; v++) lwc[v] = ac[idx(v, s, lp)];
+      if (c >= -1 && c <= n) {  // node fluxes exist on cells -1 .. n
+        double p[4], lf[4];
+        for (int v = 0; v < 4; v++) {
+          p[v] = wc[idx(v, s, lp)];
+          lf[v] = w1 * (M(c - 1, v) - M(c + 1, v)) * ih;
         }
-        if (AX == 1 && c >= 0 && c < n && (gp < 4 || gp >= nx - 4)) {
-          // x ghosts of row c whose source is this column (sources lie within
-          // four cells of the row ends)
-          const unsigned col = gp < 4 ? (unsigned)gp : (unsigned)(gp - nx + 8);
-          for (int q = 0; q < 8; q++)
-            if (((a.gx_src >> (4 * q)) & 15u) == col) {
-              const int flip = (int)((a.gx_flip >> (2 * q)) & 3u);
-              for (int v = 0; v < 4; v++) {
-                const double x = a.GHX[((long)c * 8 + q) * 4 + v];
-                lwc[v] += (flip != 0 && v == flip) ? -x : x;
-              }
+        cfdadj::node_flux_vjp<AX>(p, lf, lwc);
+      }
+      if (AX == 1 && c >= 0 && c < n && (gp < 4 || gp >= nx - 4)) {
+        // x ghosts of row c whose source is this column (sources lie within
+        // four cells of the row ends)
+        const unsigned col = gp < 4 ? (unsigned)gp : (unsigned)(gp - nx + 8);
+        for (int q = 0; q < 8; q++)
+          if (((a.gx_src >> (4 * q)) & 15u) == col) {
+            const int flip = (int)((a.gx_flip >> (2 * q)) & 3u);
+            for (int v = 0; v < 4; v++) {
+              const double x = a.GHX[((long)c * 8 + q) * 4 + v];
+              lwc[v] += (flip != 0 && v == flip) ? -x : x;
             }
-        }
-        if (c >= 0 && c < n) {
-          const long i = base + (long)c * sa;
-          double u[4], g[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
-          cfdadj::c2p_vjp(u, lwc, g);
-          for (int v = 0; v < 4; v++) {
-            const long e = v * a.N + i;
-            double base;
-            if (a.use_init) {
-              base = 0.0;
-              for (int t = 0; t < 3; t++)
-                if (a.init.X[t]) base += a.init.b[t] * a.init.X[t][e];
-            } else {
-              base = a.OUT[e];
-            }
-            a.OUT[e] = base + g[v];
           }
-        } else {
-          const int gs = c < 0 ? c + 4 : c - n + 4;  // 0..7
-          for (int v = 0; v < 4; v++) a.GH[(gp * 8 + gs) * 4 + v] = lwc[v];
+      }
+      if (fin_in) {
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        cfdadj::c2p_vjp(fu, lwc, g);
+        for (int v = 0; v < 4; v++) {
+          double b;
+          if (a.use_init) {
+            b = 0.0;
+#pragma unroll
+            for (int t = 0; t < 3; t++)
+              if (a.init.X[t]) b += a.init.b[t] * fx[t][v];
+          } else {
+            b = fx[0][v];
+          }
+          a.OUT[v * a.N + fi] = b + g[v];
         }
+      } else {
+        const int gs = c < 0 ? c + 4 : c - n + 4;  // 0..7
+        for (int v = 0; v < 4; v++) a.GH[(gp * 8 + gs) * 4 + v] = lwc[v];
       }
     }
     __syncthreads();
