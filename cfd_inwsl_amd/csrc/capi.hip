@@ -47,6 +47,9 @@ int cfd_select_scheme(int inter, int dim, int* sch, double* ct) {
   case 22: *sch = SCH_TENO; return 0;              // congTCNS5CT5 (congTcns5_ZCT5: Teno5_Z's arithmetic)
   case 23: *sch = SCH_TENO; *ct = 1e-10; return 0; // congTCNS5CT10
   case 36: *sch = SCH_TENOA; return 0;             // new_WHFTCNSA, "TENO-A" (new_whf_TCNS_A)
+  case 33: *sch = SCH_TENOSA; *ct = 1.0; return 0;  // whfzycTCNSNmyASF002_1 (whf_zyc_TcnsN_myASF002_1)
+  case 38: *sch = SCH_TENOSA; *ct = 2.0; return 0;  // new_WHFTCNSASF203_NoS (whf_TCNS_AS_myF203_NoS)
+  case 39: *sch = SCH_TENOSA; *ct = 3.0; return 0;  // new_WHFTCNSASF202_NoS (whf_TCNS_AS_myF202_NoS)
   case 8: case 37: *sch = SCH_TENOS; *ct = 0.15704178024750198; return 0;  // WCNS5CONGZ, new_WCNS5CONGZ
   case 9: *sch = SCH_TENOS; *ct = 0.23050581003334941; return 0;           // WCNS5CONGZCT4
   case 10:  // WCNS5CONGZCT7: Teno5_CongZCT7 in 2D (solverType.cpp:435-437); the 1D

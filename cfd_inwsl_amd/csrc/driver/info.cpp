@@ -27,13 +27,19 @@ std::string Info::caseName() const {
   return it == m.end() ? std::string("unknown") : it->second;
 }
 
-// disStr (zone_FD/src/src/info.cpp:155-198), the implemented entries.
+// disStr (zone_FD/src/src/info.cpp:155-198): the name each InterMethod value
+// gives the output files.
 std::string Info::schemeName() const {
   static const std::map<int, std::string> s = {
       {0, "FIRSTORDER"}, {1, "MUSCL"}, {2, "WCNS5"}, {3, "WCNSZ5"}, {4, "WCNS5Char"}, {5, "WCNSZ5Char"},
       {6, "WCNS5CONG"}, {7, "TCNSCongA"}, {8, "WCNS5CONGZ"}, {9, "WCNS5CONGZCT4"}, {10, "WCNS5CONGZCT7"},
       {11, "TCNS5"}, {12, "TCNS5CT4"}, {13, "TCNS5CT7"}, {14, "LINEAR5"}, {15, "MUCSLIN5"}, {16, "INTERMAX"},
-      {35, "TENO"}, {37, "TENO-S"}};
+      {17, "whfTCNSN"}, {18, "whfTCNSNA"}, {19, "whfTCNSNAS"}, {20, "whfTCNSNS"}, {21, "whfTCNSNLAD"},
+      {22, "congTCNS5CT5"}, {23, "congTCNS5CT10"}, {24, "whfAITCNSNS"}, {25, "whfAITCNSNA"},
+      {26, "whfAITCNSNAS_1"}, {27, "whfAITCNSNAS_2"}, {28, "whfAITCNSNLADS"}, {29, "whfAITCNSNAZS"},
+      {30, "whfAITCNSNmyASF002_1"}, {31, "whfAITCNSNmyASF002_2"}, {32, "whfAITCNSNmyASF002_ai1"},
+      {33, "whfzycTCNSNmyASF002_1"}, {34, "whfCOMPARE"}, {35, "TENO"}, {36, "TENO-A"}, {37, "TENO-S"},
+      {38, "TENO-AS-myF203_NoS"}, {39, "TENO-AS-myF202_NoS"}};
   auto it = s.find(interMethod);
   return it == s.end() ? std::to_string(interMethod) : it->second;
 }

@@ -17,7 +17,7 @@ namespace cfd {
 // zone_FD/src/include/macro.hpp:28
 #define CFD_GAMMA 1.4
 
-enum SchemeId { SCH_JS = 0, SCH_Z = 1, SCH_TENO = 2, SCH_TENOS = 3, SCH_TENOA = 4 };
+enum SchemeId { SCH_JS = 0, SCH_Z = 1, SCH_TENO = 2, SCH_TENOS = 3, SCH_TENOA = 4, SCH_TENOSA = 5 };
 enum FluxId { FLX_HLLC = 0, FLX_ROE = 1 };
 
 template <class T> __device__ __forceinline__ T sq(T a) { return a * a; }
@@ -202,6 +202,40 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
                 : betaA == 8 ? T(1e-8) : betaA == 9 ? T(1e-9) : T(1e-10);
     T rr = cta * (g0 + g1 + g2);
     unsigned flag = (g0 < rr ? 1u : 0u) + (g1 < rr ? 2u : 0u) + (g2 < rr ? 4u : 0u);
+    return teno_pick(flag, q0, q1, q2, q3, q4);
+  } else if (SCH == SCH_TENOSA) {
+    // TENO-S with its constant C_T adapted to the smoothness ratios eta of
+    // adjacent differences (those of TENO-A); ct selects the law:
+    //   1: whf_zyc_TcnsN_myASF002_1 (interScheme.hpp:3363-3460),
+    //      s = min(1, 4.1666667 eta_min), C_T = 0.157 / (11.351092 s^2 - 5.740650 s + 1.506631)
+    //   2: whf_TCNS_AS_myF203_NoS (:3984-4091), m = 1 - min(0.24, eta_min) / 0.24,
+    //      C_T = -0.452662 m^3 + 0.590811 m^2 - 0.012189 m + 0.022010
+    //   3: whf_TCNS_AS_myF202_NoS (:4093-4200), the same m, C_T = -0.088182 m^2 + 0.259395 m - 0.000616
+    const T bm = smin(smin(b0, b1), b2);
+    const T epsA = T(2.7551 * 1e-7);
+    const T d0 = q0 - q1, d1 = q1 - q2, d2 = q2 - q3, d3 = q3 - q4;
+    const T e0 = fdv(fabs(T(2.0) * d0 * d1) + epsA, d0 * d0 + d1 * d1 + epsA);
+    const T e1 = fdv(fabs(T(2.0) * d1 * d2) + epsA, d1 * d1 + d2 * d2 + epsA);
+    const T e2 = fdv(fabs(T(2.0) * d2 * d3) + epsA, d2 * d2 + d3 * d3 + epsA);
+    const T emin = smin(smin(e0, e1), e2);
+    T CT;
+    if (ct < T(1.5)) {
+      const T x = smin(T(1.0), T(4.1666667) * emin);
+      CT = fdv(T(0.157), T(11.351092) * x * x - T(5.740650) * x + T(1.506631));
+    } else {
+      const T m = T(1.0) - fdv(smin(T(0.24), emin), T(0.24));
+      if (ct < T(2.5)) {
+        const T mm = m * m, mmm = m * mm;
+        CT = T(-0.452662) * mmm + T(0.590811) * mm - T(0.012189) * m + T(0.022010);
+      } else {
+        CT = T(-0.088182) * m * m + T(0.259395) * m - T(0.000616);
+      }
+    }
+    const T CT_1 = 1 - CT;
+    T tau = fabs(b2 - b0);
+    T rr = CT * tau - CT_1 * bm;
+    T ll = tau * bm;
+    unsigned flag = (ll < rr * b0 ? 1u : 0u) + (ll < rr * b1 ? 2u : 0u) + (ll < rr * b2 ? 4u : 0u);
     return teno_pick(flag, q0, q1, q2, q3, q4);
   } else {
     // Teno5_CongZ, TENO-S (interScheme.hpp:494-585); ct = its constant CT

@@ -58,6 +58,7 @@ static hipError_t launch_sch(int sch, int flx, bool cw, int tile, bool fp32, con
   case SCH_Z: return launch_flx<D, AX, SCH_Z>(flx, cw, tile, fp32, a, b, grid, s);
   case SCH_TENO: return launch_flx<D, AX, SCH_TENO>(flx, cw, tile, fp32, a, b, grid, s);
   case SCH_TENOA: return launch_flx<D, AX, SCH_TENOA>(flx, cw, tile, fp32, a, b, grid, s);
+  case SCH_TENOSA: return launch_flx<D, AX, SCH_TENOSA>(flx, cw, tile, fp32, a, b, grid, s);
   default: return launch_flx<D, AX, SCH_TENOS>(flx, cw, tile, fp32, a, b, grid, s);
   }
 }

@@ -26,7 +26,10 @@ class InterMethod(enum.IntEnum):
     TCNS5CT7 = 13
     CONGTCNS5CT5 = 22    # congTcns5_ZCT5 (Teno5_Z's arithmetic)
     CONGTCNS5CT10 = 23   # congTcns5_ZCT10
+    WHFZYCTCNSNMYASF002_1 = 33  # whf_zyc_TcnsN_myASF002_1 (TENO-S, C_T adapted to eta_min)
     NEW_WHFTCNSA = 36    # new_whf_TCNS_A ("TENO-A")
+    NEW_WHFTCNSASF203_NOS = 38  # whf_TCNS_AS_myF203_NoS (TENO-S, cubic C_T law)
+    NEW_WHFTCNSASF202_NOS = 39  # whf_TCNS_AS_myF202_NoS (TENO-S, quadratic C_T law)
 
 
 class FluxMethod(enum.IntEnum):

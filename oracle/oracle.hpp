@@ -28,7 +28,8 @@ constexpr double GAMMA = 1.4;
 // Interpolation schemes (zone_FD/src/include/interScheme.hpp).
 enum Scheme {
   S_JS = 0, S_Z = 1, S_TENO5 = 2, S_TENO5_CT4 = 3, S_TENO5_CT7 = 4, S_TENOS = 5, S_LINEAR = 6,
-  S_TENOS_CT4 = 7, S_TENOS_CT7 = 8, S_TENO5_CT10 = 9, S_TENOA = 10
+  S_TENOS_CT4 = 7, S_TENOS_CT7 = 8, S_TENO5_CT10 = 9, S_TENOA = 10,
+  S_TENOSA_F002 = 11, S_TENOSA_F203 = 12, S_TENOSA_F202 = 13
 };
 enum Flux { F_HLLC = 0, F_ROE = 1 };
 
@@ -172,6 +173,56 @@ template <class T> inline T tenoa(const T q[5]) {
   return teno_select(flag, q);
 }
 
+// TENO-S with C_T adapted to eta_min (the smoothness ratios of TENO-A):
+//   law 1, whf_zyc_TcnsN_myASF002_1 (interScheme.hpp:3363-3460):
+//     s = std::min(1.0, 4.1666667 eta_min), C_T = 0.157/(11.351092 s s - 5.740650 s + 1.506631)
+//   law 2, whf_TCNS_AS_myF203_NoS (:3984-4091): m = 1 - std::min(0.24, eta_min)/0.24,
+//     C_T = -0.452662 m^3 + 0.590811 m^2 - 0.012189 m + 0.022010 (mm = m m, mmm = m mm)
+//   law 3, whf_TCNS_AS_myF202_NoS (:4093-4200): the same m, C_T = -0.088182 m m + 0.259395 m - 0.000616
+// then TENO-S's cut-off with that C_T.  The zyc form writes the differences
+// with the opposite sign (q_{i} - q_{i-1}); |2 a b| and a^2 + b^2 are the
+// same doubles either way.
+template <class T> inline T tenosa(const T q[5], int law) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  int mb = 0;  // std::min_element: first minimal index
+  if (b[1] < b[mb]) mb = 1;
+  if (b[2] < b[mb]) mb = 2;
+  const double epsA = 2.7551 * 1e-7;
+  const T d0 = q[0] - q[1], d1 = q[1] - q[2], d2 = q[2] - q[3], d3 = q[3] - q[4];
+  const T e0 = (abs(2.0 * d1 * d0) + epsA) / (d1 * d1 + d0 * d0 + epsA);
+  const T e1 = (abs(2.0 * d2 * d1) + epsA) / (d2 * d2 + d1 * d1 + epsA);
+  const T e2 = (abs(2.0 * d3 * d2) + epsA) / (d3 * d3 + d2 * d2 + epsA);
+  T emin = e0;  // std::min({..}): first minimal
+  if (e1 < emin) emin = e1;
+  if (e2 < emin) emin = e2;
+  T CT;
+  if (law == 1) {
+    const T x = 4.1666667 * emin;
+    const T s = x < 1.0 ? x : T(1.0);
+    CT = 0.157 / (11.351092 * s * s - 5.740650 * s + 1.506631);
+  } else {
+    const T mn = emin < 0.24 ? emin : T(0.24);
+    const T m = 1 - mn / 0.24;
+    if (law == 2) {
+      const T mm = m * m, mmm = m * mm;
+      CT = -0.452662 * mmm + 0.590811 * mm - 0.012189 * m + 0.022010;
+    } else {
+      CT = -0.088182 * m * m + 0.259395 * m - 0.000616;
+    }
+  }
+  const T CT_1 = 1 - CT;
+  T tau = abs(b[2] - b[0]);
+  T rr = CT * tau - CT_1 * b[mb];
+  T ll = tau * b[mb];
+  unsigned flag = 0;
+  if (ll < rr * b[0]) flag += 1;
+  if (ll < rr * b[1]) flag += 2;
+  if (ll < rr * b[2]) flag += 4;
+  return teno_select(flag, q);
+}
+
 // Linear5 (interScheme.hpp:1450-1453)
 template <class T> inline T linear5(const T q[5]) { return teno_select(0u, q); }
 
@@ -187,6 +238,9 @@ template <class T> inline T inter5(int s, const T q[5]) {
   case S_TENOS_CT7: return tenos(q, 0.072892337360747711);
   case S_TENO5_CT10: return teno5(q, 1e-10);
   case S_TENOA: return tenoa(q);
+  case S_TENOSA_F002: return tenosa(q, 1);
+  case S_TENOSA_F203: return tenosa(q, 2);
+  case S_TENOSA_F202: return tenosa(q, 3);
   default: return linear5(q);
   }
 }

@@ -20,6 +20,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 # oracle::Scheme
 S_JS, S_Z, S_TENO5, S_TENO5_CT4, S_TENO5_CT7, S_TENOS, S_LINEAR = range(7)
 S_TENOS_CT4, S_TENOS_CT7, S_TENO5_CT10, S_TENOA = range(7, 11)
+S_TENOSA_F002, S_TENOSA_F203, S_TENOSA_F202 = range(11, 14)
 # oracle::Flux
 F_HLLC, F_ROE = 0, 1
 

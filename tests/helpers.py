@@ -17,6 +17,9 @@ SCHEME_OF = {
     InterMethod.CONGTCNS5CT5: O.S_TENO5,
     InterMethod.CONGTCNS5CT10: O.S_TENO5_CT10,
     InterMethod.NEW_WHFTCNSA: O.S_TENOA,
+    InterMethod.WHFZYCTCNSNMYASF002_1: O.S_TENOSA_F002,
+    InterMethod.NEW_WHFTCNSASF203_NOS: O.S_TENOSA_F203,
+    InterMethod.NEW_WHFTCNSASF202_NOS: O.S_TENOSA_F202,
 }
 
 

@@ -191,6 +191,29 @@ def test_driver_reproduces_reference_solution_files(tmp_path, key, inter):
 
 
 @pytest.mark.gpu
+def test_driver_reproduces_the_zyc_research_run(tmp_path):
+    """'Sod - whfzycTCNSNmyASF002_1 - 101' (inter 33, TENO-S with C_T
+    following eta_min): the driver's t = 0.1 and t = 0.2 files equal the
+    reference's, bit for bit.  info.txt takes only values below INTERMAX
+    (zoneMain.cpp:261-263; the reference sets research variants in
+    zoneMain's source), so the scheme comes from --inter."""
+    meta, arr = _reference_fixtures()
+    m = meta["sod_zyc"]
+    nout = int(round(m["t_end"] / m["output_dt"]))
+    _info_txt(tmp_path / "info.txt", m["inter"], nout, m["output_dt"], m["cfl"], m["case"], m["zone"], m["imax"], 1)
+    r = _run(["--flux", "hllc", "--inter", str(m["inter"])], tmp_path)
+    print(r.stdout[-200:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    for t in ("0.1000", "0.2000"):
+        files = sorted(tmp_path.glob(f"*t={t}.cgns"))
+        assert len(files) == 1, files
+        sol = cgns.read_solution(files[0])
+        for f in ("Density", "XVelocity", "Pressure"):
+            want = arr[f"sod_zyc/t{t}/{f}"]
+            assert np.array_equal(sol[f], want), (t, f, np.abs(sol[f] - want).max())
+
+
+@pytest.mark.gpu
 def test_driver_double_mach_near_reference_solution_file(tmp_path):
     """Double Mach reflection, 512 x 128 cells to t = 0.2, against the
     reference's "512, 128 - TENO t=0.2000" file.  That run's settings are not

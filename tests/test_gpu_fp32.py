@@ -94,17 +94,19 @@ def test_fp32_steps_against_fp64(dim):
     assert np.all(drift < 1e-6)
 
 
-def test_fp32_teno_a_rhs_against_oracle():
+@pytest.mark.parametrize("inter", [InterMethod.NEW_WHFTCNSA, InterMethod.NEW_WHFTCNSASF203_NOS])
+def test_fp32_teno_a_rhs_against_oracle(inter):
     """TENO-A's FP32 kernels (scaled gamma terms, the adaptive constant from
-    FP32 difference ratios) on the smooth vortex against the FP64 oracle."""
+    FP32 difference ratios) and those of TENO-S with an adaptive C_T on the
+    smooth vortex against the FP64 oracle."""
     cons, h = cases.vortex(128)
     bc = [BC.PERIODIC] * 4
-    s = Solver(2, (128, 128), h, InterMethod.NEW_WHFTCNSA, FluxMethod.HLLC, Recon.CHAR, bc, precision=PRECISION_FP32)
+    s = Solver(2, (128, 128), h, inter, FluxMethod.HLLC, Recon.CHAR, bc, precision=PRECISION_FP32)
     s.set_state(cons)
     g = s.rhs()
     s.close()
-    ref, _ = O.rhs(oracle_grid(2, (128, 128), h, InterMethod.NEW_WHFTCNSA, FluxMethod.HLLC, Recon.CHAR, bc), cons)
+    ref, _ = O.rhs(oracle_grid(2, (128, 128), h, inter, FluxMethod.HLLC, Recon.CHAR, bc), cons)
     l1, linf = _errs(g, ref)
-    _say("fp32 TENO-A rhs vortex 128^2: L1", l1, "Linf", linf)
+    _say(f"fp32 {inter.name} rhs vortex 128^2: L1", l1, "Linf", linf)
     assert np.all(linf < 1e-4)
     assert np.all(l1 < 1e-5)

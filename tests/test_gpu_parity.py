@@ -42,15 +42,17 @@ COMBOS = list(itertools.product(
 
 
 RESEARCH = [InterMethod.WCNS5CONGZCT4, InterMethod.WCNS5CONGZCT7, InterMethod.CONGTCNS5CT5,
-            InterMethod.CONGTCNS5CT10, InterMethod.NEW_WHFTCNSA]
+            InterMethod.CONGTCNS5CT10, InterMethod.NEW_WHFTCNSA, InterMethod.WHFZYCTCNSNMYASF002_1,
+            InterMethod.NEW_WHFTCNSASF203_NOS, InterMethod.NEW_WHFTCNSASF202_NOS]
 
 
 @pytest.mark.parametrize("dim,inter", [(d, i) for d in (1, 2, 3) for i in RESEARCH
                                        if not (d == 1 and i == InterMethod.WCNS5CONGZCT7)])
 def test_rhs_parity_research_variants(dim, inter):
     """The research TENO variants (runtime cut-off constants of TENO5 / TENO-S,
-    and TENO-A's own kernels) against the oracle, characteristic-wise HLLC
-    and component-wise Roe, FP64."""
+    TENO-A's kernels, and the kernels of TENO-S with C_T following eta_min,
+    one runtime law each) against the oracle, characteristic-wise HLLC and
+    component-wise Roe, FP64."""
     n = SIZES[dim]
     cons = random_state(dim, n, seed=29, jumps=True)
     for flux, recon in ((FluxMethod.HLLC, Recon.CHAR), (FluxMethod.ROE, Recon.COMPONENT)):

@@ -16,7 +16,8 @@ from oracle import oracle as O
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 SCHEMES = [O.S_JS, O.S_Z, O.S_TENO5, O.S_TENO5_CT4, O.S_TENO5_CT7, O.S_TENOS, O.S_LINEAR,
-           O.S_TENOS_CT4, O.S_TENOS_CT7, O.S_TENO5_CT10, O.S_TENOA]
+           O.S_TENOS_CT4, O.S_TENOS_CT7, O.S_TENO5_CT10, O.S_TENOA, O.S_TENOSA_F002, O.S_TENOSA_F203,
+           O.S_TENOSA_F202]
 
 
 # ---------------------------------------------------------------- accuracy
@@ -82,7 +83,8 @@ def test_inter5_exact_for_quadratics(scheme):
         assert abs(O.inter5(scheme, q) - want) < 1e-12 * (1 + abs(want) + abs(c) * 10)
 
 
-@pytest.mark.parametrize("scheme", [O.S_LINEAR, O.S_TENO5, O.S_TENO5_CT10, O.S_TENOA])
+@pytest.mark.parametrize("scheme", [O.S_LINEAR, O.S_TENO5, O.S_TENO5_CT10, O.S_TENOA, O.S_TENOSA_F002,
+                                    O.S_TENOSA_F203, O.S_TENOSA_F202])
 def test_inter5_fifth_order_exact_for_quartics(scheme):
     rng = np.random.default_rng(2)
     for _ in range(20):
@@ -170,6 +172,62 @@ def test_teno_a_adapts_its_constant():
         q = rng.normal(size=5)
         assert O.inter5(O.S_TENOA, q) in {O.inter5(s, q) for s in (O.S_TENO5, O.S_TENO5_CT7, O.S_TENO5_CT10)} or \
             _cut(O.S_TENOA, q) is not None
+
+
+def _eta_min(q):
+    """min_k of the smoothness ratios (|2 a b| + eps_A) / (a^2 + b^2 + eps_A)
+    of adjacent differences a, b of the five values."""
+    d = [q[k] - q[k + 1] for k in range(4)]
+    eps = 2.7551e-7
+    return min((abs(2 * d[k] * d[k + 1]) + eps) / (d[k] ** 2 + d[k + 1] ** 2 + eps) for k in range(3))
+
+
+def test_adaptive_teno_s_laws_at_their_ends():
+    """The TENO-S variants whose C_T follows eta_min, at its two ends.
+    Resolved data (eta_min >= 0.24): the F203 / F202 laws take m = 0, i.e.
+    C_T = 0.022010 and -0.000616 (the cubic's and quadratic's constant
+    terms), the zyc law s = 1, C_T = 0.157 / 7.117073 = 0.02206 -- all below
+    TENO-S CT7's 0.0729, so they cut a subset of what CT7 cuts, and F202
+    (C_T < 0) cuts nothing.  Next to a jump (eta_min < 0.01): m > 0.958 gives
+    C_T in [0.148, 0.161] (F203) and [0.165, 0.171] (F202), s < 0.042 gives
+    C_T in [0.104, 0.122] (zyc): between CT7 and CT4 (0.2305), zyc below
+    TENO-S (0.157) and F202 above it.  TENO-S cuts more as C_T grows, so a
+    law wired with a wrong coefficient or sign breaks a chain."""
+    rng = np.random.default_rng(21)
+    seen = {"smooth": 0, "jump": 0}
+    strict = 0
+    for i in range(9000):
+        if i % 3 == 1:
+            q = rng.uniform(-1, 1) * np.sin(rng.uniform(0.05, 0.9) * np.arange(5.0) + rng.uniform(0, 6.3))
+            q = q + rng.normal(scale=rng.choice([0, 1e-4, 1e-2]), size=5)
+        elif i % 3 == 2:
+            # kinks: adjacent differences within a factor e^2 (eta >= 0.24) of any sign
+            d = np.exp(rng.uniform(-1, 1, size=4)) * rng.choice([-1.0, 1.0], size=4)
+            q = np.concatenate([[0.0], np.cumsum(d)])
+        else:
+            j = rng.integers(1, 5)
+            q = np.where(np.arange(5) >= j, rng.uniform(0.2, 2), 0.0) + rng.normal(
+                scale=rng.choice([1e-9, 1e-5, 1e-3, 1e-2]), size=5)
+        e = _eta_min(q)
+        cuts = {s: _cut(s, q) for s in (O.S_TENOS_CT7, O.S_TENOS, O.S_TENOS_CT4, O.S_TENOSA_F002,
+                                         O.S_TENOSA_F203, O.S_TENOSA_F202)}
+        if any(c is None for c in cuts.values()):
+            continue
+        if e >= 0.24:
+            seen["smooth"] += 1
+            for s in (O.S_TENOSA_F002, O.S_TENOSA_F203, O.S_TENOSA_F202):
+                assert cuts[s] <= cuts[O.S_TENOS_CT7], (s, q)
+            assert cuts[O.S_TENOSA_F202] == set(), q
+        elif e < 0.01:
+            seen["jump"] += 1
+            for s in (O.S_TENOSA_F002, O.S_TENOSA_F203, O.S_TENOSA_F202):
+                assert cuts[O.S_TENOS_CT7] <= cuts[s] <= cuts[O.S_TENOS_CT4], (s, q, cuts)
+            assert cuts[O.S_TENOSA_F002] <= cuts[O.S_TENOS] <= cuts[O.S_TENOSA_F202], (q, cuts)
+            strict += cuts[O.S_TENOSA_F002] != cuts[O.S_TENOSA_F202]
+    assert min(seen.values()) > 300 and strict > 20, (seen, strict)
+    # F202 on resolved data never cuts: the linear five-point value, exactly
+    q = np.sin(0.3 * np.arange(5.0) + 0.4)
+    assert _eta_min(q) >= 0.24 and O.inter5(O.S_TENOSA_F202, q) == O.inter5(O.S_LINEAR, q)
 
 
 def test_weno_weights_js_known_value():

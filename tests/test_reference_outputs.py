@@ -6,8 +6,10 @@ tests/golden/make_reference_outputs.py with the package's CGNS reader):
 
 * the oracle, driven through the reference's stepsLoopCFL, reproduces the
   reference's TENO5 (Teno5_Z, char-wise, HLLC, CFL 0.5) end states of Sod,
-  Shu-Osher, Lax and the double rarefaction bit for bit, and the logged
-  "Sod - congTCNS5CT5" run's dt sequence to the log's 10 printed decimals;
+  Shu-Osher, Lax and the double rarefaction bit for bit, the TENO-S ones,
+  the "Sod - whfzycTCNSNmyASF002_1" run's two outputs bit for bit, and the
+  logged "Sod - congTCNS5CT5" run's dt sequence to the log's 10 printed
+  decimals;
 * the driver's --initial-only files equal the reference's t=0 files field for
   field, bit for bit, and carry the reference's CGNS tree;
 * the CGNS writer's files read back through the reader (every HDF5 checksum
@@ -111,6 +113,28 @@ def test_oracle_follows_the_logged_sod_run():
     r, u, p = _prim(cons)
     assert np.abs(r - ARR["sod_ct5/t0.2000/Density"]).max() < 5e-12
     assert np.abs(p - ARR["sod_ct5/t0.2000/Pressure"]).max() < 5e-12
+
+
+@pytest.mark.parametrize("t_end", [0.1, 0.2])
+def test_oracle_reproduces_the_zyc_research_run(t_end):
+    """'Sod - whfzycTCNSNmyASF002_1 - 101' (whf_zyc_TcnsN_myASF002_1: TENO-S
+    whose C_T follows eta_min): the t = 0.1 and t = 0.2 files bit for bit,
+    with the output interval 0.1 of that run.  Plain TENO-S (constant C_T)
+    and the other two adaptive laws miss them by far more than rounding."""
+    m = META["sod_zyc"]
+    tag = f"sod_zyc/t{t_end:.4f}"
+    for scheme in (O.S_TENOSA_F002, O.S_TENOS, O.S_TENOSA_F203, O.S_TENOSA_F202):
+        cons, h = cases.reference_1d(0, m["zone"][0], m["zone"][1], m["imax"][0])
+        g = O.make_grid(1, (m["imax"][0] - 1,), h, scheme, O.F_HLLC, 1, [7, 7])
+        cons, t = _oracle_steps_cfl(g, cons, m["cfl"], m["output_dt"], int(round(t_end / m["output_dt"])),
+                                    nthreads=1)
+        assert abs(t - t_end) < 1e-12
+        r, u, p = _prim(cons)
+        if scheme == O.S_TENOSA_F002:
+            for a, f in zip((r, u, p), ("Density", "XVelocity", "Pressure")):
+                assert np.array_equal(a, ARR[f"{tag}/{f}"]), (f, np.abs(a - ARR[f"{tag}/{f}"]).max())
+        else:
+            assert np.abs(r - ARR[f"{tag}/Density"]).max() > 1e-6, scheme
 
 
 def _initial_only(tmp_path, dim, ncase, zone, imax, extra=()):
