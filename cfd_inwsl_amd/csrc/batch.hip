@@ -322,6 +322,8 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
     return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches take no source term");
   if (cfd_select_scheme(c->inter_method, c->dim, &sch, &ct))
     return cfd_set_error_from_batch(CFD_E_UNSUPP, "inter_method is not implemented");
+  if (sch == SCH_WHF)
+    return cfd_set_error_from_batch(CFD_E_UNSUPP, "this inter_method runs on single zones only");
   cfd_batch* b = new cfd_batch();
   b->common = *c;
   b->D = c->dim;
@@ -348,6 +350,7 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
       }
       zd.seg_len[d] = zd.n[d];
     }
+    for (int d = 0; d < 3; d++) fill_research_consts(zd.aux[d].rc);
     for (int f = 0; f < 6; f++) {
       if (zones[z].bc[f] == BC_DMR_TOP || zones[z].bc[f] == BC_DMR_WALL) {
         delete b;

@@ -48,6 +48,18 @@ int cfd_select_scheme(int inter, int dim, int* sch, double* ct) {
   case 23: *sch = SCH_TENO; *ct = 1e-10; return 0; // congTCNS5CT10
   case 36: *sch = SCH_TENOA; return 0;             // new_WHFTCNSA, "TENO-A" (new_whf_TCNS_A)
   case 33: *sch = SCH_TENOSA; *ct = 1.0; return 0;  // whfzycTCNSNmyASF002_1 (whf_zyc_TcnsN_myASF002_1)
+  case 26: case 27: *sch = SCH_TENOSA; *ct = 5.0; return 0;  // whfAITCNSNAS_1 / _2
+  case 28: *sch = SCH_TENOSA; *ct = 6.0; return 0;           // whfAITCNSNLADS (whf_ai_TcnsN_LADS_yuanbao)
+  case 29: *sch = SCH_TENOSA; *ct = 7.0; return 0;           // whfAITCNSNAZS (whf_ai_TcnsN_AZS_yuanbao)
+  case 30: case 31: case 32: *sch = SCH_TENOSA; *ct = 8.0; return 0;  // whfAITCNSNmyASF002_1 / _2 / _ai1
+  case 24:  // whfAITCNSNS (whf_ai_TcnsN_S_1): TENO-S with C_T = pow(1.5e-5 / (1 - 1e-5), 1/6)
+    *sch = SCH_TENOS;
+    *ct = std::pow(1.5 * 1e-5 / (1.0 - 1e-5), 1.0 / 6.0);
+    return 0;
+  case 17: case 18: case 19: case 20: case 21: case 25: case 34:  // whfTCNSN .. whfTCNSNLAD, whfAITCNSNA, whfCOMPARE
+    *sch = SCH_WHF;
+    *ct = inter;
+    return 0;
   case 38: *sch = SCH_TENOSA; *ct = 2.0; return 0;  // new_WHFTCNSASF203_NoS (whf_TCNS_AS_myF203_NoS)
   case 39: *sch = SCH_TENOSA; *ct = 3.0; return 0;  // new_WHFTCNSASF202_NoS (whf_TCNS_AS_myF202_NoS)
   case 8: case 37: *sch = SCH_TENOS; *ct = 0.15704178024750198; return 0;  // WCNS5CONGZ, new_WCNS5CONGZ
@@ -465,6 +477,10 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
     delete h;
     return cfd_fail(CFD_E_UNSUPP, "FP32 precision is available for dim 2 and 3");
   }
+  if (h->fp32 && sch == SCH_WHF) {
+    delete h;
+    return cfd_fail(CFD_E_UNSUPP, "inter_method " + std::to_string(cfg->inter_method) + " runs in FP64 only");
+  }
   h->esz = h->fp32 ? sizeof(float) : sizeof(double);
   for (int ax = 0; ax < 3; ax++) {
     h->tile[ax] = ax == 0 ? 0 : 1;
@@ -494,6 +510,7 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
         ha[ax].prim_lo[v] = cfg->bc_prim[2 * ax][v];
         ha[ax].prim_hi[v] = cfg->bc_prim[2 * ax + 1][v];
       }
+      fill_research_consts(ha[ax].rc);
       ha[ax].x0 = cfg->origin;
       ha[ax].hx = cfg->h[0];
       for (int d = 0; d < 3; d++) ha[ax].grav[d] = cfg->gravity[d];

@@ -12,12 +12,14 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 namespace cfd {
 
 // zone_FD/src/include/macro.hpp:28
 #define CFD_GAMMA 1.4
 
-enum SchemeId { SCH_JS = 0, SCH_Z = 1, SCH_TENO = 2, SCH_TENOS = 3, SCH_TENOA = 4, SCH_TENOSA = 5 };
+enum SchemeId { SCH_JS = 0, SCH_Z = 1, SCH_TENO = 2, SCH_TENOS = 3, SCH_TENOA = 4, SCH_TENOSA = 5, SCH_WHF = 6 };
 enum FluxId { FLX_HLLC = 0, FLX_ROE = 1 };
 
 template <class T> __device__ __forceinline__ T sq(T a) { return a * a; }
@@ -65,6 +67,74 @@ template <> __device__ __forceinline__ double fsq<double>(double x) {
   return x == 0.0 ? x : g;
 }
 
+// Constants of the research TENO variants, from the host's libm pow as the
+// reference computes them: ctp[k-5] = pow(1.5 c / (1 - c), 1/6) with
+// c = pow(10, -k), cta[k-5] = pow(10, -k) (k = 5..10), ctp_s the first formula
+// with c = 1e-5 (interScheme.hpp:1538, :1637, :2305-2306, :2598-2599).
+struct ResearchConsts {
+  double ctp[6], cta[6], ctp_s;
+};
+
+inline void fill_research_consts(ResearchConsts& x) {
+  for (int k = 5; k <= 10; k++) {
+    const double c = std::pow(10.0, -k);
+    x.cta[k - 5] = c;
+    x.ctp[k - 5] = std::pow(1.5 * c / (1.0 - c), 1.0 / 6);
+  }
+  x.ctp_s = std::pow(1.5 * 1e-5 / (1.0 - 1e-5), 1.0 / 6);
+}
+
+// ---- helpers of the research TENO variants -----------------------------------
+__device__ __forceinline__ double tpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float tpow(float x, float y) { return powf(x, y); }
+// eps_A = (0.9 C_r) / (1 - 0.9 C_r) xi^2, C_r = 0.24, xi = 1e-3 (interScheme.hpp:1620)
+constexpr double kEpsA2 = (0.9 * 0.24) / (1.0 - 0.9 * 0.24) * 1e-3 * 1e-3;
+// smoothness ratio (|2 a b| + eps) / (a^2 + b^2 + eps) of two differences
+template <class T> __device__ __forceinline__ T eta2(T a, T b, T eps) {
+  return fdv(fabs(T(2.0) * a * b) + eps, a * a + b * b + eps);
+}
+// min of the ratios of adjacent differences (q0-q1, q1-q2), (q1-q2, q2-q3), (q2-q3, q3-q4)
+template <class T> __device__ __forceinline__ T eta_adjacent(T q0, T q1, T q2, T q3, T q4, T eps) {
+  const T d0 = q0 - q1, d1 = q1 - q2, d2 = q2 - q3, d3 = q3 - q4;
+  return smin(smin(eta2(d0, d1, eps), eta2(d1, d2, eps)), eta2(d2, d3, eps));
+}
+// Integer powers the reference takes with libm pow: the products below differ
+// from it by an ulp or two, which moves a cut-off decision only where its
+// ratio ties the threshold to that precision.
+template <class T> __device__ __forceinline__ T pow4(T x) {
+  const T x2 = x * x;
+  return x2 * x2;
+}
+template <class T> __device__ __forceinline__ T pow6(T x) {
+  const T x2 = x * x;
+  return x2 * x2 * x2;
+}
+// k = floor(10 - 5 (1 - pow(1 - m, 4) (1 + 4 m))), m = 1 - min(1, eta / 0.24),
+// clamped to 5..10 (interScheme.hpp:1633-1634, :2302-2304)
+template <class T> __device__ __forceinline__ int adaptive_k(T eta) {
+  const T m = T(1.0) - smin(T(1.0), fdv(eta, T(0.24)));
+  const T g = pow4(T(1.0) - m) * (T(1.0) + T(4.0) * m);
+  const int k = (int)floor(T(10.0) - T(5.0) * (T(1.0) - g));
+  return k < 5 ? 5 : (k > 10 ? 10 : k);
+}
+// theta = 1 / (1 + max_k chi_k / 10), chi_k = tau / (beta_k + 1e-40)
+// (interScheme.hpp:1697-1701, :2541-2545)
+template <class T> __device__ __forceinline__ T lad_theta(T tau, T b0, T b1, T b2) {
+  const T c0 = fdv(tau, b0 + eps_tiny<T>()), c1 = fdv(tau, b1 + eps_tiny<T>()), c2 = fdv(tau, b2 + eps_tiny<T>());
+  return fdv(T(1.0), T(1.0) + fdv(smax(smax(c0, c1), c2), T(10.0)));
+}
+template <class T> __device__ __forceinline__ T ten_pow_neg(int k) {
+  switch (k) {
+  case 4: return T(1e-4);
+  case 5: return T(1e-5);
+  case 6: return T(1e-6);
+  case 7: return T(1e-7);
+  case 8: return T(1e-8);
+  case 9: return T(1e-9);
+  default: return T(1e-10);
+  }
+}
+
 // ---- 5-point interpolation: value at q2 + h/2 ------------------------------
 // Smoothness indicators (zone_FD/src/include/interScheme.hpp:20-27).
 template <class T>
@@ -90,7 +160,7 @@ __device__ __forceinline__ T teno_pick(unsigned flag, T q0, T q1, T q2, T q3, T 
 }
 
 template <int SCH, class T>
-__device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
+__device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const ResearchConsts* rc = nullptr) {
   T b0, b1, b2;
   beta3(q0, q1, q2, q3, q4, b0, b1, b2);
   if (SCH == SCH_JS || SCH == SCH_Z) {
@@ -211,31 +281,111 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct) {
     //   2: whf_TCNS_AS_myF203_NoS (:3984-4091), m = 1 - min(0.24, eta_min) / 0.24,
     //      C_T = -0.452662 m^3 + 0.590811 m^2 - 0.012189 m + 0.022010
     //   3: whf_TCNS_AS_myF202_NoS (:4093-4200), the same m, C_T = -0.088182 m^2 + 0.259395 m - 0.000616
+    //   5: whf_ai_TcnsN_AS_1 / _AS_2 (:2329-2511), C_T = ctp[k - 5], k = adaptive_k(eta_min), eps_A = kEpsA2
+    //   6: whf_ai_TcnsN_LADS_yuanbao (:2512-2647), C_T = ctp[m - 5], m = 5 + floor(5 theta)
+    //   7: whf_ai_TcnsN_AZS_yuanbao (:2862-2935), C_T = pow(1.5 c / (1 - c), 1/6),
+    //      c = min(1e-5, |beta0 + beta2 - 2 beta1|)
+    //   8: whf_ai_TcnsN_myASF002_1 / _2 / _ai1 (:2936-3252), law 1 with s = min(1, eta_min / 0.24), eps_A = kEpsA2
     const T bm = smin(smin(b0, b1), b2);
-    const T epsA = T(2.7551 * 1e-7);
-    const T d0 = q0 - q1, d1 = q1 - q2, d2 = q2 - q3, d3 = q3 - q4;
-    const T e0 = fdv(fabs(T(2.0) * d0 * d1) + epsA, d0 * d0 + d1 * d1 + epsA);
-    const T e1 = fdv(fabs(T(2.0) * d1 * d2) + epsA, d1 * d1 + d2 * d2 + epsA);
-    const T e2 = fdv(fabs(T(2.0) * d2 * d3) + epsA, d2 * d2 + d3 * d3 + epsA);
-    const T emin = smin(smin(e0, e1), e2);
+    T tau = fabs(b2 - b0);
+    const int law = (int)ct;
     T CT;
-    if (ct < T(1.5)) {
-      const T x = smin(T(1.0), T(4.1666667) * emin);
-      CT = fdv(T(0.157), T(11.351092) * x * x - T(5.740650) * x + T(1.506631));
-    } else {
-      const T m = T(1.0) - fdv(smin(T(0.24), emin), T(0.24));
-      if (ct < T(2.5)) {
-        const T mm = m * m, mmm = m * mm;
-        CT = T(-0.452662) * mmm + T(0.590811) * mm - T(0.012189) * m + T(0.022010);
+    if (law <= 3) {
+      const T emin = eta_adjacent(q0, q1, q2, q3, q4, T(2.7551 * 1e-7));
+      if (law == 1) {
+        const T x = smin(T(1.0), T(4.1666667) * emin);
+        CT = fdv(T(0.157), T(11.351092) * x * x - T(5.740650) * x + T(1.506631));
       } else {
-        CT = T(-0.088182) * m * m + T(0.259395) * m - T(0.000616);
+        const T m = T(1.0) - fdv(smin(T(0.24), emin), T(0.24));
+        if (law == 2) {
+          const T mm = m * m, mmm = m * mm;
+          CT = T(-0.452662) * mmm + T(0.590811) * mm - T(0.012189) * m + T(0.022010);
+        } else {
+          CT = T(-0.088182) * m * m + T(0.259395) * m - T(0.000616);
+        }
       }
+    } else if (law == 5) {
+      CT = T(rc->ctp[adaptive_k(eta_adjacent(q0, q1, q2, q3, q4, T(kEpsA2))) - 5]);
+    } else if (law == 6) {
+      int m = 5 + (int)floor(lad_theta(tau, b0, b1, b2) * T(5.0));
+      m = m < 5 ? 5 : (m > 10 ? 10 : m);
+      CT = T(rc->ctp[m - 5]);
+    } else if (law == 7) {
+      const T c = smin(T(1e-5), fabs(b0 + b2 - T(2) * b1));
+      CT = tpow(fdv(T(1.5) * c, T(1) - c), T(1.0 / 6.0));
+    } else {
+      const T x = smin(T(1.0), fdv(eta_adjacent(q0, q1, q2, q3, q4, T(kEpsA2)), T(0.24)));
+      CT = fdv(T(0.157), T(11.351092) * x * x - T(5.740650) * x + T(1.506631));
     }
     const T CT_1 = 1 - CT;
-    T tau = fabs(b2 - b0);
     T rr = CT * tau - CT_1 * bm;
     T ll = tau * bm;
     unsigned flag = (ll < rr * b0 ? 1u : 0u) + (ll < rr * b1 ? 2u : 0u) + (ll < rr * b2 ? 4u : 0u);
+    return teno_pick(flag, q0, q1, q2, q3, q4);
+  } else if (SCH == SCH_WHF) {
+    // the whf_TcnsN schemes whose cut-off is not TENO-S's; ct = the
+    // InterMethod value.  The value depends on the data through the flag
+    // only, so the libm pow of the reference (here pow4 / pow6) decides bits
+    // only where a ratio ties its threshold.
+    const int var = (int)ct;
+    T tau = fabs(b2 - b0);
+    unsigned flag;
+    if (var == 17 || var == 21 || var == 25) {
+      // whf_TcnsN (:1458-1517), whf_TcnsN_LAD (:1681-1760), whf_ai_TcnsN_A_2
+      // (:2216-2328): chi_k = gamma_k / sum gamma < C_T cuts k,
+      // gamma_k = pow(1 + tau / (beta_k + 1e-40), 6)
+      const T g0 = pow6(T(1.0) + fdv(tau, b0 + eps_tiny<T>()));
+      const T g1 = pow6(T(1.0) + fdv(tau, b1 + eps_tiny<T>()));
+      const T g2 = pow6(T(1.0) + fdv(tau, b2 + eps_tiny<T>()));
+      const T gs = g0 + g1 + g2;
+      T CT;
+      if (var == 17) {
+        CT = T(1e-10);
+      } else if (var == 21) {  // C_T = 10^-m, m = clamp(4 + floor(6 theta), 4, 10)
+        int m = 4 + (int)floor(lad_theta(tau, b0, b1, b2) * T(6.0));
+        CT = ten_pow_neg<T>(m < 4 ? 4 : (m > 10 ? 10 : m));
+      } else {  // C_TA = pow(10, -k)
+        CT = T(rc->cta[adaptive_k(eta_adjacent(q0, q1, q2, q3, q4, T(kEpsA2))) - 5]);
+      }
+      flag = (fdv(g0, gs) < CT ? 1u : 0u) + (fdv(g1, gs) < CT ? 2u : 0u) + (fdv(g2, gs) < CT ? 4u : 0u);
+    } else if (var == 18 || var == 34) {
+      // whf_TcnsN_A (:3253-3362), whf_TcnsN_COMPARE (:3472-3726, its returned
+      // branch): Teno5_Z's cut-off with C_T = 10^-beta_A, beta_A = floor(10 -
+      // 5 (1 - (1-m)^4 (1+4m))); eta of the difference pairs (d0,d2), (d1,d3), (d2,d1)
+      const T eps = var == 18 ? T(2.7551 * 1e-7) : T(kEpsA2);
+      const T d0 = q0 - q1, d1 = q1 - q2, d2 = q2 - q3, d3 = q3 - q4;
+      const T emin = smin(smin(eta2(d0, d2, eps), eta2(d1, d3, eps)), eta2(d2, d1, eps));
+      const T x = var == 18 ? T(4.1666667) * emin : fdv(emin, T(0.24));
+      const T m = T(1.0) - smin(T(1.0), x);
+      const T gm = (1 - m) * (1 - m) * (1 - m) * (1 - m) * (1 + 4 * m);
+      const int k = (int)floor(T(10.0) - T(5.0) * (1 - gm));
+      const T CT = ten_pow_neg<T>(k < 5 ? 5 : (k > 10 ? 10 : k));
+      T t0 = T(1.0) + fdv(tau, b0 + eps_tiny<T>());
+      T t1 = T(1.0) + fdv(tau, b1 + eps_tiny<T>());
+      T t2 = T(1.0) + fdv(tau, b2 + eps_tiny<T>());
+      t0 = t0 * t0;
+      t1 = t1 * t1;
+      t2 = t2 * t2;
+      const T g0 = t0 * t0 * t0, g1 = t1 * t1 * t1, g2 = t2 * t2 * t2;
+      const T rr = CT * (g0 + g1 + g2);
+      flag = (g0 < rr ? 1u : 0u) + (g1 < rr ? 2u : 0u) + (g2 < rr ? 4u : 0u);
+    } else {
+      // whf_TcnsN_S (:1519-1579, C_T' = ctp_s) and whf_TcnsN_AS (:1581-1679,
+      // C_T' = ctp[k - 5], its eta taken of the values q themselves):
+      // beta_k tau < (C_T' tau - (1 - C_T') beta_k) beta_min cuts k
+      T CTp;
+      if (var == 20) {
+        CTp = T(rc->ctp_s);
+      } else {
+        const T e = smin(smin(eta2(q0, q1, T(kEpsA2)), eta2(q1, q2, T(kEpsA2))),
+                         smin(eta2(q2, q3, T(kEpsA2)), eta2(q3, q4, T(kEpsA2))));
+        CTp = T(rc->ctp[adaptive_k(e) - 5]);
+      }
+      const T bm = smin(smin(b0, b1), b2);
+      const T c1 = T(1.0) - CTp;
+      flag = (b0 * tau < (CTp * tau - c1 * b0) * bm ? 1u : 0u) + (b1 * tau < (CTp * tau - c1 * b1) * bm ? 2u : 0u) +
+             (b2 * tau < (CTp * tau - c1 * b2) * bm ? 4u : 0u);
+    }
     return teno_pick(flag, q0, q1, q2, q3, q4);
   } else {
     // Teno5_CongZ, TENO-S (interScheme.hpp:494-585); ct = its constant CT

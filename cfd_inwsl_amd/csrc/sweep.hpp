@@ -57,6 +57,7 @@ struct SweepAux {
   // initializer.cpp:672-677)
   double x0, hx;
   double grav[3];  // SW_SOURCE: gravity (SourceTerm::calGravitySource)
+  ResearchConsts rc;  // constants of the research TENO variants
 };
 
 struct SweepArgs {
@@ -285,8 +286,8 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           for (int v = 0; v < NV; v++) {
             T q0 = cell[cidx(v, sl[0], lp)], q1 = cell[cidx(v, sl[1], lp)], q2 = cell[cidx(v, sl[2], lp)];
             T q3 = cell[cidx(v, sl[3], lp)], q4 = cell[cidx(v, sl[4], lp)], q5 = cell[cidx(v, sl[5], lp)];
-            outL[v] = inter5<SCH>(q0, q1, q2, q3, q4, ct);
-            outR[v] = inter5<SCH>(q5, q4, q3, q2, q1, ct);
+            outL[v] = inter5<SCH>(q0, q1, q2, q3, q4, ct, &a.aux->rc);
+            outR[v] = inter5<SCH>(q5, q4, q3, q2, q1, ct, &a.aux->rc);
           }
         } else {
           T pL[NV], pR[NV];
@@ -305,8 +306,8 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
                   c2v = cell[cidx(NV + 2, sl[j], lp)];
                 ch[j] = c0v * es.L[3 * k] + c1v * es.L[3 * k + 1] + c2v * es.L[3 * k + 2];
               }
-              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct);
-              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct);
+              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, &a.aux->rc);
+              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct, &a.aux->rc);
             }
             T cL[3], cR[3];
             for (int i = 0; i < 3; i++) {
@@ -334,8 +335,8 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
               es.lrow(k, l);
               T ch[6];
               for (int j = 0; j < 6; j++) ch[j] = project<D>(cw[j], l);
-              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct);
-              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct);
+              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, &a.aux->rc);
+              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct, &a.aux->rc);
             }
             // charToCons in the summation order (ch3, ch4, ch2, ch1[, ch5])
             T cL[NV], cR[NV];

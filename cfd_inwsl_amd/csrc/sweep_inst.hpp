@@ -15,7 +15,10 @@ template <> struct TilePair<2> { static constexpr int A = 1, B = 2; };
 
 template <int D, int AX, int SCH, int FLX, bool CW, int TL>
 static hipError_t launch_one(bool fp32, const SweepArgs* a, const BatchArgs* b, dim3 grid, hipStream_t s) {
-  if (b) {
+  if constexpr (SCH == SCH_WHF) {  // single-zone FP64 kernels only
+    if (b || fp32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW, TL, double>), grid, dim3(256), 0, s, *a);
+  } else if (b) {
     hipLaunchKernelGGL((sweep_batch_kernel<D, AX, SCH, FLX, CW, TL>), grid, dim3(256), 0, s, *b);
   } else if (fp32) {
 #if CFD_WITH_FP32
@@ -59,6 +62,7 @@ static hipError_t launch_sch(int sch, int flx, bool cw, int tile, bool fp32, con
   case SCH_TENO: return launch_flx<D, AX, SCH_TENO>(flx, cw, tile, fp32, a, b, grid, s);
   case SCH_TENOA: return launch_flx<D, AX, SCH_TENOA>(flx, cw, tile, fp32, a, b, grid, s);
   case SCH_TENOSA: return launch_flx<D, AX, SCH_TENOSA>(flx, cw, tile, fp32, a, b, grid, s);
+  case SCH_WHF: return launch_flx<D, AX, SCH_WHF>(flx, cw, tile, fp32, a, b, grid, s);
   default: return launch_flx<D, AX, SCH_TENOS>(flx, cw, tile, fp32, a, b, grid, s);
   }
 }

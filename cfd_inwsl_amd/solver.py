@@ -26,7 +26,22 @@ class InterMethod(enum.IntEnum):
     TCNS5CT7 = 13
     CONGTCNS5CT5 = 22    # congTcns5_ZCT5 (Teno5_Z's arithmetic)
     CONGTCNS5CT10 = 23   # congTcns5_ZCT10
+    WHFTCNSN = 17        # whf_TcnsN (chi_k < 1e-10)
+    WHFTCNSNA = 18       # whf_TcnsN_A
+    WHFTCNSNAS = 19      # whf_TcnsN_AS
+    WHFTCNSNS = 20       # whf_TcnsN_S
+    WHFTCNSNLAD = 21     # whf_TcnsN_LAD
+    WHFAITCNSNS = 24     # whf_ai_TcnsN_S_1
+    WHFAITCNSNA = 25     # whf_ai_TcnsN_A_2
+    WHFAITCNSNAS_1 = 26  # whf_ai_TcnsN_AS_1
+    WHFAITCNSNAS_2 = 27  # whf_ai_TcnsN_AS_2
+    WHFAITCNSNLADS = 28  # whf_ai_TcnsN_LADS_yuanbao
+    WHFAITCNSNAZS = 29   # whf_ai_TcnsN_AZS_yuanbao
+    WHFAITCNSNMYASF002_1 = 30    # whf_ai_TcnsN_myASF002_1
+    WHFAITCNSNMYASF002_2 = 31    # whf_ai_TcnsN_myASF002_2
+    WHFAITCNSNMYASF002_AI1 = 32  # whf_ai_TcnsN_myASF002_ai1
     WHFZYCTCNSNMYASF002_1 = 33  # whf_zyc_TcnsN_myASF002_1 (TENO-S, C_T adapted to eta_min)
+    WHFCOMPARE = 34      # whf_TcnsN_COMPARE
     NEW_WHFTCNSA = 36    # new_whf_TCNS_A ("TENO-A")
     NEW_WHFTCNSASF203_NOS = 38  # whf_TCNS_AS_myF203_NoS (TENO-S, cubic C_T law)
     NEW_WHFTCNSASF202_NOS = 39  # whf_TCNS_AS_myF202_NoS (TENO-S, quadratic C_T law)

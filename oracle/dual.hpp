@@ -46,5 +46,11 @@ inline Dual sqrt(const Dual& a) {
 }
 inline Dual abs(const Dual& a) { return a.v < 0 ? -a : a; }
 inline Dual fabs(const Dual& a) { return abs(a); }
+inline double value_of(const Dual& a) { return a.v; }
+// a^n for a constant exponent (libm pow on the value)
+inline Dual pow(const Dual& a, double n) {
+  const double p = std::pow(a.v, n);
+  return Dual(p, n * std::pow(a.v, n - 1.0) * a.d);
+}
 
 }  // namespace oracle

@@ -29,7 +29,9 @@ constexpr double GAMMA = 1.4;
 enum Scheme {
   S_JS = 0, S_Z = 1, S_TENO5 = 2, S_TENO5_CT4 = 3, S_TENO5_CT7 = 4, S_TENOS = 5, S_LINEAR = 6,
   S_TENOS_CT4 = 7, S_TENOS_CT7 = 8, S_TENO5_CT10 = 9, S_TENOA = 10,
-  S_TENOSA_F002 = 11, S_TENOSA_F203 = 12, S_TENOSA_F202 = 13
+  S_TENOSA_F002 = 11, S_TENOSA_F203 = 12, S_TENOSA_F202 = 13,
+  S_WHF_N = 14, S_WHF_A = 15, S_WHF_AS = 16, S_WHF_S = 17, S_WHF_LAD = 18,
+  S_AI_S = 19, S_AI_A = 20, S_AI_AS = 21, S_AI_LADS = 22, S_AI_AZS = 23, S_AI_F002 = 24, S_WHF_COMPARE = 25
 };
 enum Flux { F_HLLC = 0, F_ROE = 1 };
 
@@ -223,6 +225,307 @@ template <class T> inline T tenosa(const T q[5], int law) {
   return teno_select(flag, q);
 }
 
+inline double value_of(double x) { return x; }
+
+// ---- the whf_TcnsN family (InterMethod whfTCNSN .. whfTCNSNLAD = 17..21) ----
+// The interpolated value depends on the data only through the flag, so the
+// libm pow calls below decide bits only where a ratio ties its threshold.
+
+// whf_TcnsN (interScheme.hpp:1458-1517): chi_k = gamma_k / sum gamma,
+// gamma_k = pow(1 + tau / (beta_k + 1e-40), 6); cut where chi_k < 1e-10.
+template <class T> inline T whf_n(const T q[5]) {
+  using std::abs;
+  using std::pow;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  T g[3];
+  for (int i = 0; i < 3; i++) g[i] = pow(1.0 + tau / (b[i] + 1e-40), 6);
+  T gs = g[0] + g[1] + g[2];
+  unsigned flag = 0;
+  if (g[0] / gs < 1e-10) flag += 1;
+  if (g[1] / gs < 1e-10) flag += 2;
+  if (g[2] / gs < 1e-10) flag += 4;
+  return teno_select(flag, q);
+}
+
+// whf_TcnsN_A (interScheme.hpp:3253-3362): Teno5_Z's cut-off with
+// CT = 10^-beta_A, beta_A = floor(10 - 5 (1 - g)), g = (1-m)^4 (1+4m),
+// m = 1 - min(1, 4.1666667 eta_min); its eta pair the differences
+// (d0, d2), (d1, d3), (d2, d1).
+template <class T> inline T whf_a(const T q[5]) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  const double epsA = 2.7551 * 1e-7;
+  const T d0 = q[0] - q[1], d1 = q[1] - q[2], d2 = q[2] - q[3], d3 = q[3] - q[4];
+  const T e0 = (abs(2.0 * d0 * d2) + epsA) / (d0 * d0 + d2 * d2 + epsA);
+  const T e1 = (abs(2.0 * d1 * d3) + epsA) / (d1 * d1 + d3 * d3 + epsA);
+  const T e2 = (abs(2.0 * d2 * d1) + epsA) / (d2 * d2 + d1 * d1 + epsA);
+  T emin = e0;
+  if (e1 < emin) emin = e1;
+  if (e2 < emin) emin = e2;
+  const T x = 4.1666667 * emin;
+  const T m = 1.0 - (x < 1.0 ? x : T(1.0));
+  const T gm = (1 - m) * (1 - m) * (1 - m) * (1 - m) * (1 + 4 * m);
+  const int betaA = (int)std::floor((double)(10.0 - 5.0 * (1 - gm)));
+  const double cts[6] = {1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10};
+  const double CTA = cts[betaA < 5 ? 0 : (betaA > 10 ? 5 : betaA - 5)];
+  T g[3];
+  for (int i = 0; i < 3; i++) {
+    T t = 1.0 + tau / (b[i] + 1e-40);
+    t = t * t;
+    g[i] = t * t * t;
+  }
+  T rr = CTA * (g[0] + g[1] + g[2]);
+  unsigned flag = 0;
+  if (g[0] < rr) flag += 1;
+  if (g[1] < rr) flag += 2;
+  if (g[2] < rr) flag += 4;
+  return teno_select(flag, q);
+}
+
+// The cut-off of whf_TcnsN_S / _AS: stencil k is cut where
+// beta_k tau < (CT_p tau - (1 - CT_p) beta_k) beta_min.
+template <class T> inline T whf_s_cut(const T q[5], const T b[3], double CTp) {
+  using std::abs;
+  T tau = abs(b[2] - b[0]);
+  T bm = b[0];  // std::min({..})
+  if (b[1] < bm) bm = b[1];
+  if (b[2] < bm) bm = b[2];
+  unsigned flag = 0;
+  for (int k = 0; k < 3; k++)
+    if (b[k] * tau < (CTp * tau - 1.0 * (1.0 - CTp) * b[k]) * bm) flag += 1u << k;
+  return teno_select(flag, q);
+}
+
+// CT_p = pow(1.5 CT / (1 - CT), 1/6) (interScheme.hpp:1538, :1637)
+inline double whf_ct_p(double CT) { return std::pow(1.5 * CT / (1.0 - CT), 1.0 / 6); }
+
+// whf_TcnsN_S (interScheme.hpp:1519-1579): CT = 1e-5.
+template <class T> inline T whf_s(const T q[5]) {
+  T b[3];
+  betas(q, b);
+  return whf_s_cut(q, b, whf_ct_p(1e-5));
+}
+
+// whf_TcnsN_AS (interScheme.hpp:1581-1679): CT = pow(10, -beta_A) with
+// beta_A = floor(10 - 5 (1 - pow(1 - m, 4) (1 + 4 m))), m = 1 - min(1,
+// eta_min / 0.24); its eta are taken of the values q themselves (not of their
+// differences), eps_A = (0.9 0.24) / (1 - 0.9 0.24) 1e-3 1e-3.
+template <class T> inline T whf_as(const T q[5]) {
+  using std::abs;
+  using std::pow;
+  T b[3];
+  betas(q, b);
+  const double epsA = (0.9 * 0.24) / (1.0 - 0.9 * 0.24) * 1e-3 * 1e-3;
+  auto eta = [&](T a, T bb, T c) {
+    T e1 = (abs(2 * bb * a) + epsA) / (bb * bb + a * a + epsA);
+    T e2 = (abs(2 * c * bb) + epsA) / (c * c + bb * bb + epsA);
+    return e2 < e1 ? e2 : e1;
+  };
+  T emin = eta(q[0], q[1], q[2]);
+  T ek = eta(q[1], q[2], q[3]);
+  if (ek < emin) emin = ek;
+  ek = eta(q[2], q[3], q[4]);
+  if (ek < emin) emin = ek;
+  const T x = emin / 0.24;
+  const T m = 1.0 - (x < 1.0 ? x : T(1.0));
+  const T gm = pow(1 - m, 4) * (1 + 4 * m);
+  const int betaA = (int)std::floor((double)(10.0 - 5.0 * (1 - gm)));
+  return whf_s_cut(q, b, whf_ct_p(std::pow(10, -betaA)));
+}
+
+// whf_TcnsN_LAD (interScheme.hpp:1681-1760): CT = 10^-m from a table,
+// m = clamp(4 + floor(theta 6), 4, 10), theta = 1 / (1 + max_k chi_k / 10),
+// chi_k = tau / (beta_k + 1e-40); cut where gamma_k / sum gamma < CT,
+// gamma_k = pow(1 + tau / (beta_k + 1e-40), 6).
+template <class T> inline T whf_lad(const T q[5]) {
+  using std::abs;
+  using std::pow;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  T chi[3];
+  for (int i = 0; i < 3; i++) chi[i] = tau / (b[i] + 1e-40);
+  T cm = chi[0];  // std::max({..})
+  if (cm < chi[1]) cm = chi[1];
+  if (cm < chi[2]) cm = chi[2];
+  const T theta = 1.0 / (1.0 + cm / 10.0);
+  int m = (int)(4 + std::floor((double)(theta * (10 - 4))));
+  m = m < 4 ? 4 : (m > 10 ? 10 : m);
+  const double table[7] = {1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10};
+  const double CT = table[m - 4];
+  T g[3];
+  for (int i = 0; i < 3; i++) g[i] = pow(1.0 + tau / (b[i] + 1e-40), 6);
+  T gs = g[0] + g[1] + g[2];
+  unsigned flag = 0;
+  if (g[0] / gs < CT) flag += 1;
+  if (g[1] / gs < CT) flag += 2;
+  if (g[2] / gs < CT) flag += 4;
+  return teno_select(flag, q);
+}
+
+// ---- the whf_ai_TcnsN family (InterMethod whfAITCNSNS .. whfAITCNSNmyASF002_ai1
+// = 24..32) and whf_TcnsN_COMPARE (34) ----------------------------------------
+// eps_A = (0.9 C_r) / (1 - 0.9 C_r) xi^2 with C_r = 0.24, xi = 1e-3
+constexpr double kEpsA2 = (0.9 * 0.24) / (1.0 - 0.9 * 0.24) * 1e-3 * 1e-3;
+
+// min of the smoothness ratios of adjacent differences (d0,d1), (d1,d2),
+// (d2,d3); the reference writes them q_i - q_{i-1}, the same doubles
+template <class T> inline T eta_adjacent(const T q[5], double epsA) {
+  using std::abs;
+  const T d0 = q[0] - q[1], d1 = q[1] - q[2], d2 = q[2] - q[3], d3 = q[3] - q[4];
+  const T e0 = (abs(2.0 * d1 * d0) + epsA) / (d1 * d1 + d0 * d0 + epsA);
+  const T e1 = (abs(2.0 * d2 * d1) + epsA) / (d2 * d2 + d1 * d1 + epsA);
+  const T e2 = (abs(2.0 * d3 * d2) + epsA) / (d3 * d3 + d2 * d2 + epsA);
+  T e = e0;
+  if (e1 < e) e = e1;
+  if (e2 < e) e = e2;
+  return e;
+}
+
+// m = 1 - min(1, eta / 0.24), beta_A = 10 - 5 (1 - pow(1 - m, 4) (1 + 4 m)),
+// C_TA = pow(10, -floor(beta_A)) (interScheme.hpp:2300-2305, :2375-2380)
+template <class T> inline double ai_cta(const T& eta) {
+  using std::pow;
+  const T x = eta / 0.24;
+  const T m = 1.0 - (x < 1.0 ? x : T(1.0));
+  const T g = pow(1.0 - m, 4) * (1.0 + 4.0 * m);
+  const T bA = 10.0 - 5.0 * (1.0 - g);
+  return std::pow(10.0, -std::floor(value_of(bA)));
+}
+
+// TENO-S's cut-off with a given C_T (the delta_k tests of the whf_ai
+// schemes: beta_min tau < (C_T tau - (1 - C_T) beta_min) beta_k cuts k)
+template <class T, class C> inline T tenos_cut(const T q[5], const T b[3], const C& CT) {
+  using std::abs;
+  int mb = 0;
+  if (b[1] < b[mb]) mb = 1;
+  if (b[2] < b[mb]) mb = 2;
+  const C CT_1 = 1 - CT;
+  T tau = abs(b[2] - b[0]);
+  T rr = CT * tau - CT_1 * b[mb];
+  T ll = tau * b[mb];
+  unsigned flag = 0;
+  if (ll < rr * b[0]) flag += 1;
+  if (ll < rr * b[1]) flag += 2;
+  if (ll < rr * b[2]) flag += 4;
+  return teno_select(flag, q);
+}
+
+// whf_ai_TcnsN_S_1 (interScheme.hpp:1901-1995): C_T' = pow(1.5e-5 / (1 - 1e-5), 1/6)
+template <class T> inline T ai_s(const T q[5]) {
+  T b[3];
+  betas(q, b);
+  return tenos_cut(q, b, whf_ct_p(1e-5));
+}
+
+// whf_ai_TcnsN_A_2 (interScheme.hpp:2216-2328): chi_k = gamma_k / sum gamma,
+// gamma_k = pow(1 + tau / (beta_k + 1e-40), 6); cut where chi_k < C_TA
+template <class T> inline T ai_a(const T q[5]) {
+  using std::abs;
+  using std::pow;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  T g[3];
+  for (int i = 0; i < 3; i++) g[i] = pow(1.0 + tau / (b[i] + 1e-40), 6.0);
+  T gs = g[0] + g[1] + g[2];
+  const double CTA = ai_cta(eta_adjacent(q, kEpsA2));
+  unsigned flag = 0;
+  if (g[0] / gs < CTA) flag += 1;
+  if (g[1] / gs < CTA) flag += 2;
+  if (g[2] / gs < CTA) flag += 4;
+  return teno_select(flag, q);
+}
+
+// whf_ai_TcnsN_AS_1 / _AS_2 (interScheme.hpp:2329-2511, the same arithmetic):
+// TENO-S with C_T' = pow(1.5 C_TA / (1 - C_TA), 1/6)
+template <class T> inline T ai_as(const T q[5]) {
+  T b[3];
+  betas(q, b);
+  return tenos_cut(q, b, whf_ct_p(ai_cta(eta_adjacent(q, kEpsA2))));
+}
+
+// whf_ai_TcnsN_LADS_yuanbao (interScheme.hpp:2512-2647): C_T = pow(10, -m),
+// m = 5 + floor(5 theta), theta = 1 / (1 + max_k chi_k / 10)
+template <class T> inline T ai_lads(const T q[5]) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  T chi[3];
+  for (int i = 0; i < 3; i++) chi[i] = tau / (b[i] + 1e-40);
+  T cm = chi[0];
+  if (cm < chi[1]) cm = chi[1];
+  if (cm < chi[2]) cm = chi[2];
+  const T theta = 1.0 / (1.0 + cm / 10.0);
+  const int m = 5 + (int)std::floor(value_of(theta * (10 - 5)));
+  return tenos_cut(q, b, whf_ct_p(std::pow(10.0, -m)));
+}
+
+// whf_ai_TcnsN_AZS_yuanbao (interScheme.hpp:2862-2935): C_T = min(1e-5,
+// |beta0 + beta2 - 2 beta1|), C_T' = pow(1.5 C_T / (1 - C_T), 1/6)
+template <class T> inline T ai_azs(const T q[5]) {
+  using std::abs;
+  using std::pow;
+  T b[3];
+  betas(q, b);
+  const T tA = abs(b[0] + b[2] - 2 * b[1]);
+  const T C = tA < 1e-5 ? tA : T(1e-5);
+  const T CTp = pow(1.5 * C / (1 - C), 1.0 / 6.0);
+  return tenos_cut(q, b, CTp);
+}
+
+// whf_ai_TcnsN_myASF002_1 / _2 / _ai1 (interScheme.hpp:2936-3252, the same
+// values): C_T' = 0.157 / (11.351092 s^2 - 5.740650 s + 1.506631), s = min(1, eta_min / 0.24)
+template <class T> inline T ai_f002(const T q[5]) {
+  T b[3];
+  betas(q, b);
+  const T x = eta_adjacent(q, kEpsA2) / 0.24;
+  const T s = x < 1.0 ? x : T(1.0);
+  const T CT = 0.157 / (11.351092 * s * s - 5.740650 * s + 1.506631);
+  return tenos_cut(q, b, CT);
+}
+
+// whf_TcnsN_COMPARE (interScheme.hpp:3472-3726): returns its "my" branch,
+// whf_TcnsN_A's cut-off with eps_A = (0.9 C_r)/(1 - 0.9 C_r) xi^2 and
+// m = 1 - min(1, eta_min / 0.24); the other branch it evaluates is unused.
+template <class T> inline T whf_compare(const T q[5]) {
+  using std::abs;
+  T b[3];
+  betas(q, b);
+  T tau = abs(b[2] - b[0]);
+  const double epsA = kEpsA2;
+  const T d0 = q[0] - q[1], d1 = q[1] - q[2], d2 = q[2] - q[3], d3 = q[3] - q[4];
+  const T e0 = (abs(2.0 * d0 * d2) + epsA) / (d0 * d0 + d2 * d2 + epsA);
+  const T e1 = (abs(2.0 * d1 * d3) + epsA) / (d1 * d1 + d3 * d3 + epsA);
+  const T e2 = (abs(2.0 * d2 * d1) + epsA) / (d2 * d2 + d1 * d1 + epsA);
+  T emin = e0;
+  if (e1 < emin) emin = e1;
+  if (e2 < emin) emin = e2;
+  const T x = emin / 0.24;
+  const T m = 1.0 - (x < 1.0 ? x : T(1.0));
+  const T gm = (1 - m) * (1 - m) * (1 - m) * (1 - m) * (1 + 4 * m);
+  const int betaA = (int)std::floor(value_of(10.0 - 5.0 * (1 - gm)));
+  const double cts[6] = {1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10};
+  const double CTA = cts[betaA < 5 ? 0 : (betaA > 10 ? 5 : betaA - 5)];
+  T g[3];
+  for (int i = 0; i < 3; i++) {
+    T t = 1.0 + tau / (b[i] + 1e-40);
+    t = t * t;
+    g[i] = t * t * t;
+  }
+  T rr = CTA * (g[0] + g[1] + g[2]);
+  unsigned flag = 0;
+  if (g[0] < rr) flag += 1;
+  if (g[1] < rr) flag += 2;
+  if (g[2] < rr) flag += 4;
+  return teno_select(flag, q);
+}
+
 // Linear5 (interScheme.hpp:1450-1453)
 template <class T> inline T linear5(const T q[5]) { return teno_select(0u, q); }
 
@@ -241,6 +544,18 @@ template <class T> inline T inter5(int s, const T q[5]) {
   case S_TENOSA_F002: return tenosa(q, 1);
   case S_TENOSA_F203: return tenosa(q, 2);
   case S_TENOSA_F202: return tenosa(q, 3);
+  case S_WHF_N: return whf_n(q);
+  case S_WHF_A: return whf_a(q);
+  case S_WHF_AS: return whf_as(q);
+  case S_WHF_S: return whf_s(q);
+  case S_WHF_LAD: return whf_lad(q);
+  case S_AI_S: return ai_s(q);
+  case S_AI_A: return ai_a(q);
+  case S_AI_AS: return ai_as(q);
+  case S_AI_LADS: return ai_lads(q);
+  case S_AI_AZS: return ai_azs(q);
+  case S_AI_F002: return ai_f002(q);
+  case S_WHF_COMPARE: return whf_compare(q);
   default: return linear5(q);
   }
 }

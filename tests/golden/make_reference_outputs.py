@@ -5,7 +5,8 @@ CongFD1.0/cases, with their per-step logs.  This script reads them with the
 package's CGNS reader (cfd_inwsl_amd/cgns.py) and writes:
 
 * reference_outputs.npz: cell-centred Density / XVelocity / Pressure of the
-  1D end states listed in CASES (TENO.cgns and TENOS.cgns), the t=0 and t=0.2 states and the logged dt
+  1D end states listed in CASES (TENO.cgns, TENOS.cgns, TENOAS.cgns and,
+  where present, TENOLAD.cgns), the t=0 and t=0.2 states and the logged dt
   sequence of the "Sod - congTCNS5CT5 - 101" run, the t=0.1 and t=0.2 states
   of the "Sod - whfzycTCNSNmyASF002_1 - 101" run, and the 512x128 double-Mach
   density at t=0.2 (float32: it is compared loosely);
@@ -92,6 +93,14 @@ def main(root):
         sol = cgns.read_solution(os.path.join(root, c["file"].replace("TENO.cgns", "TENOS.cgns")))
         for f in ("Density", "XVelocity", "Pressure"):
             arrays[f"{key}_tenos/{f}"] = sol[f]
+        # TENOAS.cgns (whf_ai_TcnsN_AS_1/_2, InterMethod 26/27) and, where the
+        # reference has it, TENOLAD.cgns (whf_TcnsN_LAD, InterMethod 21)
+        for variant in ("TENOAS", "TENOLAD"):
+            path = os.path.join(root, c["file"].replace("TENO.cgns", variant + ".cgns"))
+            if os.path.exists(path):
+                sol = cgns.read_solution(path)
+                for f in ("Density", "XVelocity", "Pressure"):
+                    arrays[f"{key}_{variant.lower()}/{f}"] = sol[f]
     for key, c in INITIAL.items():
         sol = cgns.read_solution(os.path.join(root, c["file"]))
         zone, imax = zone_of(sol)

@@ -20,6 +20,21 @@ SCHEME_OF = {
     InterMethod.WHFZYCTCNSNMYASF002_1: O.S_TENOSA_F002,
     InterMethod.NEW_WHFTCNSASF203_NOS: O.S_TENOSA_F203,
     InterMethod.NEW_WHFTCNSASF202_NOS: O.S_TENOSA_F202,
+    InterMethod.WHFTCNSN: O.S_WHF_N,
+    InterMethod.WHFTCNSNA: O.S_WHF_A,
+    InterMethod.WHFTCNSNAS: O.S_WHF_AS,
+    InterMethod.WHFTCNSNS: O.S_WHF_S,
+    InterMethod.WHFTCNSNLAD: O.S_WHF_LAD,
+    InterMethod.WHFAITCNSNS: O.S_AI_S,
+    InterMethod.WHFAITCNSNA: O.S_AI_A,
+    InterMethod.WHFAITCNSNAS_1: O.S_AI_AS,
+    InterMethod.WHFAITCNSNAS_2: O.S_AI_AS,
+    InterMethod.WHFAITCNSNLADS: O.S_AI_LADS,
+    InterMethod.WHFAITCNSNAZS: O.S_AI_AZS,
+    InterMethod.WHFAITCNSNMYASF002_1: O.S_AI_F002,
+    InterMethod.WHFAITCNSNMYASF002_2: O.S_AI_F002,
+    InterMethod.WHFAITCNSNMYASF002_AI1: O.S_AI_F002,
+    InterMethod.WHFCOMPARE: O.S_WHF_COMPARE,
 }
 
 

@@ -214,6 +214,26 @@ def test_driver_reproduces_the_zyc_research_run(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("key,variant,inter", [(k, "tenoas", i) for k in ("sod", "shu_osher", "lax", "double_rarefaction")
+                                               for i in (26, 27)] + [(k, "tenolad", 21) for k in ("shu_osher", "lax")])
+def test_driver_reproduces_reference_research_files(tmp_path, key, variant, inter):
+    """TENOAS.cgns (whf_ai_TcnsN_AS_1 / _AS_2, inter 26 / 27) and
+    TENOLAD.cgns (whf_TcnsN_LAD, inter 21) through the driver (--inter, as
+    info.txt takes only values below INTERMAX): bit for bit."""
+    meta, arr = _reference_fixtures()
+    m = meta["cases"][key]
+    nout = int(round(m["t_end"] / m["output_dt"]))
+    _info_txt(tmp_path / "info.txt", 11, nout, m["output_dt"], m["cfl"], m["case"], m["zone"], m["imax"], 1)
+    r = _run(["--flux", "hllc", "--inter", str(inter)], tmp_path)
+    print(key, variant, inter, r.stdout[-150:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    sol = cgns.read_solution(_last_output(tmp_path))
+    for f in ("Density", "XVelocity", "Pressure"):
+        want = arr[f"{key}_{variant}/{f}"]
+        assert np.array_equal(sol[f], want), (f, np.abs(sol[f] - want).max())
+
+
+@pytest.mark.gpu
 def test_driver_double_mach_near_reference_solution_file(tmp_path):
     """Double Mach reflection, 512 x 128 cells to t = 0.2, against the
     reference's "512, 128 - TENO t=0.2000" file.  That run's settings are not

@@ -105,3 +105,25 @@ def test_launch_plan_does_not_change_results(dim):
     s.close()
     print("autotuned plan", tuned, flush=True)
     assert np.array_equal(r0, r1)
+
+
+def test_batch_research_constants_and_whf_refusal():
+    """Batches carry the research-variant constants per zone: an adaptive
+    TENO-S of the whf_ai family (C_T' from the per-zone table) evolves each
+    zone as a single-zone run does.  The whf_TcnsN-type schemes, which have
+    single-zone kernels only, are refused."""
+    zs = _zones()[:3]
+    b = Batch(2, [z for z, _ in zs], InterMethod.WHFAITCNSNAS_1, FluxMethod.HLLC, Recon.CHAR)
+    for i, (_, c) in enumerate(zs):
+        b.set_state(i, c)
+    b.steps_cfl(0.5, 2, BATCH_GRAPH)
+    got = [b.get_state(i) for i in range(len(zs))]
+    b.close()
+    for i, (z, c) in enumerate(zs):
+        s = Solver(2, z["n"], z["h"], InterMethod.WHFAITCNSNAS_1, FluxMethod.HLLC, Recon.CHAR, z["bc"])
+        s.set_state(c)
+        s.rk3_steps_cfl(0.5, 2)
+        assert np.array_equal(got[i], s.get_state()), i
+        s.close()
+    with pytest.raises(Exception, match="single zones only"):
+        Batch(2, [z for z, _ in zs], InterMethod.WHFTCNSNLAD, FluxMethod.HLLC, Recon.CHAR)

@@ -43,7 +43,12 @@ COMBOS = list(itertools.product(
 
 RESEARCH = [InterMethod.WCNS5CONGZCT4, InterMethod.WCNS5CONGZCT7, InterMethod.CONGTCNS5CT5,
             InterMethod.CONGTCNS5CT10, InterMethod.NEW_WHFTCNSA, InterMethod.WHFZYCTCNSNMYASF002_1,
-            InterMethod.NEW_WHFTCNSASF203_NOS, InterMethod.NEW_WHFTCNSASF202_NOS]
+            InterMethod.NEW_WHFTCNSASF203_NOS, InterMethod.NEW_WHFTCNSASF202_NOS,
+            InterMethod.WHFTCNSN, InterMethod.WHFTCNSNA, InterMethod.WHFTCNSNAS, InterMethod.WHFTCNSNS,
+            InterMethod.WHFTCNSNLAD, InterMethod.WHFAITCNSNS, InterMethod.WHFAITCNSNA, InterMethod.WHFAITCNSNAS_1,
+            InterMethod.WHFAITCNSNAS_2, InterMethod.WHFAITCNSNLADS, InterMethod.WHFAITCNSNAZS,
+            InterMethod.WHFAITCNSNMYASF002_1, InterMethod.WHFAITCNSNMYASF002_2, InterMethod.WHFAITCNSNMYASF002_AI1,
+            InterMethod.WHFCOMPARE]
 
 
 @pytest.mark.parametrize("dim,inter", [(d, i) for d in (1, 2, 3) for i in RESEARCH
@@ -61,7 +66,23 @@ def test_rhs_parity_research_variants(dim, inter):
         g = s.rhs()
         s.close()
         ref, _ = O.rhs(oracle_grid(dim, n, 0.05, inter, flux, recon, [BC.PERIODIC] * 6), cons)
-        _check(g, ref, f"research rhs dim={dim} inter={inter.name} {flux.name} {recon.name}")
+        what = f"research rhs dim={dim} inter={inter.name} {flux.name} {recon.name}"
+        if np.isnan(ref).any():
+            # whf_TcnsN_AS keeps the linear stencil across these jumps in
+            # component-wise form, and the Roe flux of the negative pressure
+            # it interpolates is NaN -- in the oracle as on the GPU
+            _say(f"{what}: NaN in {np.isnan(ref).mean():.3f} of the oracle's entries")
+            assert np.array_equal(np.isnan(g), np.isnan(ref)), what
+            g, ref = np.where(np.isnan(g), 0.0, g), np.where(np.isnan(ref), 0.0, ref)
+        _check(g, ref, what)
+
+
+def test_whf_kernels_are_fp64_single_zone_only():
+    """The whf_TcnsN-type schemes (chi / Teno5_Z / whf-S cut-offs) have FP64
+    single-zone kernels only: FP32 handles and batches refuse them."""
+    with pytest.raises(Exception, match="FP64 only"):
+        Solver(2, (40, 40), 0.05, InterMethod.WHFTCNSNLAD, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 4,
+               precision=1)
 
 
 def test_research_variant_wired_to_another_scheme_in_1d_is_refused():

@@ -15,9 +15,11 @@ import pytest
 from oracle import oracle as O
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
+WHF = [O.S_WHF_N, O.S_WHF_A, O.S_WHF_AS, O.S_WHF_S, O.S_WHF_LAD, O.S_AI_S, O.S_AI_A, O.S_AI_AS, O.S_AI_LADS,
+       O.S_AI_AZS, O.S_AI_F002, O.S_WHF_COMPARE]
 SCHEMES = [O.S_JS, O.S_Z, O.S_TENO5, O.S_TENO5_CT4, O.S_TENO5_CT7, O.S_TENOS, O.S_LINEAR,
            O.S_TENOS_CT4, O.S_TENOS_CT7, O.S_TENO5_CT10, O.S_TENOA, O.S_TENOSA_F002, O.S_TENOSA_F203,
-           O.S_TENOSA_F202]
+           O.S_TENOSA_F202] + WHF
 
 
 # ---------------------------------------------------------------- accuracy
@@ -84,7 +86,7 @@ def test_inter5_exact_for_quadratics(scheme):
 
 
 @pytest.mark.parametrize("scheme", [O.S_LINEAR, O.S_TENO5, O.S_TENO5_CT10, O.S_TENOA, O.S_TENOSA_F002,
-                                    O.S_TENOSA_F203, O.S_TENOSA_F202])
+                                    O.S_TENOSA_F203, O.S_TENOSA_F202] + WHF)
 def test_inter5_fifth_order_exact_for_quartics(scheme):
     rng = np.random.default_rng(2)
     for _ in range(20):
@@ -228,6 +230,77 @@ def test_adaptive_teno_s_laws_at_their_ends():
     # F202 on resolved data never cuts: the linear five-point value, exactly
     q = np.sin(0.3 * np.arange(5.0) + 0.4)
     assert _eta_min(q) >= 0.24 and O.inter5(O.S_TENOSA_F202, q) == O.inter5(O.S_LINEAR, q)
+
+
+def test_research_ct_prime_against_the_reference_constants():
+    """C_T' = (1.5 C_T / (1 - C_T))^(1/6), the constant the whf_*_S / _AS
+    schemes derive from TENO5's C_T, against the literal constants of
+    Teno5_CongZ / CT4 / CT7 (interScheme.hpp:494, :586, :661), which are
+    (1.5 C_T)^(1/6): the two differ by the factor (1 - C_T)^(-1/6) only.
+    The oracle's C_T' is read off where it decides a cut: ai_S cuts exactly
+    what TENO-S with that constant cuts."""
+    for ct, lit in ((1e-5, 0.15704178024750198), (1e-4, 0.23050581003334941), (1e-7, 0.072892337360747711)):
+        ctp = math.pow(1.5 * ct / (1.0 - ct), 1.0 / 6)
+        assert abs(ctp * (1.0 - ct) ** (1.0 / 6) - lit) <= 4e-16 * lit
+    # ai_S_1 (C_T' of 1e-5) and TENO-S (0.157041780...) differ in C_T by
+    # 1.7e-6 relative: the same cut except where a test ties that closely
+    rng = np.random.default_rng(5)
+    same = sum(O.inter5(O.S_AI_S, q) == O.inter5(O.S_TENOS, q) for q in rng.normal(size=(2000, 5)))
+    assert same >= 1995
+
+
+def _jump(rng):
+    j = rng.integers(1, 5)
+    return np.where(np.arange(5) >= j, rng.uniform(0.5, 2), 0.0) + rng.normal(scale=1e-9, size=5)
+
+
+def _resolved(rng):
+    # adjacent differences within a factor e^0.5 and of one sign: every eta > 0.88
+    d = np.exp(rng.uniform(-0.25, 0.25, size=4)) * rng.choice([-1.0, 1.0]) * rng.uniform(0.1, 1)
+    return np.concatenate([[0.0], np.cumsum(d)]) + rng.uniform(-1, 1)
+
+
+@pytest.mark.parametrize("scheme,ends", [
+    # Teno5_Z's cut-off with C_T from beta_A: 10 on resolved data, 5 next to jumps
+    (O.S_WHF_A, (O.S_TENO5_CT10, O.S_TENO5)),
+    (O.S_WHF_COMPARE, (O.S_TENO5_CT10, O.S_TENO5)),
+    # chi < C_TA with C_TA = 1e-10 on resolved data: whf_TcnsN's 1e-10
+    (O.S_AI_A, (O.S_WHF_N, None)),
+    # the eta-law TENO-S forms: resolved data give s = 1 in both F002 laws
+    (O.S_AI_F002, (O.S_TENOSA_F002, None)),
+    # C = min(1e-5, |b0 + b2 - 2 b1|) = 1e-5 next to jumps: ai_S's C_T'
+    (O.S_AI_AZS, (None, O.S_AI_S)),
+    # k = 5 next to jumps: ctp[0] = C_T' of pow(10, -5) = 1e-5, ai_S's
+    (O.S_AI_AS, (None, O.S_AI_S)),
+])
+def test_research_variants_reduce_at_the_ends_of_eta(scheme, ends):
+    """Each adaptive research variant reduces, bit for bit, to a fixed
+    scheme at one end of its indicator: resolved data (eta_min > 0.88, so
+    m = 0) and data next to a jump (eta_min ~ 1e-7, so m ~ 1).  A wrong
+    branch, constant or eps in the adaptive part breaks an equality."""
+    rng = np.random.default_rng(31 + scheme)
+    for which, gen in ((0, _resolved), (1, _jump)):
+        if ends[which] is None:
+            continue
+        diff = [q for q in (gen(rng) for _ in range(400)) if O.inter5(scheme, q) != O.inter5(ends[which], q)]
+        assert not diff, (which, diff[:2])
+
+
+def test_whf_s_forms_equal_at_k5_and_lad_bounds():
+    """whf_TcnsN_AS at k = 5 (eta of the raw values tiny: a zero next to a
+    jump) is whf_TcnsN_S (C_T' of 1e-5); whf_TcnsN_LAD on data with
+    tau = 0 (theta = 1, m = 10) is whf_TcnsN (C_T = 1e-10) and keeps every
+    stencil; ai LADS at tau = 0 takes C_T' of 1e-10."""
+    rng = np.random.default_rng(8)
+    for _ in range(300):
+        q = _jump(rng)
+        q[0] = 0.0
+        assert O.inter5(O.S_WHF_AS, q) == O.inter5(O.S_WHF_S, q), q
+    for _ in range(300):
+        a, b, c = rng.normal(size=3)
+        q = np.array([a, b, c, b, a])  # beta0 == beta2 exactly: tau = 0
+        assert O.inter5(O.S_WHF_LAD, q) == O.inter5(O.S_WHF_N, q) == O.inter5(O.S_LINEAR, q)
+        assert O.inter5(O.S_AI_LADS, q) == O.inter5(O.S_LINEAR, q)
 
 
 def test_weno_weights_js_known_value():

@@ -76,6 +76,19 @@ def test_oracle_reproduces_reference_tenos_end_states(key):
             assert np.array_equal(a, ARR[f"{key}_tenos/{f}"]), (f, np.abs(a - ARR[f"{key}_tenos/{f}"]).max())
 
 
+@pytest.mark.parametrize("key,variant,scheme", [(k, "tenoas", O.S_AI_AS) for k in META["cases"]]
+                         + [(k, "tenolad", O.S_WHF_LAD) for k in ("shu_osher", "lax")])
+def test_oracle_reproduces_reference_research_end_states(key, variant, scheme):
+    """TENOAS.cgns = whf_ai_TcnsN_AS_1 / _AS_2 (InterMethod 26 / 27, the
+    same arithmetic) and TENOLAD.cgns = whf_TcnsN_LAD (21): Density,
+    XVelocity and Pressure bit for bit, with the same runs' settings."""
+    m = META["cases"][key]
+    cons, t = _run_1d(m, m["output_dt"], scheme)
+    for a, f in zip(_prim(cons), ("Density", "XVelocity", "Pressure")):
+        want = ARR[f"{key}_{variant}/{f}"]
+        assert np.array_equal(a, want), (f, np.abs(a - want).max())
+
+
 def test_oracle_output_dt_matters():
     """The pin above is not trivially met: the same run with the output
     interval of another case (a different dt clamp sequence) differs."""
