@@ -65,11 +65,16 @@ int cfd_select_scheme(int inter, int dim, int* sch, double* ct) {
   case 8: case 37: *sch = SCH_TENOS; *ct = 0.15704178024750198; return 0;  // WCNS5CONGZ, new_WCNS5CONGZ
   case 9: *sch = SCH_TENOS; *ct = 0.23050581003334941; return 0;           // WCNS5CONGZCT4
   case 10:  // WCNS5CONGZCT7: Teno5_CongZCT7 in 2D (solverType.cpp:435-437); the 1D
-            // Euler path wires Teno5_CongAA instead (:248-250), which is not built
-    if (dim == 1) return -1;
+            // Euler path wires Teno5_CongAA instead (:248-250)
+    if (dim == 1) {
+      *sch = SCH_TENOSA;
+      *ct = 9.0;
+      return 0;
+    }
     *sch = SCH_TENOS;
     *ct = 0.072892337360747711;
     return 0;
+  case 15: *sch = SCH_WHF; *ct = 15.0; return 0;  // MUCSLIN5 (musclIn5)
   case 0: case 1: case 4: case 5: case 6: case 7: case 14: case 16:
     *sch = SCH_JS;  // reference default branch
     return 0;

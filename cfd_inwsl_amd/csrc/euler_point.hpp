@@ -109,6 +109,12 @@ template <class T> __device__ __forceinline__ T pow6(T x) {
   const T x2 = x * x;
   return x2 * x2 * x2;
 }
+// min of the ratios of adjacent values (q0,q1) .. (q3,q4) -- whf_TcnsN_AS and
+// Teno5_CongAA take eta of the values themselves (interScheme.hpp:318-330, :1613-1627)
+template <class T> __device__ __forceinline__ T eta_values(T q0, T q1, T q2, T q3, T q4) {
+  return smin(smin(eta2(q0, q1, T(kEpsA2)), eta2(q1, q2, T(kEpsA2))),
+              smin(eta2(q2, q3, T(kEpsA2)), eta2(q3, q4, T(kEpsA2))));
+}
 // k = floor(10 - 5 (1 - pow(1 - m, 4) (1 + 4 m))), m = 1 - min(1, eta / 0.24),
 // clamped to 5..10 (interScheme.hpp:1633-1634, :2302-2304)
 template <class T> __device__ __forceinline__ int adaptive_k(T eta) {
@@ -286,6 +292,8 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
     //   7: whf_ai_TcnsN_AZS_yuanbao (:2862-2935), C_T = pow(1.5 c / (1 - c), 1/6),
     //      c = min(1e-5, |beta0 + beta2 - 2 beta1|)
     //   8: whf_ai_TcnsN_myASF002_1 / _2 / _ai1 (:2936-3252), law 1 with s = min(1, eta_min / 0.24), eps_A = kEpsA2
+    //   9: Teno5_CongAA (:304-382), C_T = cta[k - 5] with k = round(10 - 5 (1 - (1-m)^4 (1+4m))),
+    //      eta of the values q themselves, eps_A = kEpsA2
     const T bm = smin(smin(b0, b1), b2);
     T tau = fabs(b2 - b0);
     const int law = (int)ct;
@@ -313,9 +321,15 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
     } else if (law == 7) {
       const T c = smin(T(1e-5), fabs(b0 + b2 - T(2) * b1));
       CT = tpow(fdv(T(1.5) * c, T(1) - c), T(1.0 / 6.0));
-    } else {
+    } else if (law == 8) {
       const T x = smin(T(1.0), fdv(eta_adjacent(q0, q1, q2, q3, q4, T(kEpsA2)), T(0.24)));
       CT = fdv(T(0.157), T(11.351092) * x * x - T(5.740650) * x + T(1.506631));
+    } else {
+      const T e = eta_values(q0, q1, q2, q3, q4);
+      const T m = T(1.0) - smin(T(1.0), fdv(e, T(0.24)));
+      const T g = pow4(T(1.0) - m) * (T(1.0) + T(4.0) * m);
+      int k = (int)round(T(10.0) - T(5.0) * (T(1.0) - g));
+      CT = T(rc->cta[(k < 5 ? 5 : (k > 10 ? 10 : k)) - 5]);
     }
     const T CT_1 = 1 - CT;
     T rr = CT * tau - CT_1 * bm;
@@ -328,6 +342,13 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
     // only, so the libm pow of the reference (here pow4 / pow6) decides bits
     // only where a ratio ties its threshold.
     const int var = (int)ct;
+    if (var == 15) {
+      // musclIn5 (:1386-1406): q2 + delta / 2, delta the minmod slope of q1, q2, q3
+      const T dm = q2 - q1, dp = q3 - q2;
+      const T delta = dp > T(0) ? smax(T(0.0), smax(smin(dm, dp), smin(dm, dp)))
+                                : smin(T(0.0), smin(smax(dm, dp), smax(dm, dp)));
+      return q2 + delta * T(0.5);
+    }
     T tau = fabs(b2 - b0);
     unsigned flag;
     if (var == 17 || var == 21 || var == 25) {
@@ -377,9 +398,7 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
       if (var == 20) {
         CTp = T(rc->ctp_s);
       } else {
-        const T e = smin(smin(eta2(q0, q1, T(kEpsA2)), eta2(q1, q2, T(kEpsA2))),
-                         smin(eta2(q2, q3, T(kEpsA2)), eta2(q3, q4, T(kEpsA2))));
-        CTp = T(rc->ctp[adaptive_k(e) - 5]);
+        CTp = T(rc->ctp[adaptive_k(eta_values(q0, q1, q2, q3, q4)) - 5]);
       }
       const T bm = smin(smin(b0, b1), b2);
       const T c1 = T(1.0) - CTp;

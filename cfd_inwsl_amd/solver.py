@@ -20,10 +20,11 @@ class InterMethod(enum.IntEnum):
     WCNSZ5 = 3           # weno5_Z
     WCNS5CONGZ = 8       # Teno5_CongZ (TENO-S)
     WCNS5CONGZCT4 = 9    # Teno5_CongZCT4
-    WCNS5CONGZCT7 = 10   # Teno5_CongZCT7 (2D/3D; the reference's 1D path wires another scheme)
+    WCNS5CONGZCT7 = 10   # Teno5_CongZCT7 in 2D/3D; Teno5_CongAA in 1D, as the reference wires it
     TCNS5 = 11           # Teno5_Z (TENO5)
     TCNS5CT4 = 12
     TCNS5CT7 = 13
+    MUCSLIN5 = 15        # musclIn5 (minmod MUSCL on q[1..3])
     CONGTCNS5CT5 = 22    # congTcns5_ZCT5 (Teno5_Z's arithmetic)
     CONGTCNS5CT10 = 23   # congTcns5_ZCT10
     WHFTCNSN = 17        # whf_TcnsN (chi_k < 1e-10)

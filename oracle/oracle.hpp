@@ -31,7 +31,8 @@ enum Scheme {
   S_TENOS_CT4 = 7, S_TENOS_CT7 = 8, S_TENO5_CT10 = 9, S_TENOA = 10,
   S_TENOSA_F002 = 11, S_TENOSA_F203 = 12, S_TENOSA_F202 = 13,
   S_WHF_N = 14, S_WHF_A = 15, S_WHF_AS = 16, S_WHF_S = 17, S_WHF_LAD = 18,
-  S_AI_S = 19, S_AI_A = 20, S_AI_AS = 21, S_AI_LADS = 22, S_AI_AZS = 23, S_AI_F002 = 24, S_WHF_COMPARE = 25
+  S_AI_S = 19, S_AI_A = 20, S_AI_AS = 21, S_AI_LADS = 22, S_AI_AZS = 23, S_AI_F002 = 24, S_WHF_COMPARE = 25,
+  S_CONGAA = 26, S_MUSCL5 = 27
 };
 enum Flux { F_HLLC = 0, F_ROE = 1 };
 
@@ -526,6 +527,47 @@ template <class T> inline T whf_compare(const T q[5]) {
   return teno_select(flag, q);
 }
 
+// Teno5_CongAA (interScheme.hpp:304-382; the 1D Euler path's scheme for
+// WCNS5CONGZCT7, solverType.cpp:248-250): TENO-S's cut-off with
+// C_T = pow(10, -round(10 - 5 (1 - pow(1 - m, 4) (1 + 4 m)))), m = 1 - min(1,
+// eta_min / 0.24), eta of the values q themselves (as whf_TcnsN_AS).
+template <class T> inline T congaa(const T q[5]) {
+  using std::abs;
+  using std::pow;
+  T b[3];
+  betas(q, b);
+  const double epsA = kEpsA2;
+  auto eta = [&](T a, T bb, T c) {
+    T e1 = (abs(2 * bb * a) + epsA) / (bb * bb + a * a + epsA);
+    T e2 = (abs(2 * c * bb) + epsA) / (c * c + bb * bb + epsA);
+    return e2 < e1 ? e2 : e1;
+  };
+  T emin = eta(q[0], q[1], q[2]);
+  T ek = eta(q[1], q[2], q[3]);
+  if (ek < emin) emin = ek;
+  ek = eta(q[2], q[3], q[4]);
+  if (ek < emin) emin = ek;
+  const T x = emin / 0.24;
+  const T m = 1.0 - (x < 1.0 ? x : T(1.0));
+  const T g = pow(1 - m, 4) * (1 + 4 * m);
+  const double CT = std::pow(10, -std::round(value_of(10.0 - 5.0 * (1 - g))));
+  return tenos_cut(q, b, CT);
+}
+
+// musclIn5 (interScheme.hpp:1386-1406): q2 + delta / 2 with the minmod slope
+// of q1, q2, q3 (beta = 1).
+template <class T> inline T muscl5(const T q[5]) {
+  const T dm = q[2] - q[1], dp = q[3] - q[2];
+  auto mx = [](T a, T b) { return a < b ? b : a; };  // std::max
+  auto mn = [](T a, T b) { return b < a ? b : a; };  // std::min
+  T delta;
+  if (dp > 0)
+    delta = mx(T(0.0), mx(mn(1.0 * dm, dp), mn(dm, 1.0 * dp)));
+  else
+    delta = mn(T(0.0), mn(mx(1.0 * dm, dp), mx(dm, 1.0 * dp)));
+  return q[2] + delta * 0.5;
+}
+
 // Linear5 (interScheme.hpp:1450-1453)
 template <class T> inline T linear5(const T q[5]) { return teno_select(0u, q); }
 
@@ -556,6 +598,8 @@ template <class T> inline T inter5(int s, const T q[5]) {
   case S_AI_AZS: return ai_azs(q);
   case S_AI_F002: return ai_f002(q);
   case S_WHF_COMPARE: return whf_compare(q);
+  case S_CONGAA: return congaa(q);
+  case S_MUSCL5: return muscl5(q);
   default: return linear5(q);
   }
 }

@@ -23,6 +23,7 @@ S_TENOS_CT4, S_TENOS_CT7, S_TENO5_CT10, S_TENOA = range(7, 11)
 S_TENOSA_F002, S_TENOSA_F203, S_TENOSA_F202 = range(11, 14)
 S_WHF_N, S_WHF_A, S_WHF_AS, S_WHF_S, S_WHF_LAD = range(14, 19)
 S_AI_S, S_AI_A, S_AI_AS, S_AI_LADS, S_AI_AZS, S_AI_F002, S_WHF_COMPARE = range(19, 26)
+S_CONGAA, S_MUSCL5 = range(26, 28)
 # oracle::Flux
 F_HLLC, F_ROE = 0, 1
 

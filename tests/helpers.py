@@ -35,11 +35,21 @@ SCHEME_OF = {
     InterMethod.WHFAITCNSNMYASF002_2: O.S_AI_F002,
     InterMethod.WHFAITCNSNMYASF002_AI1: O.S_AI_F002,
     InterMethod.WHFCOMPARE: O.S_WHF_COMPARE,
+    InterMethod.MUCSLIN5: O.S_MUSCL5,
 }
 
 
+def scheme_of(inter, dim):
+    """The oracle scheme the reference wires to an InterMethod in dim
+    dimensions (its 1D Euler path takes Teno5_CongAA for WCNS5CONGZCT7,
+    solverType.cpp:248-250)."""
+    if dim == 1 and inter == InterMethod.WCNS5CONGZCT7:
+        return O.S_CONGAA
+    return SCHEME_OF[inter]
+
+
 def oracle_grid(dim, n, h, inter, flux, recon, bc, bc_prim=None, x0=0.0, t=0.0, gravity=(0.0, 0.0, 0.0)):
-    return O.make_grid(dim, n, h, SCHEME_OF[inter], int(flux), 1 if recon == Recon.CHAR else 0,
+    return O.make_grid(dim, n, h, scheme_of(inter, dim), int(flux), 1 if recon == Recon.CHAR else 0,
                        [int(b) for b in bc], bc_prim, x0=x0, t=t, gravity=gravity)
 
 

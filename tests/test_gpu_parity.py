@@ -48,11 +48,10 @@ RESEARCH = [InterMethod.WCNS5CONGZCT4, InterMethod.WCNS5CONGZCT7, InterMethod.CO
             InterMethod.WHFTCNSNLAD, InterMethod.WHFAITCNSNS, InterMethod.WHFAITCNSNA, InterMethod.WHFAITCNSNAS_1,
             InterMethod.WHFAITCNSNAS_2, InterMethod.WHFAITCNSNLADS, InterMethod.WHFAITCNSNAZS,
             InterMethod.WHFAITCNSNMYASF002_1, InterMethod.WHFAITCNSNMYASF002_2, InterMethod.WHFAITCNSNMYASF002_AI1,
-            InterMethod.WHFCOMPARE]
+            InterMethod.WHFCOMPARE, InterMethod.MUCSLIN5]
 
 
-@pytest.mark.parametrize("dim,inter", [(d, i) for d in (1, 2, 3) for i in RESEARCH
-                                       if not (d == 1 and i == InterMethod.WCNS5CONGZCT7)])
+@pytest.mark.parametrize("dim,inter", [(d, i) for d in (1, 2, 3) for i in RESEARCH])
 def test_rhs_parity_research_variants(dim, inter):
     """The research TENO variants (runtime cut-off constants of TENO5 / TENO-S,
     TENO-A's kernels, and the kernels of TENO-S with C_T following eta_min,
@@ -85,9 +84,19 @@ def test_whf_kernels_are_fp64_single_zone_only():
                precision=1)
 
 
-def test_research_variant_wired_to_another_scheme_in_1d_is_refused():
-    with pytest.raises(Exception, match="not implemented"):
-        Solver(1, (40,), 0.05, InterMethod.WCNS5CONGZCT7, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 2)
+def test_wcns5congzct7_is_teno5_congaa_in_1d():
+    """The reference's 1D Euler path wires WCNS5CONGZCT7 to Teno5_CongAA, its
+    2D path to Teno5_CongZCT7: the 1D handle equals the oracle's CongAA and
+    differs from CT7 on this data."""
+    cons = random_state(1, (61,), seed=29, jumps=True)
+    s = Solver(1, (61,), 0.05, InterMethod.WCNS5CONGZCT7, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 2)
+    s.set_state(cons)
+    g = s.rhs()
+    s.close()
+    ref, _ = O.rhs(O.make_grid(1, (61,), 0.05, O.S_CONGAA, O.F_HLLC, 1, [1, 1]), cons)
+    ct7, _ = O.rhs(O.make_grid(1, (61,), 0.05, O.S_TENOS_CT7, O.F_HLLC, 1, [1, 1]), cons)
+    assert np.array_equal(g, ref)
+    assert not np.array_equal(g, ct7)
 
 
 @pytest.mark.parametrize("dim,inter,flux,recon", COMBOS)

@@ -16,7 +16,7 @@ from oracle import oracle as O
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 WHF = [O.S_WHF_N, O.S_WHF_A, O.S_WHF_AS, O.S_WHF_S, O.S_WHF_LAD, O.S_AI_S, O.S_AI_A, O.S_AI_AS, O.S_AI_LADS,
-       O.S_AI_AZS, O.S_AI_F002, O.S_WHF_COMPARE]
+       O.S_AI_AZS, O.S_AI_F002, O.S_WHF_COMPARE, O.S_CONGAA]
 SCHEMES = [O.S_JS, O.S_Z, O.S_TENO5, O.S_TENO5_CT4, O.S_TENO5_CT7, O.S_TENOS, O.S_LINEAR,
            O.S_TENOS_CT4, O.S_TENOS_CT7, O.S_TENO5_CT10, O.S_TENOA, O.S_TENOSA_F002, O.S_TENOSA_F203,
            O.S_TENOSA_F202] + WHF
@@ -301,6 +301,28 @@ def test_whf_s_forms_equal_at_k5_and_lad_bounds():
         q = np.array([a, b, c, b, a])  # beta0 == beta2 exactly: tau = 0
         assert O.inter5(O.S_WHF_LAD, q) == O.inter5(O.S_WHF_N, q) == O.inter5(O.S_LINEAR, q)
         assert O.inter5(O.S_AI_LADS, q) == O.inter5(O.S_LINEAR, q)
+
+
+def test_muscl5_minmod_values():
+    """musclIn5: q2 + minmod(q2 - q1, q3 - q2) / 2 of its middle three values."""
+    assert O.inter5(O.S_MUSCL5, [9.0, 1.0, 2.0, 4.0, -7.0]) == 2.5   # slopes 1, 2: the smaller
+    assert O.inter5(O.S_MUSCL5, [0.0, 4.0, 3.0, 1.0, 5.0]) == 2.5    # slopes -1, -2
+    assert O.inter5(O.S_MUSCL5, [0.0, 1.0, 3.0, 2.0, 0.0]) == 3.0    # extremum: no slope
+    assert O.inter5(O.S_MUSCL5, [5.0, 1.0, 1.0, 7.0, 2.0]) == 1.0    # one slope zero
+
+
+def test_congaa_cut_is_within_teno_s_ct7():
+    """Teno5_CongAA uses TENO-S's test with C_T = 10^-k, k in 5..10 -- below
+    CT7's 0.0729, so it cuts a subset of what TENO-S CT7 cuts; on resolved
+    data (k = 10) it keeps every stencil."""
+    rng = np.random.default_rng(17)
+    for _ in range(1500):
+        q = rng.normal(size=5) * rng.choice([1e-3, 1.0])
+        a, b = _cut(O.S_CONGAA, q), _cut(O.S_TENOS_CT7, q)
+        if a is not None and b is not None:
+            assert a <= b, q
+    q = 1.0 + 0.1 * np.arange(5.0) ** 2  # values of one sign and scale: eta ~ 1
+    assert O.inter5(O.S_CONGAA, q) == O.inter5(O.S_LINEAR, q)
 
 
 def test_weno_weights_js_known_value():
