@@ -41,8 +41,8 @@ def test_version_string():
     ("dim", 4, "dim"),
     ("diff_method", 1, "MND6"),
     ("flux_method", 5, "flux_method"),
-    ("inter_method", 15, "not implemented"),  # MUCSLIN5 (musclIn5): not built
-    ("inter_method", 40, "not implemented"),
+    ("inter_method", -1, "not implemented"),
+    ("inter_method", 40, "not implemented"),  # beyond the reference's InterMethod values
 ])
 def test_bad_config_is_rejected_before_any_gpu_call(field, value, msg):
     lib = solver.load_library()
