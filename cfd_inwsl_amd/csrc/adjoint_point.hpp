@@ -791,6 +791,12 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
     lQL[k] = f.R[0][k] * lcL[0] + f.R[1][k] * lcL[1] + f.R[2][k] * lcL[2] + f.R[3][k] * lcL[3];
     lQR[k] = f.R[0][k] * lcR[0] + f.R[1][k] * lcR[1] + f.R[2][k] * lcR[2] + f.R[3][k] * lcR[3];
   }
+  // conservative cotangents of the six cells, summed over the waves before
+  // the transpose of prim -> cons (linear in them) is applied once per cell
+  double lc[6][4];
+  CFD_UNROLL
+  for (int j = 0; j < 6; j++)
+    for (int v = 0; v < 4; v++) lc[j][v] = 0.0;
   CFD_UNROLL
   for (int k = 0; k < 4; k++) {
     double ch[6];
@@ -816,16 +822,18 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
     lch[5] = gr[0];
     double gLk[4] = {0, 0, 0, 0};
     CFD_UNROLL
-    for (int j = 0; j < 6; j++) {
-      double p[4], lt[4];
+    for (int j = 0; j < 6; j++)
       for (int v = 0; v < 4; v++) {
-        p[v] = w(j, v);
         gLk[v] += lch[j] * w.cons(j, v);
-        lt[v] = lch[j] * f.L[k][v];
+        lc[j][v] += lch[j] * f.L[k][v];
       }
-      p2c_vjp(p, lt, lw[j]);
-    }
     es.acc_Lrow(k, gLk, acc);
+  }
+  CFD_UNROLL
+  for (int j = 0; j < 6; j++) {
+    double p[4];
+    for (int v = 0; v < 4; v++) p[v] = w(j, v);
+    p2c_vjp(p, lc[j], lw[j]);
   }
   es.finish(acc, pl, pr, lw[2], lw[3]);
 }
