@@ -172,6 +172,19 @@ def test_driver_initial_state_equals_reference_file(tmp_path, key):
         assert _digest(sol[name]) == want, name
 
 
+@pytest.mark.parametrize("key,inter", [("sod_101", 33), ("woodward_colella_40", 34), ("woodward_colella_400", 34)])
+def test_driver_output_names_are_the_reference_names(tmp_path, key, inter):
+    """The research runs' t = 0 files carry the reference's names ("<case> -
+    <scheme> - <grid> - t=0.0000.cgns", disStr of info.cpp:155-198) as well
+    as its fields."""
+    m = META["initial"][key]
+    path = _initial_only(tmp_path, m["dim"], m["case"], m["zone"], m["imax"], extra=("--inter", str(inter)))
+    assert path.name == os.path.basename(m["file"])
+    sol = cgns.read_solution(path)
+    for name, want in m["sha256"].items():
+        assert _digest(sol[name]) == want, name
+
+
 def test_driver_cgns_tree_is_the_reference_tree(tmp_path):
     m = META["cases"]["sod"]
     path = _initial_only(tmp_path, 1, 0, m["zone"], m["imax"])
