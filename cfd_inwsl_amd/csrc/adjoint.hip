@@ -1,5 +1,5 @@
 // Discrete adjoint of the 2D right-hand side and of the SSP-RK3 step for the
-// characteristic-wise WCNS5 (WENO-JS) + HLLC + MND6 configuration, FP64.
+// characteristic-wise WCNS5 (WENO-JS) or TENO5 + HLLC + MND6 configurations, FP64.
 //
 // Forward map (per direction, per pencil of n cells, G_k the interface flux
 // between cells k-1 and k built from cells k-3 .. k+2, f_j the node flux):
@@ -85,6 +85,7 @@ struct VjpArgs {
   // component it negates (0: none)
   unsigned gx_src, gx_flip;
   double coef;
+  double ct;         // TENO5 cut-off constant
   long N;
   int n[2];
   int seg_len;
@@ -99,7 +100,8 @@ struct VjpArgs {
 // pressure clipping, shock factors, wave region -- is the branch the solver
 // took, also where a predicate is tied at rounding level (pressure
 // equilibrium makes p* - p_L vanish up to rounding).
-template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd& f) {
+template <int AX, class IP, class W>
+__device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd& f, double ct) {
   double pL[4], pR[4];
   for (int v = 0; v < 4; v++) {
     pL[v] = w(2, v);
@@ -107,6 +109,7 @@ template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(c
   }
   cfd::RoeSys<2, AX, double> es;
   es.build(pL, pR);
+  f.tflag = 0u;
   for (int k = 0; k < 4; k++) {
     double l[4];
     es.lrow(k, l);
@@ -117,8 +120,16 @@ template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(c
       ch[j] = cfd::project<2>(c, l);
     }
     for (int v = 0; v < 4; v++) f.L[k][v] = l[v];
-    f.QL[k] = cfd::inter5<cfd::SCH_JS>(ch[0], ch[1], ch[2], ch[3], ch[4], 0.0);
-    f.QR[k] = cfd::inter5<cfd::SCH_JS>(ch[5], ch[4], ch[3], ch[2], ch[1], 0.0);
+    if constexpr (IP::kLinear) {
+      const unsigned fl = cfd::teno5_flag(ch[0], ch[1], ch[2], ch[3], ch[4], ct);
+      const unsigned fr = cfd::teno5_flag(ch[5], ch[4], ch[3], ch[2], ch[1], ct);
+      f.QL[k] = cfd::teno_pick(fl, ch[0], ch[1], ch[2], ch[3], ch[4]);
+      f.QR[k] = cfd::teno_pick(fr, ch[5], ch[4], ch[3], ch[2], ch[1]);
+      f.tflag |= (fl | (fr << 3)) << (6 * k);
+    } else {
+      f.QL[k] = cfd::inter5<cfd::SCH_JS>(ch[0], ch[1], ch[2], ch[3], ch[4], 0.0);
+      f.QR[k] = cfd::inter5<cfd::SCH_JS>(ch[5], ch[4], ch[3], ch[2], ch[1], 0.0);
+    }
   }
   // charToCons in the summation order (ch3, ch4, ch2, ch1)
   const int order[4] = {2, 3, 1, 0};
@@ -152,7 +163,7 @@ template <int AX, class W> __device__ __forceinline__ void iface_forward_exact(c
   f.hb = cfdadj::hllc_branch_bits<AX>(f.oL, f.oR);
 }
 
-template <int AX, int CA, int CP>
+template <int AX, int CA, int CP, class IP>
 __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
   __shared__ double wc[8 * RC * CP];  // primitive state, then the conservative state rebuilt from it
@@ -286,8 +297,8 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) * ih;
       const Cells cells(wc, k, lp);
       cfdadj::IfaceFwd f;
-      iface_forward_exact<AX>(cells, f);
-      cfdadj::iface_backward<AX>(cells, f, lG, lw);
+      iface_forward_exact<AX, IP>(cells, f, a.ct);
+      cfdadj::iface_backward<AX, IP>(cells, f, lG, lw);
     }
     for (int j = 0; j < 6; j++) {
       if (act) {
@@ -432,9 +443,9 @@ void cfd_adjoint_release(cfd_handle* h) {
 }
 
 static int adj_supported(cfd_handle* h) {
-  if (h->D != 2 || h->fp32 || !h->cw || h->flx != 0 || h->sch != cfd::SCH_JS)
+  if (h->D != 2 || h->fp32 || !h->cw || h->flx != 0 || (h->sch != cfd::SCH_JS && h->sch != cfd::SCH_TENO))
     return cfd_fail(CFD_E_UNSUPP,
-                    "the adjoint covers 2D FP64 characteristic-wise WCNS5 (WENO-JS) with HLLC flux");
+                    "the adjoint covers 2D FP64 characteristic-wise WCNS5 (WENO-JS) or TENO5 with HLLC flux");
   if (h->cfg.gravity[0] != 0.0 || h->cfg.gravity[1] != 0.0 || h->cfg.gravity[2] != 0.0)
     return cfd_fail(CFD_E_UNSUPP, "the adjoint has no source term");
   for (int f = 0; f < 4; f++)
@@ -529,6 +540,7 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
       a.gx_flip |= (unsigned)(o.flip > 0 ? o.flip : 0) << (2 * q);
     }
     a.coef = coef;
+    a.ct = h->ct;
     a.use_init = ax == 0 && init != nullptr;
     if (a.use_init) a.init = *init;
     a.N = h->N;
@@ -542,14 +554,17 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
     const unsigned nseg = (unsigned)((n + a.seg_len - 1) / a.seg_len);
+    const bool teno = h->sch == cfd::SCH_TENO;
     if (ax == 0) {
       constexpr int CA = VjpTile<0>::CA, CP = VjpTile<0>::CP;
       const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
-      hipLaunchKernelGGL((k_rhs_vjp<0, CA, CP>), grid, dim3(CA * CP), 0, h->stream, a);
+      if (teno) hipLaunchKernelGGL((k_rhs_vjp<0, CA, CP, cfdadj::Teno>), grid, dim3(CA * CP), 0, h->stream, a);
+      else hipLaunchKernelGGL((k_rhs_vjp<0, CA, CP, cfdadj::JS>), grid, dim3(CA * CP), 0, h->stream, a);
     } else {
       constexpr int CA = VjpTile<1>::CA, CP = VjpTile<1>::CP;
       const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
-      hipLaunchKernelGGL((k_rhs_vjp<1, CA, CP>), grid, dim3(CA * CP), 0, h->stream, a);
+      if (teno) hipLaunchKernelGGL((k_rhs_vjp<1, CA, CP, cfdadj::Teno>), grid, dim3(CA * CP), 0, h->stream, a);
+      else hipLaunchKernelGGL((k_rhs_vjp<1, CA, CP, cfdadj::JS>), grid, dim3(CA * CP), 0, h->stream, a);
       hipLaunchKernelGGL(k_ghost_fold_y, dim3((unsigned)((pencils * 8 + 255) / 256)), dim3(256), 0, h->stream, a);
     }
     AHIP(hipGetLastError());

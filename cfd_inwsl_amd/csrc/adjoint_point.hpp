@@ -1,11 +1,12 @@
 // Hand-written reverse-mode (vector-Jacobian) counterparts of the 2D
-// characteristic-wise WENO-JS + HLLC interface flux, the node flux and the
+// characteristic-wise WENO-JS / TENO5 + HLLC interface flux, the node flux and the
 // conservative->primitive map.  Each *_vjp function recomputes its forward
 // intermediates and walks them backwards; branch decisions (HLLC wave
 // pattern, ps clipping, shock factors, positivity fallback) are those of the
 // forward evaluation, i.e. the derivative of the active branch.
 //
 // Forward formulas: weno5_JSchen (zone_FD/src/include/interScheme.hpp:17-43),
+// Teno5_Z (interScheme.hpp:384-452),
 // eigensystemEuler2D (zone_FD/src/src/eigenSystem.cpp:13-168),
 // Recon5Order2DEulerEig::reconI (zone_FD/src/include/reconstructor5order.hpp:247-356),
 // HLLCFlux2D2 (zone_FD/src/include/fluxSchemes.hpp:202-327),
@@ -57,6 +58,7 @@ CFD_HD inline double rcp(double x) {
 
 // ---- WENO-JS half-node value and its gradient ---------------------------------
 struct JS {
+  static constexpr bool kLinear = false;
   double b[3], a[3], u[3], e[3], S, iS, out;
   CFD_HD void fwd(const double* q) {
     CFD_CONTRACT
@@ -105,6 +107,56 @@ struct JS {
     g[2] += s0A + 3.0 * s0B - 2.0 * s1A + s2A + 3.0 * s2B;
     g[3] += s1A - s1B - 2.0 * s2A - 4.0 * s2B;
     g[4] += s2A + s2B;
+  }
+};
+
+// ---- TENO5 (Teno5_Z) half-node value and its gradient --------------------------
+// The value is the fixed stencil its cut-off flag selects, so the gradient
+// is that stencil's coefficients; the flag is a branch of the forward
+// (decided by the forward's arithmetic, interScheme.hpp:384-452).
+struct Teno {
+  static constexpr bool kLinear = true;
+  double ct = 1e-5;
+  unsigned flag = 0;
+  double out = 0.0;
+  // flag in the oracle's arithmetic (no contraction): host checks
+  CFD_HD void fwd(const double* q) {
+    const double b0 = (q[0] - 2.0 * q[1] + q[2]) * (q[0] - 2.0 * q[1] + q[2]) +
+                      0.25 * ((q[0] - 4.0 * q[1] + 3.0 * q[2]) * (q[0] - 4.0 * q[1] + 3.0 * q[2]));
+    const double b1 = (q[1] - 2.0 * q[2] + q[3]) * (q[1] - 2.0 * q[2] + q[3]) + 0.25 * ((q[1] - q[3]) * (q[1] - q[3]));
+    const double b2 = (q[2] - 2.0 * q[3] + q[4]) * (q[2] - 2.0 * q[3] + q[4]) +
+                      0.25 * ((3.0 * q[2] - 4.0 * q[3] + q[4]) * (3.0 * q[2] - 4.0 * q[3] + q[4]));
+    const double tau = fabs(b2 - b0);
+    const double bb[3] = {b0, b1, b2};
+    double g[3];
+    for (int i = 0; i < 3; i++) {
+      double t = 1.0 + tau / (bb[i] + 1e-40);
+      t = t * t;
+      g[i] = t * t * t;
+    }
+    const double CT = ct * (g[0] + g[1] + g[2]);
+    flag = (g[0] < CT ? 1u : 0u) + (g[1] < CT ? 2u : 0u) + (g[2] < CT ? 4u : 0u);
+    double c[5];
+    coef(flag, c);
+    out = c[0] * q[0] + c[1] * q[1] + c[2] * q[2] + c[3] * q[3] + c[4] * q[4];
+  }
+  // stencil coefficients of a flag (interScheme.hpp:421-449)
+  CFD_HD static void coef(unsigned f, double* c) {
+    const double t[8][5] = {{0.0234375, -0.15625, 0.703125, 0.46875, -0.0390625},
+                            {0.0, -0.0625, 0.5625, 0.5625, -0.0625},
+                            {0.0, 0.0, 0.375, 0.75, -0.125},
+                            {0.0, 0.0, 0.375, 0.75, -0.125},
+                            {0.0625, -0.3125, 0.9375, 0.3125, 0.0},
+                            {0.0, -0.125, 0.75, 0.375, 0.0},
+                            {0.375, -1.25, 1.875, 0.0, 0.0},
+                            {0.0, 0.0, 1.0, 0.0, 0.0}};
+    for (int j = 0; j < 5; j++) c[j] = t[f & 7u][j];
+  }
+  // adds lam * d(out)/dq to g
+  CFD_HD void bwd(const double*, double lam, double* g) const {
+    double c[5];
+    coef(flag, c);
+    for (int j = 0; j < 5; j++) g[j] += lam * c[j];
   }
 };
 
@@ -555,8 +607,9 @@ template <int AX> struct Roe2 {
 
 // ---- interface flux: cells w[0..5] (primitive), interface between w[2], w[3] ------
 // Forward (for checks) and reverse: lw[j][v] += (dF/dw[j][v])^T lF.
-template <int AX> CFD_HD inline void interface(const double w[6][4], double* F, double lw[6][4] = nullptr,
-                                               const double* lF = nullptr) {
+template <int AX, class IP = JS>
+CFD_HD inline void interface(const double w[6][4], double* F, double lw[6][4] = nullptr, const double* lF = nullptr,
+                             const IP& proto = IP()) {
   Roe2<AX> es;
   es.fwd(w[2], w[3]);
   double L[4][4], R[4][4];
@@ -566,11 +619,13 @@ template <int AX> CFD_HD inline void interface(const double w[6][4], double* F, 
   double ch[4][6];
   for (int k = 0; k < 4; k++)
     for (int j = 0; j < 6; j++) ch[k][j] = L[k][0] * c[j][0] + L[k][1] * c[j][1] + L[k][2] * c[j][2] + L[k][3] * c[j][3];
-  JS jl[4], jr[4];
+  IP jl[4], jr[4];
   double QL[4], QR[4];
   for (int k = 0; k < 4; k++) {
     double ql[5] = {ch[k][0], ch[k][1], ch[k][2], ch[k][3], ch[k][4]};
     double qr[5] = {ch[k][5], ch[k][4], ch[k][3], ch[k][2], ch[k][1]};
+    jl[k] = proto;
+    jr[k] = proto;
     jl[k].fwd(ql);
     jr[k].fwd(qr);
     QL[k] = jl[k].out;
@@ -678,6 +733,7 @@ struct IfaceFwd {
   double oL[4], oR[4];      // primitive states given to the flux
   bool fbL, fbR;            // positivity fallback taken
   int hb;                   // HLLC branch bits, or -1: decided by the reverse pass
+  unsigned tflag;           // TENO5 flags: bits 6k..6k+2 the left value of wave k, 6k+3.. the right
 };
 
 // HLLC branch bits in HLLCFlux2D2's own arithmetic (fluxSchemes.hpp:202-327),
@@ -709,7 +765,8 @@ template <int AX> CFD_HD inline int hllc_branch_bits(const double* Lp, const dou
 // Forward record in this file's own arithmetic (host checks).  The six
 // cells are read through w(j, v) (primitive) and w.cons(j, v) (the
 // conservative state rebuilt from it); the interface lies between cells 2 and 3.
-template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd& f) {
+template <int AX, class IP = JS, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd& f,
+                                                                           const IP& proto = IP()) {
   double pl[4], pr[4];
   for (int v = 0; v < 4; v++) {
     pl[v] = w(2, v);
@@ -718,17 +775,19 @@ template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd
   Roe2<AX> es;
   es.fwd(pl, pr);
   es.mats(f.L, f.R);
+  f.tflag = 0u;
   for (int k = 0; k < 4; k++) {
     double ch[6];
     for (int j = 0; j < 6; j++)
       ch[j] = f.L[k][0] * w.cons(j, 0) + f.L[k][1] * w.cons(j, 1) + f.L[k][2] * w.cons(j, 2) + f.L[k][3] * w.cons(j, 3);
     double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
     double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
-    JS a, b;
+    IP a = proto, b = proto;
     a.fwd(ql);
     b.fwd(qr);
     f.QL[k] = a.out;
     f.QR[k] = b.out;
+    if constexpr (IP::kLinear) f.tflag |= (a.flag | (b.flag << 3)) << (6 * k);
   }
   for (int i = 0; i < 4; i++) {
     f.cL[i] = f.R[i][0] * f.QL[0] + f.R[i][1] * f.QL[1] + f.R[i][2] * f.QL[2] + f.R[i][3] * f.QL[3];
@@ -754,8 +813,8 @@ template <int AX, class W> CFD_HD inline void iface_forward(const W& w, IfaceFwd
 // intermediates and projections are recomputed rather than held, the
 // eigenvector cotangents are folded into six scalars as they appear, and the
 // projection cotangents go straight through p2c's transpose wave by wave.
-template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF,
-                                                             double lw[6][4]) {
+template <int AX, class IP = JS, class W>
+CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF, double lw[6][4]) {
   CFD_CONTRACT
   double loL[4] = {0, 0, 0, 0}, loR[4] = {0, 0, 0, 0};
   {
@@ -799,22 +858,30 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
     for (int v = 0; v < 4; v++) lc[j][v] = 0.0;
   CFD_UNROLL
   for (int k = 0; k < 4; k++) {
-    double ch[6];
-    CFD_UNROLL
-    for (int j = 0; j < 6; j++)
-      ch[j] = f.L[k][0] * w.cons(j, 0) + f.L[k][1] * w.cons(j, 1) + f.L[k][2] * w.cons(j, 2) + f.L[k][3] * w.cons(j, 3);
-    double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
-    double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
     double gl[5] = {0, 0, 0, 0, 0}, gr[5] = {0, 0, 0, 0, 0};
-    {
-      JS a;
-      a.fwd(ql);
-      a.bwd(ql, lQL[k], gl);
-    }
-    {
-      JS b;
-      b.fwd(qr);
-      b.bwd(qr, lQR[k], gr);
+    if constexpr (IP::kLinear) {  // the recorded flags' stencils
+      IP a, b;
+      a.flag = (f.tflag >> (6 * k)) & 7u;
+      b.flag = (f.tflag >> (6 * k + 3)) & 7u;
+      a.bwd(nullptr, lQL[k], gl);
+      b.bwd(nullptr, lQR[k], gr);
+    } else {
+      double ch[6];
+      CFD_UNROLL
+      for (int j = 0; j < 6; j++)
+        ch[j] = f.L[k][0] * w.cons(j, 0) + f.L[k][1] * w.cons(j, 1) + f.L[k][2] * w.cons(j, 2) + f.L[k][3] * w.cons(j, 3);
+      double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
+      double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
+      {
+        IP a;
+        a.fwd(ql);
+        a.bwd(ql, lQL[k], gl);
+      }
+      {
+        IP b;
+        b.fwd(qr);
+        b.bwd(qr, lQR[k], gr);
+      }
     }
     double lch[6];
     lch[0] = gl[0];
@@ -839,10 +906,11 @@ template <int AX, class W> CFD_HD inline void iface_backward(const W& w, const I
 }
 
 // Same derivative as interface<AX>(w, F, lw, lF), from the two passes above.
-template <int AX, class W> CFD_HD inline void interface_vjp_lean(const W& w, const double* lF, double lw[6][4]) {
+template <int AX, class IP = JS, class W>
+CFD_HD inline void interface_vjp_lean(const W& w, const double* lF, double lw[6][4], const IP& proto = IP()) {
   IfaceFwd f;
-  iface_forward<AX>(w, f);
-  iface_backward<AX>(w, f, lF, lw);
+  iface_forward<AX>(w, f, proto);
+  iface_backward<AX, IP>(w, f, lF, lw);
 }
 
 }  // namespace cfdadj

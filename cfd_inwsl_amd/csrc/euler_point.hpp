@@ -165,6 +165,43 @@ __device__ __forceinline__ T teno_pick(unsigned flag, T q0, T q1, T q2, T q3, T 
   }
 }
 
+// Teno5_Z's cut-off flags (interScheme.hpp:384-452; ct = 1e-5, CT4 1e-4,
+// CT7 1e-7): bit k set where stencil k is cut.
+template <class T>
+__device__ __forceinline__ unsigned teno5_flag(T q0, T q1, T q2, T q3, T q4, T ct) {
+  T b0, b1, b2;
+  beta3(q0, q1, q2, q3, q4, b0, b1, b2);
+  T tau = fabs(b2 - b0);
+  T t0, t1, t2;
+  if (sizeof(T) == sizeof(double)) {
+    t0 = T(1.0) + fdv(tau, b0 + eps_tiny<T>());
+    t1 = T(1.0) + fdv(tau, b1 + eps_tiny<T>());
+    t2 = T(1.0) + fdv(tau, b2 + eps_tiny<T>());
+  } else {
+    // FP32: (1 + tau / beta)^6 overflows for tau / beta > ~2.6e6.  The
+    // cut-off test g_k < ct * sum(g) is invariant to a common factor, so
+    // the terms are scaled by min_k beta_k / tau when tau exceeds it.
+    T s0 = b0 + eps_tiny<T>(), s1 = b1 + eps_tiny<T>(), s2 = b2 + eps_tiny<T>();
+    T sm = smin(smin(s0, s1), s2);
+    T base = T(1.0);
+    T num = tau;
+    if (tau > sm) {
+      base = sm / tau;
+      num = sm;
+    }
+    t0 = base + num / s0;
+    t1 = base + num / s1;
+    t2 = base + num / s2;
+  }
+  t0 = t0 * t0;
+  t1 = t1 * t1;
+  t2 = t2 * t2;
+  T g0 = t0 * t0 * t0, g1 = t1 * t1 * t1, g2 = t2 * t2 * t2;
+  T CT = ct * (g0 + g1 + g2);
+  unsigned flag = (g0 < CT ? 1u : 0u) + (g1 < CT ? 2u : 0u) + (g2 < CT ? 4u : 0u);
+  return flag;
+}
+
 template <int SCH, class T>
 __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const ResearchConsts* rc = nullptr) {
   T b0, b1, b2;
@@ -207,36 +244,7 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
     T r = a0 * u0 + a1 * u1 + a2 * u2;
     return fdv(r, s);
   } else if (SCH == SCH_TENO) {
-    // Teno5_Z (interScheme.hpp:384-452); ct = 1e-5 (CT4: 1e-4, CT7: 1e-7)
-    T tau = fabs(b2 - b0);
-    T t0, t1, t2;
-    if (sizeof(T) == sizeof(double)) {
-      t0 = T(1.0) + fdv(tau, b0 + eps_tiny<T>());
-      t1 = T(1.0) + fdv(tau, b1 + eps_tiny<T>());
-      t2 = T(1.0) + fdv(tau, b2 + eps_tiny<T>());
-    } else {
-      // FP32: (1 + tau / beta)^6 overflows for tau / beta > ~2.6e6.  The
-      // cut-off test g_k < ct * sum(g) is invariant to a common factor, so
-      // the terms are scaled by min_k beta_k / tau when tau exceeds it.
-      T s0 = b0 + eps_tiny<T>(), s1 = b1 + eps_tiny<T>(), s2 = b2 + eps_tiny<T>();
-      T sm = smin(smin(s0, s1), s2);
-      T base = T(1.0);
-      T num = tau;
-      if (tau > sm) {
-        base = sm / tau;
-        num = sm;
-      }
-      t0 = base + num / s0;
-      t1 = base + num / s1;
-      t2 = base + num / s2;
-    }
-    t0 = t0 * t0;
-    t1 = t1 * t1;
-    t2 = t2 * t2;
-    T g0 = t0 * t0 * t0, g1 = t1 * t1 * t1, g2 = t2 * t2 * t2;
-    T CT = ct * (g0 + g1 + g2);
-    unsigned flag = (g0 < CT ? 1u : 0u) + (g1 < CT ? 2u : 0u) + (g2 < CT ? 4u : 0u);
-    return teno_pick(flag, q0, q1, q2, q3, q4);
+    return teno_pick(teno5_flag(q0, q1, q2, q3, q4, ct), q0, q1, q2, q3, q4);
   } else if (SCH == SCH_TENOA) {
     // new_whf_TCNS_A, TENO-A (interScheme.hpp:3863-3983): Teno5_Z's cut-off with
     // the adaptive constant 10^-beta_A, beta_A = floor(10 - 5 (1 - g)),

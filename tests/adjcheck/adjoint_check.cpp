@@ -7,7 +7,43 @@
 
 using namespace cfdadj;
 
+namespace {
+// the interface pass with WENO-JS (teno = 0) or TENO5 with cut-off ct
+template <int AX>
+void iface(int teno, double ct, const double w[6][4], double* F, double lw[6][4], const double* lF) {
+  if (teno) {
+    Teno t;
+    t.ct = ct;
+    interface<AX, Teno>(w, F, lw, lF, t);
+  } else {
+    interface<AX>(w, F, lw, lF);
+  }
+}
+}  // namespace
+
 extern "C" {
+
+void adj_interface_fwd_s(int ax, const double* w, double* F, int teno, double ct) {
+  double ww[6][4];
+  for (int j = 0; j < 6; j++)
+    for (int v = 0; v < 4; v++) ww[j][v] = w[4 * j + v];
+  if (ax == 0) iface<0>(teno, ct, ww, F, nullptr, nullptr);
+  else iface<1>(teno, ct, ww, F, nullptr, nullptr);
+}
+
+void adj_interface_vjp_s(int ax, const double* w, const double* lF, double* lw, int teno, double ct) {
+  double ww[6][4], lww[6][4];
+  for (int j = 0; j < 6; j++)
+    for (int v = 0; v < 4; v++) {
+      ww[j][v] = w[4 * j + v];
+      lww[j][v] = 0.0;
+    }
+  double F[4];
+  if (ax == 0) iface<0>(teno, ct, ww, F, lww, lF);
+  else iface<1>(teno, ct, ww, F, lww, lF);
+  for (int j = 0; j < 6; j++)
+    for (int v = 0; v < 4; v++) lw[4 * j + v] = lww[j][v];
+}
 
 void adj_interface_fwd(int ax, const double* w, double* F) {
   double ww[6][4];
@@ -90,15 +126,16 @@ void adj_interface_vjp_lean(int ax, const double* w, const double* lF, double* l
 }
 
 // Host statement of the 2D RHS vector-Jacobian product for characteristic-wise
-// WENO-JS + HLLC + MND6 (the map of oracle::rhs_eval): g += (d rhs/d U)^T mu.
+// WENO-JS (or TENO5, teno != 0) + HLLC + MND6 (the map of oracle::rhs_eval):
+// g += (d rhs/d U)^T mu.
 // Arrays AoS [cell][4], x fastest; bc per face (xlo, xhi, ylo, yhi);
 // bc_prim the Dirichlet ghost states (primitive).  Per direction and pencil:
 // rhs_i += (w1 (f_{i+1} - f_{i-1}) + w0 (G_{i+1} - G_i) + w2 (G_{i+2} - G_{i-1})) / h
 // with G_k the interface flux between cells k-1 and k (cells k-3 .. k+2),
 // so lambda_G_k = (w0 (mu_{k-1} - mu_k) + w2 (mu_{k-2} - mu_{k+1})) / h and
 // lambda_f_j = w1 (mu_{j-1} - mu_{j+1}) / h, mu = 0 outside the pencil.
-void adj_rhs_vjp(const int* n, const double* hgrid, const int* bc, const double* bc_prim, const double* U,
-                 const double* mu, double* g) {
+void adj_rhs_vjp_s(const int* n, const double* hgrid, const int* bc, const double* bc_prim, const double* U,
+                   const double* mu, double* g, int teno, double ct) {
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
   const int nx = n[0], ny = n[1];
   for (int ax = 0; ax < 2; ax++) {
@@ -134,8 +171,8 @@ void adj_rhs_vjp(const int* n, const double* hgrid, const int* bc, const double*
         for (int j = 0; j < 6; j++)
           for (int v = 0; v < 4; v++) ww[j][v] = w[(k - 3 + j + 4) * 4 + v];
         double F[4];
-        if (ax == 0) interface<0>(ww, F, lww, lG);
-        else interface<1>(ww, F, lww, lG);
+        if (ax == 0) iface<0>(teno, ct, ww, F, lww, lG);
+        else iface<1>(teno, ct, ww, F, lww, lG);
         for (int j = 0; j < 6; j++)
           for (int v = 0; v < 4; v++) lw[(k - 3 + j + 4) * 4 + v] += lww[j][v];
       }
@@ -158,6 +195,11 @@ void adj_rhs_vjp(const int* n, const double* hgrid, const int* bc, const double*
       for (int i = 0; i < nn; i++) c2p_vjp(&U[cell(i) * 4], &lw[(i + 4) * 4], &g[cell(i) * 4]);
     }
   }
+}
+
+void adj_rhs_vjp(const int* n, const double* hgrid, const int* bc, const double* bc_prim, const double* U,
+                 const double* mu, double* g) {
+  adj_rhs_vjp_s(n, hgrid, bc, bc_prim, U, mu, g, 0, 0.0);
 }
 
 }  // extern "C"

@@ -32,6 +32,7 @@ def lib():
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "adjcheck")], check=True)
     L = ctypes.CDLL(LIB)
     L.adj_rhs_vjp.argtypes = [PI, PD, PI, PD, PD, PD, PD]
+    L.adj_rhs_vjp_s.argtypes = [PI, PD, PI, PD, PD, PD, PD, ctypes.c_int, ctypes.c_double]
     L.adj_interface_vjp.argtypes = [ctypes.c_int, PD, PD, PD]
     L.adj_interface_vjp_lean.argtypes = [ctypes.c_int, PD, PD, PD]
     return L
@@ -41,7 +42,8 @@ def _p(a):
     return a.ctypes.data_as(PD)
 
 
-def host_vjp(lib, n, h, bc, bc_prim, U, mu):
+def host_vjp(lib, n, h, bc, bc_prim, U, mu, teno_ct=None):
+    """(d rhs / d U)^T mu on the host: WENO-JS, or TENO5 with cut-off teno_ct."""
     nn = (ctypes.c_int * 2)(*n)
     hh = (ctypes.c_double * 2)(*h)
     bb = (ctypes.c_int * 4)(*[int(b) for b in bc])
@@ -49,7 +51,10 @@ def host_vjp(lib, n, h, bc, bc_prim, U, mu):
     U = np.ascontiguousarray(U, dtype=np.float64)
     mu = np.ascontiguousarray(mu, dtype=np.float64)
     g = np.zeros_like(U)
-    lib.adj_rhs_vjp(nn, hh, bb, _p(bp), _p(U), _p(mu), _p(g))
+    if teno_ct is None:
+        lib.adj_rhs_vjp(nn, hh, bb, _p(bp), _p(U), _p(mu), _p(g))
+    else:
+        lib.adj_rhs_vjp_s(nn, hh, bb, _p(bp), _p(U), _p(mu), _p(g), 1, float(teno_ct))
     return g
 
 
@@ -117,20 +122,24 @@ BC_SETS = {
 }
 
 
+TENO_CT = {InterMethod.WCNS5: None, InterMethod.TCNS5: 1e-5, InterMethod.TCNS5CT4: 1e-4}
+
+
+@pytest.mark.parametrize("inter", list(TENO_CT))
 @pytest.mark.parametrize("name", list(BC_SETS))
-def test_host_rhs_vjp_matches_dual_oracle(lib, name):
+def test_host_rhs_vjp_matches_dual_oracle(lib, name, inter):
     bc, bc_prim = BC_SETS[name]
     n = (21, 17)
     h = (0.07, 0.09)
     cons = random_state(2, n, seed=23, jumps=True)
-    g = oracle_grid(2, n, h, InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, list(bc) + [BC.OUTFLOW] * 2,
-                    bc_prim)
+    g = oracle_grid(2, n, h, inter, FluxMethod.HLLC, Recon.CHAR, list(bc) + [BC.OUTFLOW] * 2, bc_prim)
     rng = np.random.default_rng(9)
     for _ in range(4):
         d = rng.normal(size=cons.shape)
         mu = rng.normal(size=cons.shape)
         _, jd = O.dual_rhs(g, cons, d)
-        gt = host_vjp(lib, n, h, bc, bc_prim if bc_prim is not None else np.zeros((6, 5)), cons, mu)
+        gt = host_vjp(lib, n, h, bc, bc_prim if bc_prim is not None else np.zeros((6, 5)), cons, mu,
+                      TENO_CT[inter])
         lhs, rhs = float((gt * d).sum()), float((mu * jd).sum())
         scale = np.abs(gt * d).sum()
         assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)

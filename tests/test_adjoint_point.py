@@ -23,6 +23,8 @@ def lib():
     pd = ctypes.POINTER(ctypes.c_double)
     L.adj_interface_fwd.argtypes = [ctypes.c_int, pd, pd]
     L.adj_interface_vjp.argtypes = [ctypes.c_int, pd, pd, pd]
+    L.adj_interface_fwd_s.argtypes = [ctypes.c_int, pd, pd, ctypes.c_int, ctypes.c_double]
+    L.adj_interface_vjp_s.argtypes = [ctypes.c_int, pd, pd, pd, ctypes.c_int, ctypes.c_double]
     L.adj_hllc_fwd.argtypes = [ctypes.c_int, pd, pd, pd]
     L.adj_hllc_vjp.argtypes = [ctypes.c_int, pd, pd, pd, pd, pd]
     L.adj_js_fwd.argtypes = [pd]
@@ -153,3 +155,40 @@ def test_interface_vjp(lib, ax, kind):
         # the forward model is the solver's: oracle recon (JS, char) + HLLC
         Lr, Rr = O.recon(2, O.S_JS, 1, w, ax)
         np.testing.assert_allclose(f(x), O.riemann(2, O.F_HLLC, Lr, Rr, ax), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("ax", [0, 1])
+@pytest.mark.parametrize("kind", ["smooth", "jump"])
+def test_interface_vjp_teno5(lib, ax, kind):
+    """The TENO5 interface pass: its interpolation is the flag's fixed
+    stencil (linear), so the gradient is the stencil's; against central
+    differences of the same forward (flags do not move under the 1e-7
+    step on these states), and the forward against the oracle's TENO5
+    reconstruction + HLLC."""
+    rng = np.random.default_rng(17 + ax)
+    cuts = 0
+    for trial in range(15):
+        w = rand_prim(rng, 6)
+        if kind == "smooth":
+            base = rand_prim(rng, 1)[0]
+            w = base + 0.05 * (w - w.mean(0))
+        else:
+            w[3:, 0] *= 0.3
+            w[3:, 3] *= 0.2
+        lF = rng.normal(size=4)
+        lw = np.zeros(24)
+        lib.adj_interface_vjp_s(ax, P(w), P(lF), lw.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, 1e-5)
+
+        def f(x):
+            F = np.zeros(4)
+            lib.adj_interface_fwd_s(ax, P(x), F.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, 1e-5)
+            return F
+
+        x = w.reshape(-1)
+        np.testing.assert_allclose(lw, fd_grad(f, x, lF, eps=1e-7), rtol=5e-5, atol=1e-6)
+        Lr, Rr = O.recon(2, O.S_TENO5, 1, w, ax)
+        np.testing.assert_allclose(f(x), O.riemann(2, O.F_HLLC, Lr, Rr, ax), rtol=1e-10, atol=1e-12)
+        Lj, _ = O.recon(2, O.S_JS, 1, w, ax)
+        cuts += not np.allclose(Lr, Lj, rtol=1e-3)
+    if kind == "jump":
+        assert cuts > 5  # TENO cuts stencils here: a JS pass would not match

@@ -29,24 +29,26 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _solver(n, h, bc, bc_prim=None):
-    return Solver(2, n, h, InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, list(bc) + [BC.OUTFLOW] * 2, bc_prim)
+def _solver(n, h, bc, bc_prim=None, inter=InterMethod.WCNS5):
+    return Solver(2, n, h, inter, FluxMethod.HLLC, Recon.CHAR, list(bc) + [BC.OUTFLOW] * 2, bc_prim)
 
 
+@pytest.mark.parametrize("inter", [InterMethod.WCNS5, InterMethod.TCNS5])
 @pytest.mark.parametrize("name", list(BC_SETS))
-def test_rhs_vjp_matches_host(lib, name):  # noqa: F811
+def test_rhs_vjp_matches_host(lib, name, inter):  # noqa: F811
     bc, bc_prim = BC_SETS[name]
+    teno_ct = 1e-5 if inter == InterMethod.TCNS5 else None
     for n, seed in (((40, 36), 3), ((130, 70), 4)):
         h = (0.07, 0.09)
         cons = random_state(2, n, seed=seed, jumps=True)
         mu = np.random.default_rng(seed).normal(size=cons.shape)
-        s = _solver(n, h, bc, bc_prim)
+        s = _solver(n, h, bc, bc_prim, inter)
         s.set_state(cons)
         g = s.rhs_vjp(mu)
         s.close()
-        ref = host_vjp(lib, n, h, bc, bc_prim if bc_prim is not None else np.zeros((6, 5)), cons, mu)
+        ref = host_vjp(lib, n, h, bc, bc_prim if bc_prim is not None else np.zeros((6, 5)), cons, mu, teno_ct)
         err = np.abs(g - ref).max() / np.abs(ref).max()
-        _say(f"rhs_vjp {name} {n}: max rel {err:.2e}")
+        _say(f"rhs_vjp {inter.name} {name} {n}: max rel {err:.2e}")
         assert err < 1e-12
 
 
@@ -99,9 +101,9 @@ def test_rhs_vjp_on_tied_predicates_matches_dual_oracle():
     assert worst < 1e-12
 
 
-def _gpu_gradient(case, U0, dt, nsteps, every):
+def _gpu_gradient(case, U0, dt, nsteps, every, inter=InterMethod.WCNS5):
     n = case["rho0"].shape[0]
-    s = _solver((n, n), (case["h"], case["h"]), [BC.PERIODIC] * 4)
+    s = _solver((n, n), (case["h"], case["h"]), [BC.PERIODIC] * 4, inter=inter)
     s.set_state(U0)
     s.adjoint_forward(dt, nsteps, every)
     UT = s.get_state()
@@ -111,12 +113,15 @@ def _gpu_gradient(case, U0, dt, nsteps, every):
     return J, UT, gU
 
 
-def test_rk3_adjoint_matches_dual_oracle_32_directions():
+@pytest.mark.parametrize("inter", [InterMethod.WCNS5, InterMethod.TCNS5])
+def test_rk3_adjoint_matches_dual_oracle_32_directions(inter):
+    """WENO-JS and TENO5 (whose stencil flags are branches of the forward,
+    recorded by the VJP's exact forward re-evaluation)."""
     n, nsteps = 64, 60
     case, U0 = _shock(n)
     dt = cases.cfl_dt_2d(U0, n, case["h"])
-    J, UT, gU = _gpu_gradient(case, U0, dt, nsteps, 20)
-    g = oracle_grid(2, (n, n), case["h"], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 4)
+    J, UT, gU = _gpu_gradient(case, U0, dt, nsteps, 20, inter)
+    g = oracle_grid(2, (n, n), case["h"], inter, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 4)
     grho = _grad_rho(case, gU)
     _, gJ = adjoint.window_pressure_objective(UT, case)
     worst = 0.0
@@ -126,7 +131,7 @@ def test_rk3_adjoint_matches_dual_oracle_32_directions():
         jvp = float((gJ * duT).sum())
         adj = float((grho * d).sum())
         worst = max(worst, abs(adj - jvp) / abs(jvp))
-    _say(f"adjoint vs dual oracle, {n}^2 x {nsteps} steps, 32 directions: worst rel {worst:.2e}")
+    _say(f"adjoint vs dual oracle ({inter.name}), {n}^2 x {nsteps} steps, 32 directions: worst rel {worst:.2e}")
     assert worst < 1e-9
 
 
@@ -212,3 +217,13 @@ def test_full_workload_against_central_differences():
     _say(f"adjoint vs central FD, 512^2 x 400 steps: median {np.median(rels):.2e} worst {max(rels):.2e}")
     assert np.median(rels) < 1e-4
     assert max(rels) < 2e-3
+
+
+def test_adjoint_refuses_schemes_it_does_not_differentiate():
+    """WENO-Z (and Roe, 3D, FP32, component-wise) have no reverse pass: the
+    entry points refuse them instead of returning a wrong gradient."""
+    s = _solver((32, 32), (0.1, 0.1), [BC.PERIODIC] * 4, inter=InterMethod.WCNSZ5)
+    s.set_state(random_state(2, (32, 32), seed=1))
+    with pytest.raises(Exception, match="WENO-JS\\) or TENO5"):
+        s.rhs_vjp(np.zeros((32 * 32, 4)))
+    s.close()
