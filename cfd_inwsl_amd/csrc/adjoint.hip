@@ -1,5 +1,8 @@
 // Discrete adjoint of the 2D right-hand side and of the SSP-RK3 step for the
-// characteristic-wise WCNS5 (WENO-JS) or TENO5 + HLLC + MND6 configurations, FP64.
+// characteristic-wise WCNS5 (WENO-JS) or TENO-family (TENO5, TENO-S, TENO-A,
+// the whf_* research variants) + HLLC + MND6 configurations, FP64.  The TENO
+// family's half-node value is the stencil its cut-off flag selects, so its
+// reverse pass is that stencil with the flag the forward took.
 //
 // Forward map (per direction, per pencil of n cells, G_k the interface flux
 // between cells k-1 and k built from cells k-3 .. k+2, f_j the node flux):
@@ -44,6 +47,7 @@
 #include "adjoint_point.hpp"
 #include "euler_point.hpp"
 #include "handle.hpp"
+#include "sweep.hpp"
 
 namespace {
 
@@ -85,7 +89,8 @@ struct VjpArgs {
   // component it negates (0: none)
   unsigned gx_src, gx_flip;
   double coef;
-  double ct;         // TENO5 cut-off constant
+  double ct;         // cut-off constant / law of the TENO-family scheme
+  const cfd::ResearchConsts* rc;  // constants of the research variants (device)
   long N;
   int n[2];
   int seg_len;
@@ -100,8 +105,9 @@ struct VjpArgs {
 // pressure clipping, shock factors, wave region -- is the branch the solver
 // took, also where a predicate is tied at rounding level (pressure
 // equilibrium makes p* - p_L vanish up to rounding).
-template <int AX, class IP, class W>
-__device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd& f, double ct) {
+template <int AX, class IP, int SCH, class W>
+__device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd& f, double ct,
+                                                    const cfd::ResearchConsts* rc) {
   double pL[4], pR[4];
   for (int v = 0; v < 4; v++) {
     pL[v] = w(2, v);
@@ -121,8 +127,8 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
     }
     for (int v = 0; v < 4; v++) f.L[k][v] = l[v];
     if constexpr (IP::kLinear) {
-      const unsigned fl = cfd::teno5_flag(ch[0], ch[1], ch[2], ch[3], ch[4], ct);
-      const unsigned fr = cfd::teno5_flag(ch[5], ch[4], ch[3], ch[2], ch[1], ct);
+      const unsigned fl = cfd::cut_flag<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, rc);
+      const unsigned fr = cfd::cut_flag<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct, rc);
       f.QL[k] = cfd::teno_pick(fl, ch[0], ch[1], ch[2], ch[3], ch[4]);
       f.QR[k] = cfd::teno_pick(fr, ch[5], ch[4], ch[3], ch[2], ch[1]);
       f.tflag |= (fl | (fr << 3)) << (6 * k);
@@ -163,7 +169,7 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
   f.hb = cfdadj::hllc_branch_bits<AX>(f.oL, f.oR);
 }
 
-template <int AX, int CA, int CP, class IP>
+template <int AX, int CA, int CP, class IP, int SCH>
 __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
   __shared__ double wc[8 * RC * CP];  // primitive state, then the conservative state rebuilt from it
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) * ih;
       const Cells cells(wc, k, lp);
       cfdadj::IfaceFwd f;
-      iface_forward_exact<AX, IP>(cells, f, a.ct);
+      iface_forward_exact<AX, IP, SCH>(cells, f, a.ct, a.rc);
       cfdadj::iface_backward<AX, IP>(cells, f, lG, lw);
     }
     for (int j = 0; j < 6; j++) {
@@ -443,9 +449,12 @@ void cfd_adjoint_release(cfd_handle* h) {
 }
 
 static int adj_supported(cfd_handle* h) {
-  if (h->D != 2 || h->fp32 || !h->cw || h->flx != 0 || (h->sch != cfd::SCH_JS && h->sch != cfd::SCH_TENO))
+  const bool teno_family = h->sch == cfd::SCH_TENO || h->sch == cfd::SCH_TENOS || h->sch == cfd::SCH_TENOA ||
+                           h->sch == cfd::SCH_TENOSA || (h->sch == cfd::SCH_WHF && (int)h->ct != 15);
+  if (h->D != 2 || h->fp32 || !h->cw || h->flx != 0 || (h->sch != cfd::SCH_JS && !teno_family))
     return cfd_fail(CFD_E_UNSUPP,
-                    "the adjoint covers 2D FP64 characteristic-wise WCNS5 (WENO-JS) or TENO5 with HLLC flux");
+                    "the adjoint covers 2D FP64 characteristic-wise WCNS5 (WENO-JS) or TENO-family schemes "
+                    "(TENO5, TENO-S, TENO-A, the whf_* variants) with HLLC flux");
   if (h->cfg.gravity[0] != 0.0 || h->cfg.gravity[1] != 0.0 || h->cfg.gravity[2] != 0.0)
     return cfd_fail(CFD_E_UNSUPP, "the adjoint has no source term");
   for (int f = 0; f < 4; f++)
@@ -516,6 +525,22 @@ static int adj_ensure(cfd_handle* h) {
   return 0;
 }
 
+// The VJP kernel of a scheme: WENO-JS's reverse pass, or the TENO family's
+// (the flag's stencil, the flag from the scheme's own cut-off).
+template <int AX, int CA, int CP>
+static void launch_vjp(int sch, dim3 grid, hipStream_t s, const VjpArgs& a) {
+  using cfdadj::JS;
+  using cfdadj::Teno;
+  switch (sch) {
+  case cfd::SCH_JS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, JS, cfd::SCH_JS>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENO: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENO>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENOS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOS>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENOA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOA>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENOSA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOSA>), grid, dim3(CA * CP), 0, s, a); break;
+  default: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_WHF>), grid, dim3(CA * CP), 0, s, a); break;
+  }
+}
+
 // OUT = base + coef * (d rhs / d U)^T MU at the state U, both directions;
 // base = init's combination (null init: the current OUT).
 static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef, double* OUT,
@@ -541,6 +566,7 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     }
     a.coef = coef;
     a.ct = h->ct;
+    a.rc = &h->aux[ax].rc;
     a.use_init = ax == 0 && init != nullptr;
     if (a.use_init) a.init = *init;
     a.N = h->N;
@@ -554,17 +580,14 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
     const unsigned nseg = (unsigned)((n + a.seg_len - 1) / a.seg_len);
-    const bool teno = h->sch == cfd::SCH_TENO;
     if (ax == 0) {
       constexpr int CA = VjpTile<0>::CA, CP = VjpTile<0>::CP;
       const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
-      if (teno) hipLaunchKernelGGL((k_rhs_vjp<0, CA, CP, cfdadj::Teno>), grid, dim3(CA * CP), 0, h->stream, a);
-      else hipLaunchKernelGGL((k_rhs_vjp<0, CA, CP, cfdadj::JS>), grid, dim3(CA * CP), 0, h->stream, a);
+      launch_vjp<0, CA, CP>(h->sch, grid, h->stream, a);
     } else {
       constexpr int CA = VjpTile<1>::CA, CP = VjpTile<1>::CP;
       const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
-      if (teno) hipLaunchKernelGGL((k_rhs_vjp<1, CA, CP, cfdadj::Teno>), grid, dim3(CA * CP), 0, h->stream, a);
-      else hipLaunchKernelGGL((k_rhs_vjp<1, CA, CP, cfdadj::JS>), grid, dim3(CA * CP), 0, h->stream, a);
+      launch_vjp<1, CA, CP>(h->sch, grid, h->stream, a);
       hipLaunchKernelGGL(k_ghost_fold_y, dim3((unsigned)((pencils * 8 + 255) / 256)), dim3(256), 0, h->stream, a);
     }
     AHIP(hipGetLastError());

@@ -202,49 +202,14 @@ __device__ __forceinline__ unsigned teno5_flag(T q0, T q1, T q2, T q3, T q4, T c
   return flag;
 }
 
+// Cut-off flag of the TENO-family schemes (bit k set where stencil k is
+// cut); the interpolated value is the flag's stencil (teno_pick).
 template <int SCH, class T>
-__device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const ResearchConsts* rc = nullptr) {
+__device__ __forceinline__ unsigned cut_flag(T q0, T q1, T q2, T q3, T q4, T ct, const ResearchConsts* rc) {
   T b0, b1, b2;
   beta3(q0, q1, q2, q3, q4, b0, b1, b2);
-  if (SCH == SCH_JS || SCH == SCH_Z) {
-    // weno5_JSchen (interScheme.hpp:17-43) / weno5_Z (:85-114)
-    T u0 = T(0.375) * q0 - T(1.25) * q1 + T(1.875) * q2;
-    T u1 = -T(0.125) * q1 + T(0.75) * q2 + T(0.375) * q3;
-    T u2 = T(0.375) * q2 + T(0.75) * q3 - T(0.125) * q4;
-    T a0, a1, a2;
-    if (SCH == SCH_JS) {
-      a0 = fdv(T(0.0625), sq(T(1e-6) + b0));
-      a1 = fdv(T(0.625), sq(T(1e-6) + b1));
-      a2 = fdv(T(0.3125), sq(T(1e-6) + b2));
-    } else if (sizeof(T) == sizeof(double)) {
-      T tau = fabs(b2 - b0);
-      a0 = T(0.0625) * (T(1.0) + sq(fdv(tau, b0 + eps_tiny<T>())));
-      a1 = T(0.625) * (T(1.0) + sq(fdv(tau, b1 + eps_tiny<T>())));
-      a2 = T(0.3125) * (T(1.0) + sq(fdv(tau, b2 + eps_tiny<T>())));
-    } else {
-      // FP32: (tau / beta)^2 overflows the float range once beta is ~19
-      // orders below tau, and inf / inf would follow.  The weights are
-      // invariant to a common factor, so they are scaled by (min_k beta_k /
-      // tau)^2 whenever tau exceeds the smallest beta: every term stays in
-      // [0, 2] and the normalised weights are the same real numbers.
-      T tau = fabs(b2 - b0);
-      T s0 = b0 + eps_tiny<T>(), s1 = b1 + eps_tiny<T>(), s2 = b2 + eps_tiny<T>();
-      T sm = smin(smin(s0, s1), s2);
-      T base = T(1.0);
-      T num = tau;
-      if (tau > sm) {
-        base = sq(sm / tau);
-        num = sm;
-      }
-      a0 = T(0.0625) * (base + sq(num / s0));
-      a1 = T(0.625) * (base + sq(num / s1));
-      a2 = T(0.3125) * (base + sq(num / s2));
-    }
-    T s = a0 + a1 + a2;
-    T r = a0 * u0 + a1 * u1 + a2 * u2;
-    return fdv(r, s);
-  } else if (SCH == SCH_TENO) {
-    return teno_pick(teno5_flag(q0, q1, q2, q3, q4, ct), q0, q1, q2, q3, q4);
+  if (SCH == SCH_TENO) {
+    return teno5_flag(q0, q1, q2, q3, q4, ct);
   } else if (SCH == SCH_TENOA) {
     // new_whf_TCNS_A, TENO-A (interScheme.hpp:3863-3983): Teno5_Z's cut-off with
     // the adaptive constant 10^-beta_A, beta_A = floor(10 - 5 (1 - g)),
@@ -286,7 +251,7 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
                 : betaA == 8 ? T(1e-8) : betaA == 9 ? T(1e-9) : T(1e-10);
     T rr = cta * (g0 + g1 + g2);
     unsigned flag = (g0 < rr ? 1u : 0u) + (g1 < rr ? 2u : 0u) + (g2 < rr ? 4u : 0u);
-    return teno_pick(flag, q0, q1, q2, q3, q4);
+    return flag;
   } else if (SCH == SCH_TENOSA) {
     // TENO-S with its constant C_T adapted to the smoothness ratios eta of
     // adjacent differences (those of TENO-A); ct selects the law:
@@ -343,20 +308,13 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
     T rr = CT * tau - CT_1 * bm;
     T ll = tau * bm;
     unsigned flag = (ll < rr * b0 ? 1u : 0u) + (ll < rr * b1 ? 2u : 0u) + (ll < rr * b2 ? 4u : 0u);
-    return teno_pick(flag, q0, q1, q2, q3, q4);
+    return flag;
   } else if (SCH == SCH_WHF) {
     // the whf_TcnsN schemes whose cut-off is not TENO-S's; ct = the
     // InterMethod value.  The value depends on the data through the flag
     // only, so the libm pow of the reference (here pow4 / pow6) decides bits
     // only where a ratio ties its threshold.
     const int var = (int)ct;
-    if (var == 15) {
-      // musclIn5 (:1386-1406): q2 + delta / 2, delta the minmod slope of q1, q2, q3
-      const T dm = q2 - q1, dp = q3 - q2;
-      const T delta = dp > T(0) ? smax(T(0.0), smax(smin(dm, dp), smin(dm, dp)))
-                                : smin(T(0.0), smin(smax(dm, dp), smax(dm, dp)));
-      return q2 + delta * T(0.5);
-    }
     T tau = fabs(b2 - b0);
     unsigned flag;
     if (var == 17 || var == 21 || var == 25) {
@@ -413,7 +371,7 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
       flag = (b0 * tau < (CTp * tau - c1 * b0) * bm ? 1u : 0u) + (b1 * tau < (CTp * tau - c1 * b1) * bm ? 2u : 0u) +
              (b2 * tau < (CTp * tau - c1 * b2) * bm ? 4u : 0u);
     }
-    return teno_pick(flag, q0, q1, q2, q3, q4);
+    return flag;
   } else {
     // Teno5_CongZ, TENO-S (interScheme.hpp:494-585); ct = its constant CT
     // (0.15704178024750198; Teno5_CongZCT4 / CT7, :586 / :661, differ in it only)
@@ -426,8 +384,60 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
     T rr = CT * tau - CT_1 * bm;
     T ll = tau * bm;
     unsigned flag = (ll < rr * b0 ? 1u : 0u) + (ll < rr * b1 ? 2u : 0u) + (ll < rr * b2 ? 4u : 0u);
-    return teno_pick(flag, q0, q1, q2, q3, q4);
+    return flag;
   }
+}
+
+template <int SCH, class T>
+__device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const ResearchConsts* rc = nullptr) {
+  T b0, b1, b2;
+  beta3(q0, q1, q2, q3, q4, b0, b1, b2);
+  if (SCH == SCH_JS || SCH == SCH_Z) {
+    // weno5_JSchen (interScheme.hpp:17-43) / weno5_Z (:85-114)
+    T u0 = T(0.375) * q0 - T(1.25) * q1 + T(1.875) * q2;
+    T u1 = -T(0.125) * q1 + T(0.75) * q2 + T(0.375) * q3;
+    T u2 = T(0.375) * q2 + T(0.75) * q3 - T(0.125) * q4;
+    T a0, a1, a2;
+    if (SCH == SCH_JS) {
+      a0 = fdv(T(0.0625), sq(T(1e-6) + b0));
+      a1 = fdv(T(0.625), sq(T(1e-6) + b1));
+      a2 = fdv(T(0.3125), sq(T(1e-6) + b2));
+    } else if (sizeof(T) == sizeof(double)) {
+      T tau = fabs(b2 - b0);
+      a0 = T(0.0625) * (T(1.0) + sq(fdv(tau, b0 + eps_tiny<T>())));
+      a1 = T(0.625) * (T(1.0) + sq(fdv(tau, b1 + eps_tiny<T>())));
+      a2 = T(0.3125) * (T(1.0) + sq(fdv(tau, b2 + eps_tiny<T>())));
+    } else {
+      // FP32: (tau / beta)^2 overflows the float range once beta is ~19
+      // orders below tau, and inf / inf would follow.  The weights are
+      // invariant to a common factor, so they are scaled by (min_k beta_k /
+      // tau)^2 whenever tau exceeds the smallest beta: every term stays in
+      // [0, 2] and the normalised weights are the same real numbers.
+      T tau = fabs(b2 - b0);
+      T s0 = b0 + eps_tiny<T>(), s1 = b1 + eps_tiny<T>(), s2 = b2 + eps_tiny<T>();
+      T sm = smin(smin(s0, s1), s2);
+      T base = T(1.0);
+      T num = tau;
+      if (tau > sm) {
+        base = sq(sm / tau);
+        num = sm;
+      }
+      a0 = T(0.0625) * (base + sq(num / s0));
+      a1 = T(0.625) * (base + sq(num / s1));
+      a2 = T(0.3125) * (base + sq(num / s2));
+    }
+    T s = a0 + a1 + a2;
+    T r = a0 * u0 + a1 * u1 + a2 * u2;
+    return fdv(r, s);
+  }
+  if (SCH == SCH_WHF && (int)ct == 15) {
+    // musclIn5 (interScheme.hpp:1386-1406): q2 + delta / 2, delta the minmod slope of q1, q2, q3
+    const T dm = q2 - q1, dp = q3 - q2;
+    const T delta = dp > T(0) ? smax(T(0.0), smax(smin(dm, dp), smin(dm, dp)))
+                              : smin(T(0.0), smin(smax(dm, dp), smax(dm, dp)));
+    return q2 + delta * T(0.5);
+  }
+  return teno_pick(cut_flag<SCH>(q0, q1, q2, q3, q4, ct, rc), q0, q1, q2, q3, q4);
 }
 
 // ---- conservative <-> primitive ----------------------------------------------

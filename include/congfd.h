@@ -222,11 +222,12 @@ int cfd_batch_synchronize(cfd_batch* b);
  * hand-written (the reference has no adjoint; the map differentiated is
  * SpDistributor::rhsSolve, zone_FD/src/src/sp_distributor.cpp:9-78, with
  * BlockSolver::RK3_SSP, zone_FD/src/src/blockSolver.cpp:43-87).  Covers 2D,
- * FP64, characteristic-wise WCNS5 (WENO-JS) or TENO5 (Teno5_Z, any cut-off
- * constant) + HLLC, any boundary codes; other configurations return
- * CFD_E_UNSUPP.  Branches (HLLC wave pattern, pressure clipping, positivity
- * fallback, TENO5 stencil flags) take the derivative of the branch the
- * forward evaluation takes.  Arrays are host AoS [cell][4]. */
+ * FP64, characteristic-wise WCNS5 (WENO-JS) or a TENO-family scheme (TENO5,
+ * TENO-S, TENO-A, the whf_* variants; not musclIn5) + HLLC, any boundary
+ * codes; other configurations return CFD_E_UNSUPP.  Branches (HLLC wave
+ * pattern, pressure clipping, positivity fallback, TENO stencil flags) take
+ * the derivative of the branch the forward evaluation takes.  Arrays are host
+ * AoS [cell][4]. */
 
 /* grad = (d rhs / d U)^T cot at the current state. */
 int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad);

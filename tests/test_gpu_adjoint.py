@@ -113,10 +113,13 @@ def _gpu_gradient(case, U0, dt, nsteps, every, inter=InterMethod.WCNS5):
     return J, UT, gU
 
 
-@pytest.mark.parametrize("inter", [InterMethod.WCNS5, InterMethod.TCNS5])
+@pytest.mark.parametrize("inter", [InterMethod.WCNS5, InterMethod.TCNS5, InterMethod.WCNS5CONGZ,
+                                   InterMethod.NEW_WHFTCNSA, InterMethod.WHFAITCNSNAS_1, InterMethod.WHFTCNSNLAD])
 def test_rk3_adjoint_matches_dual_oracle_32_directions(inter):
-    """WENO-JS and TENO5 (whose stencil flags are branches of the forward,
-    recorded by the VJP's exact forward re-evaluation)."""
+    """WENO-JS and the TENO family -- TENO5, TENO-S, TENO-A, an adaptive
+    TENO-S (whf_ai AS) and a chi-type whf scheme (LAD) -- whose stencil
+    flags are branches of the forward, recorded by the VJP's exact forward
+    re-evaluation."""
     n, nsteps = 64, 60
     case, U0 = _shock(n)
     dt = cases.cfl_dt_2d(U0, n, case["h"])
@@ -224,6 +227,6 @@ def test_adjoint_refuses_schemes_it_does_not_differentiate():
     entry points refuse them instead of returning a wrong gradient."""
     s = _solver((32, 32), (0.1, 0.1), [BC.PERIODIC] * 4, inter=InterMethod.WCNSZ5)
     s.set_state(random_state(2, (32, 32), seed=1))
-    with pytest.raises(Exception, match="WENO-JS\\) or TENO5"):
+    with pytest.raises(Exception, match="WENO-JS\\) or TENO-family"):
         s.rhs_vjp(np.zeros((32 * 32, 4)))
     s.close()
