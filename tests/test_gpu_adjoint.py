@@ -222,10 +222,13 @@ def test_full_workload_against_central_differences():
     assert max(rels) < 2e-3
 
 
-def test_adjoint_refuses_schemes_it_does_not_differentiate():
-    """WENO-Z (and Roe, 3D, FP32, component-wise) have no reverse pass: the
-    entry points refuse them instead of returning a wrong gradient."""
-    s = _solver((32, 32), (0.1, 0.1), [BC.PERIODIC] * 4, inter=InterMethod.WCNSZ5)
+@pytest.mark.parametrize("flux,inter", [(FluxMethod.HLLC, InterMethod.WCNSZ5), (FluxMethod.ROE, InterMethod.WCNS5),
+                                        (FluxMethod.HLLC, InterMethod.MUCSLIN5)])
+def test_adjoint_refuses_schemes_it_does_not_differentiate(flux, inter):
+    """WENO-Z, the Roe flux and musclIn5 (like 3D, FP32 and component-wise
+    reconstruction) have no reverse pass: the entry points refuse them
+    instead of returning a wrong gradient."""
+    s = Solver(2, (32, 32), (0.1, 0.1), inter, flux, Recon.CHAR, [BC.PERIODIC] * 6)
     s.set_state(random_state(2, (32, 32), seed=1))
     with pytest.raises(Exception, match="WENO-JS\\) or TENO-family"):
         s.rhs_vjp(np.zeros((32 * 32, 4)))
