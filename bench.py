@@ -251,7 +251,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(the path shards by slabs with a halo exchange; see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by slabs along the slowest axis, with a 4-layer halo exchange per "
+              "stage and a max all-reduce of the CFL speeds per step)", file=sys.stderr)
         sys.exit(2)
 
     cfl = 0.5

@@ -6,7 +6,7 @@ group.
     python -m cfd_inwsl_amd.profile_run --case periodic3d --n 256 --steps 2
     python -m cfd_inwsl_amd.profile_run --case vortex2d --n 2048 --copy 5
     python -m cfd_inwsl_amd.profile_run --case adjoint2d --n 512 --steps 40
-    python -m cfd_inwsl_amd.profile_run --summarize gpurun_out/x/run_results.db [...]
+    python -m cfd_inwsl_amd.profile_run --summarize <rocprofv3 output dir>/run_results.db [...]
 
 --autotune picks the launch plan by timing first (as bench.py does); the plan
 kept is printed, so its dispatches can be told apart by grid size.
