@@ -39,6 +39,10 @@ CASES = {
     "lax": dict(file="zone_FD/cases/lax sod tube    t=0.1400/TENO.cgns", case=2, t_end=0.14, output_dt=0.01),
     "double_rarefaction": dict(file="zone_FD/cases/ShuangXiShuBo   t=1.0000/TENO.cgns", case=5, t_end=1.0,
                                output_dt=0.01),
+    # reflecting walls (SYMMETRY1D); that run used CFL 0.1 (0.5 blows up, in
+    # the reference as here); its TENOS.cgns came from other settings
+    "woodward_colella": dict(file="zone_FD/cases/Woodward-Colella   t=0.0380/TENO.cgns", case=4, t_end=0.038,
+                             output_dt=0.038, cfl=0.1, tenos_pinned=False),
 }
 INITIAL = {
     "sod_101": dict(file="zone_FD/cases/Sod - whfzycTCNSNmyASF002_1 - 101 - t=0.0000.cgns", dim=1, case=0),
@@ -88,7 +92,8 @@ def main(root):
         zone, imax = zone_of(sol)
         for f in ("Density", "XVelocity", "Pressure"):
             arrays[f"{key}/{f}"] = sol[f]
-        meta["cases"][key] = dict(c, dim=1, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5)
+        meta["cases"][key] = dict(dict(cfl=0.5, tenos_pinned=True), **c, dim=1, zone=zone, imax=imax, inter=11,
+                                  flux="hllc")
         # the same run with TENO-S (Teno5_CongZ, InterMethod WCNS5CONGZ = 8)
         sol = cgns.read_solution(os.path.join(root, c["file"].replace("TENO.cgns", "TENOS.cgns")))
         for f in ("Density", "XVelocity", "Pressure"):

@@ -169,8 +169,9 @@ def _reference_fixtures():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("key,inter", [(k, i) for k in ("sod", "shu_osher", "lax", "double_rarefaction")
-                                       for i in (11, 8) if not (k == "shu_osher" and i == 8)])
+@pytest.mark.parametrize("key,inter", [(k, i) for k in ("sod", "shu_osher", "lax", "double_rarefaction",
+                                                        "woodward_colella")
+                                       for i in (11, 8) if not (k in ("shu_osher", "woodward_colella") and i == 8)])
 def test_driver_reproduces_reference_solution_files(tmp_path, key, inter):
     """The reference's own solution files (zone_FD/cases: TENO.cgns = Teno5_Z,
     inter 11; TENOS.cgns = Teno5_CongZ, inter 8; char-wise + HLLC, CFL 0.5):
@@ -214,7 +215,8 @@ def test_driver_reproduces_the_zyc_research_run(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("key,variant,inter", [(k, "tenoas", i) for k in ("sod", "shu_osher", "lax", "double_rarefaction")
+@pytest.mark.parametrize("key,variant,inter", [(k, "tenoas", i) for k in ("sod", "shu_osher", "lax", "double_rarefaction",
+                                                                          "woodward_colella")
                                                for i in (26, 27)] + [(k, "tenolad", 21) for k in ("shu_osher", "lax")])
 def test_driver_reproduces_reference_research_files(tmp_path, key, variant, inter):
     """TENOAS.cgns (whf_ai_TcnsN_AS_1 / _AS_2, inter 26 / 27) and

@@ -66,6 +66,8 @@ def test_oracle_reproduces_reference_tenos_end_states(key):
     rarefaction bit for bit; Shu-Osher within 1e-5 (4.9e-6 measured: that
     file was not reproduced exactly by any output interval tried)."""
     m = META["cases"][key]
+    if not m["tenos_pinned"]:
+        pytest.skip("the reference's TENOS file of this case came from settings not recorded")
     cons, t = _run_1d(m, m["output_dt"], O.S_TENOS)
     r, u, p = _prim(cons)
     want = ARR[f"{key}_tenos/Density"]
