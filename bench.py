@@ -43,7 +43,7 @@ def measured_traffic(dim):
     return entry["traffic_bytes_per_launch"] if entry else None
 
 
-def measure(sol, nv, dim, ncell, steps, warmup, cfl):
+def measure(sol, nv, dim, ncell, steps, warmup, cfl, traffic=True):
     tiles, segs = sol.autotune(reps=3)
     log(f"[bench] launch plan: tiles {tiles} segments {segs}")
     sol.rk3_steps_cfl(cfl, warmup, want_time=False)
@@ -62,7 +62,7 @@ def measure(sol, nv, dim, ncell, steps, warmup, cfl):
     bytes_per_launch = 24 * nv * ncell
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dim),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dim) if traffic else None,
             "kernel": "sweep_kernel", "sweep_ms": [round(float(x), 4) for x in ms],
             "bytes_per_launch": bytes_per_launch, "launch_plan": {"tiles": tiles, "segments": segs}}
     return value, sec * 1e3 / steps, roof
@@ -245,6 +245,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D 256^3 secondary figure")
+    ap.add_argument("--no-large", action="store_true", help="skip the 2D 4096^2 / 3D 384^3 figures")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-zones figure")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage figures")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint figures")
@@ -280,6 +281,34 @@ def main():
         secondary = {"workload": "3D periodic random-phase flow 256^3, TENO5 char-wise + HLLC, FP64",
                      "value": round(v3, 2), "unit": "Mcell-stage/s", "ms_per_step": round(ms3, 4),
                      "steps": st3, "roofline": roof3}
+
+    large = None
+    if not args.no_large:
+        large = {}
+        n = 4096
+        log(f"[bench] 2D vortex {n}^2 setup")
+        cons, h = cases.vortex(n)
+        sol = Solver(2, (n, n), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, [BC.PERIODIC] * 4)
+        sol.set_state(cons)
+        del cons
+        st = max(2, args.steps // 4)
+        v, ms, rf = measure(sol, 4, 2, n * n, st, max(1, args.warmup // 2), cfl, traffic=False)
+        sol.close()
+        log(f"[bench] 2D {n}^2: {v:.1f} Mcell-stage/s, {ms:.3f} ms/step, roofline {rf['frac']:.3f}")
+        large["vortex2d_4096"] = {"value": round(v, 2), "unit": "Mcell-stage/s", "ms_per_step": round(ms, 4),
+                                  "steps": st, "roofline": rf}
+        n3 = 384
+        log(f"[bench] 3D periodic {n3}^3 setup")
+        c3, h3 = cases.periodic3d(n3)
+        s3 = Solver(3, (n3, n3, n3), h3, InterMethod.TCNS5, FluxMethod.HLLC, Recon.CHAR, [BC.PERIODIC] * 6)
+        s3.set_state(c3)
+        del c3
+        st3 = max(2, args.steps // 8)
+        v3, ms3, rf3 = measure(s3, 5, 3, n3 ** 3, st3, 1, cfl, traffic=False)
+        s3.close()
+        log(f"[bench] 3D {n3}^3: {v3:.1f} Mcell-stage/s, {ms3:.3f} ms/step, roofline {rf3['frac']:.3f}")
+        large["periodic3d_384"] = {"value": round(v3, 2), "unit": "Mcell-stage/s", "ms_per_step": round(ms3, 4),
+                                   "steps": st3, "roofline": rf3}
 
     batch = None
     if not args.no_batch:
@@ -321,6 +350,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "secondary": secondary,
+        "large": large,
         "batch": batch,
         "fp32": fp32,
         "adjoint": adj,
