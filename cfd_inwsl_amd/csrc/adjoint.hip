@@ -18,11 +18,12 @@
 // Kernel: one workgroup owns CP pencils x one segment of cells and marches
 // along the pencil in chunks of CA interfaces.  Per chunk: load cells
 // (primitive from U, scaled cotangent, zeroed accumulator) into LDS rings;
-// each lane takes one interface through the reverse pass; its six cell
-// cotangents are added to the ring in six phases separated by barriers (lane
-// k adds to cell k-3+j in phase j, so no two lanes meet and the summation
-// order is fixed: results are deterministic); finally each lane finalises one
-// cell whose six interfaces are done.  Ghost-cell cotangents go to a small
+// each lane takes one interface through the reverse pass and stages its six
+// cell cotangents in LDS; after one barrier each lane adds the staged
+// contributions to its own cells in a fixed interface order (no two lanes
+// write one cell and the summation order is fixed: results are
+// deterministic); finally each lane finalises one cell whose six interfaces
+// are done.  Ghost-cell cotangents go to a small
 // per-pencil buffer; the x sweep's are folded onto their source cells by the
 // y sweep's cell phase (stream order makes them visible), the y sweep's by a
 // small kernel after it.
@@ -175,9 +176,12 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   __shared__ double wc[8 * RC * CP];  // primitive state, then the conservative state rebuilt from it
   __shared__ double mc[4 * RC * CP];  // coef * cotangent
   __shared__ double ac[4 * RC * CP];  // accumulated primitive cotangent
+  __shared__ double st[24 * CA * CP];  // the chunk's six cell cotangents per interface
   auto idx = [](int v, int slot, int p) -> int {
     return AX == 0 ? (v * CP + p) * RC + slot : (v * RC + slot) * CP + p;
   };
+  // (cell j of the interface, variable) x interface x pencil, laid out in lane order
+  auto sidx = [](int jv, int kk, int p) -> int { return jv * CA * CP + (AX == 0 ? p * CA + kk : kk * CP + p); };
   auto slot_of = [](int c) -> int { return (c + 8 * RC) % RC; };
 
   const int lane = threadIdx.x;
@@ -306,12 +310,37 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       iface_forward_exact<AX, IP, SCH>(cells, f, a.ct, a.rc);
       cfdadj::iface_backward<AX, IP>(cells, f, lG, lw);
     }
-    for (int j = 0; j < 6; j++) {
-      if (act) {
-        const int s = slot_of(k - 3 + j);
-        for (int v = 0; v < 4; v++) ac[idx(v, s, lp)] += lw[j][v];
-      }
-      __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < 6; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) st[sidx(j * 4 + v, la, lp)] = lw[j][v];
+    }
+    __syncthreads();
+    // Each lane adds the chunk's contributions to its own cells: cell k0-3+la
+    // (finalised below by the same lane) and, for la < 5, cell k0+CA-3+la
+    // (finalised by the same lane one chunk later).  Interfaces come in the
+    // order k = c+3, c+2, .., c-2, so every cell's sum has a fixed order.
+    if (pvalid) {
+      const int kend = min(k0 + CA - 1, kmax);
+      auto gather = [&](int c) {
+        const int s = slot_of(c);
+        double acc[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[v] = ac[idx(v, s, lp)];
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+          const int kk = c + 3 - j;
+          if (kk >= k0 && kk <= kend) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[v] += st[sidx(j * 4 + v, kk - k0, lp)];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) ac[idx(v, s, lp)] = acc[v];
+      };
+      gather(k0 - 3 + la);
+      if (la < 5) gather(k0 + CA - 3 + la);
     }
     if (fin) {
       const int c = cf;
