@@ -451,6 +451,7 @@ struct AdjointStore {
   int stage_cap = 0;                                   // states the stage buffer holds
   std::vector<double*> ckpt;                           // checkpointed states
   int nsteps = 0, every = 0;
+  bool last_in_store = false;  // the stage store holds the last segment (written by the forward)
   double dt = 0.0;
   int vjp_seg[2] = {0, 0};
 };
@@ -662,6 +663,7 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
   if (int rc = adj_ensure(h)) return rc;
   AdjointStore* s = h->adj;
   s->nsteps = 0;  // valid again only once this forward completes
+  s->last_in_store = false;
   const bool keep_all = checkpoint_every == 0;
   const int nck = keep_all ? 0 : (nsteps + checkpoint_every - 1) / checkpoint_every;
   while ((int)s->ckpt.size() > nck) {
@@ -701,7 +703,10 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
     AHIP(hipMemcpyAsync(h->U0, st + (size_t)(3 * nsteps) * nd, s->state_bytes, hipMemcpyDeviceToDevice,
                         h->stream));
   } else {
-    for (int st = 0; st < nsteps; st++) {
+    // the last segment runs through the stage store, so that the backward
+    // starts on it without recomputing it
+    const int last = (nck - 1) * checkpoint_every;
+    for (int st = 0; st < last; st++) {
       if (st % checkpoint_every == 0)
         AHIP(hipMemcpyAsync(s->ckpt[st / checkpoint_every], h->U0, s->state_bytes, hipMemcpyDeviceToDevice,
                             h->stream));
@@ -709,10 +714,23 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
       if ((rc = rk_stage(h, 2, h->Ua, h->U0, h->Ub, nullptr))) return rc;
       if ((rc = rk_stage(h, 3, h->Ub, h->U0, h->U0, nullptr))) return rc;
     }
+    const size_t nd = (size_t)h->N * 4;
+    double* st = s->stage;
+    AHIP(hipMemcpyAsync(s->ckpt[nck - 1], h->U0, s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+    AHIP(hipMemcpyAsync(st, h->U0, s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+    for (int i = 0; i < nsteps - last; i++) {
+      double *S0 = st + (size_t)(3 * i) * nd, *S1 = S0 + nd, *S2 = S1 + nd, *S3 = S2 + nd;
+      if ((rc = rk_stage(h, 1, S0, S0, S1, nullptr))) return rc;
+      if ((rc = rk_stage(h, 2, S1, S0, S2, nullptr))) return rc;
+      if ((rc = rk_stage(h, 3, S2, S0, S3, nullptr))) return rc;
+    }
+    AHIP(hipMemcpyAsync(h->U0, st + (size_t)(3 * (nsteps - last)) * nd, s->state_bytes, hipMemcpyDeviceToDevice,
+                        h->stream));
   }
   AHIP(hipGetLastError());
   AHIP(hipStreamSynchronize(h->stream));
   s->nsteps = nsteps;
+  s->last_in_store = !keep_all;
   return CFD_OK;
 }
 
@@ -737,12 +755,15 @@ int cfd_adjoint_backward(cfd_handle* h, const double* cot_final, double* grad_in
     for (int m = nck - 1; m >= 0; m--) {
       const int first = m * s->every;
       const int len = std::min(s->every, s->nsteps - first);
-      AHIP(hipMemcpyAsync(S(0), s->ckpt[m], s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
-      for (int i = 0; i < len; i++) {
-        if ((rc = rk_stage(h, 1, S(i), S(i), U1(i), nullptr))) return rc;
-        if ((rc = rk_stage(h, 2, U1(i), S(i), U2(i), nullptr))) return rc;
-        if (i + 1 < len)
-          if ((rc = rk_stage(h, 3, U2(i), S(i), S(i + 1), nullptr))) return rc;
+      if (!(m == nck - 1 && s->last_in_store)) {
+        s->last_in_store = false;  // the store now holds another segment
+        AHIP(hipMemcpyAsync(S(0), s->ckpt[m], s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+        for (int i = 0; i < len; i++) {
+          if ((rc = rk_stage(h, 1, S(i), S(i), U1(i), nullptr))) return rc;
+          if ((rc = rk_stage(h, 2, U1(i), S(i), U2(i), nullptr))) return rc;
+          if (i + 1 < len)
+            if ((rc = rk_stage(h, 3, U2(i), S(i), S(i + 1), nullptr))) return rc;
+        }
       }
       for (int i = len - 1; i >= 0; i--)
         if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt))) return rc;

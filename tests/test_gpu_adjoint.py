@@ -147,6 +147,25 @@ def test_checkpoint_interval_does_not_change_the_gradient():
         assert np.array_equal(_gpu_gradient(case, U0, dt, nsteps, every)[2], ref), every
 
 
+@pytest.mark.parametrize("every", [1, 7, 25, 0])
+def test_backward_repeats_after_one_forward(every):
+    """The forward leaves its last segment in the stage store and the first
+    backward starts on it; a second backward on the same forward (the store
+    now holding the first segment) recomputes it and gives the same bits."""
+    n, nsteps = 48, 25
+    case, U0 = _shock(n)
+    dt = cases.cfl_dt_2d(U0, n, case["h"])
+    s = _solver((n, n), (case["h"], case["h"]), [BC.PERIODIC] * 4)
+    s.set_state(U0)
+    s.adjoint_forward(dt, nsteps, every)
+    _, gJ = adjoint.window_pressure_objective(s.get_state(), case)
+    g1 = s.adjoint_backward(gJ)
+    g2 = s.adjoint_backward(gJ)
+    s.close()
+    assert np.array_equal(g1, g2)
+    assert np.array_equal(g1, _gpu_gradient(case, U0, dt, nsteps, 1)[2])
+
+
 def test_torch_harness_equals_direct_call():
     torch = pytest.importorskip("torch")
     n, nsteps = 48, 12
