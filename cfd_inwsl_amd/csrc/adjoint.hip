@@ -412,6 +412,14 @@ __global__ __launch_bounds__(256) void k_ghost_fold_y(VjpArgs a) {
     const cfdadj::GhostSrc o = src_of(q);
     if (!o.fixed && o.src == gs.src) return;
   }
+  // the source cell's state and output first: their loads overlap the ghost reads
+  const long i = gp + (long)gs.src * nx;
+  double u[4], o4[4], gr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    u[v] = a.U[v * a.N + i];
+    o4[v] = a.OUT[v * a.N + i];
+  }
   double l[4] = {0.0, 0.0, 0.0, 0.0};
   for (int q = g; q < 8; q++) {
     const cfdadj::GhostSrc o = src_of(q);
@@ -421,11 +429,9 @@ __global__ __launch_bounds__(256) void k_ghost_fold_y(VjpArgs a) {
       l[v] += v == o.flip ? -x : x;
     }
   }
-  const long i = gp + (long)gs.src * nx;
-  double u[4], gr[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int v = 0; v < 4; v++) u[v] = a.U[v * a.N + i];
   cfdadj::c2p_vjp(u, l, gr);
-  for (int v = 0; v < 4; v++) a.OUT[v * a.N + i] += gr[v];
+#pragma unroll
+  for (int v = 0; v < 4; v++) a.OUT[v * a.N + i] = o4[v] + gr[v];
 }
 
 __global__ void k_aos_to_soa4(const double* __restrict__ in, double* __restrict__ out, long N) {
