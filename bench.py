@@ -217,25 +217,62 @@ def bench_adjoint(n=512, nsteps=400, every=20, ndir=32):
     return out
 
 
-def cpu_baseline(seconds_target=15.0):
+def _host_cpu():
+    """CPU model string and logical CPU count of the host (Linux)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: the CPUs this process may run on
+    (affinity mask), capped by OMP_NUM_THREADS when the host sets it (a
+    shared host gives one GPU's job a fixed CPU share that way)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(avail, env) if env > 0 else avail
+
+
+def cpu_baseline(seconds_target=12.0):
     """In-repo oracle (OpenMP over pencils, like SpDistributor::rhsSolve) timed
-    on the host cores on a bounded sample of the same workload."""
+    on the host cores, threads pinned one per core, on bounded samples of
+    both headline workloads (2D vortex 2048^2 and 3D periodic 256^3)."""
+    threads = cpu_threads()
+    # pin before the OpenMP runtime starts (the oracle library loads it)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    n = 2048
-    cons, h = cases.vortex(n)
-    grid = O.make_grid(2, (n, n), h, O.S_Z, O.F_ROE, 1, [1] * 4)
-    # one step first to size the sample
-    t0 = time.perf_counter()
-    cur, _ = O.rk3(grid, cons, cfl=0.5, nsteps=1, nthreads=threads)
-    one = time.perf_counter() - t0
-    steps = max(1, min(8, int(seconds_target / max(one, 1e-3)) - 1))
-    t0 = time.perf_counter()
-    O.rk3(grid, cur, cfl=0.5, nsteps=steps, nthreads=threads)
-    sec = time.perf_counter() - t0
-    val = n * n * 3 * steps / sec / 1e6
-    return {"value": round(val, 3), "unit": "Mcell-stage/s", "cores": threads, "kind": "port",
-            "sample": f"vortex 2048^2 WCNS5-Z char + Roe, {steps} CFL RK3 steps after 1 untimed"}
+    model, ncpu = _host_cpu()
+
+    def timed(dim, n, mk, sch, flx):
+        cons, h = mk(n)
+        grid = O.make_grid(dim, (n,) * dim, h, sch, flx, 1, [1] * (2 * dim))
+        t0 = time.perf_counter()
+        cur, _ = O.rk3(grid, cons, cfl=0.5, nsteps=1, nthreads=threads)
+        one = time.perf_counter() - t0
+        steps = max(1, min(8, int(seconds_target / max(one, 1e-3)) - 1))
+        t0 = time.perf_counter()
+        O.rk3(grid, cur, cfl=0.5, nsteps=steps, nthreads=threads)
+        sec = time.perf_counter() - t0
+        return n ** dim * 3 * steps / sec / 1e6, steps
+
+    v2, st2 = timed(2, 2048, cases.vortex, O.S_Z, O.F_ROE)
+    log(f"[bench] CPU 2D: {v2:.2f} Mcell-stage/s")
+    v3, st3 = timed(3, 256, cases.periodic3d, O.S_TENO5, O.F_HLLC)
+    log(f"[bench] CPU 3D: {v3:.2f} Mcell-stage/s")
+    return {"value": round(v2, 3), "unit": "Mcell-stage/s", "cores": threads, "kind": "port",
+            "sample": f"vortex 2048^2 WCNS5-Z char + Roe, {st2} CFL RK3 steps after 1 untimed",
+            "host_cpu": model, "host_logical_cpus": ncpu,
+            "threads_note": "one OpenMP thread per available CPU (affinity mask, capped by OMP_NUM_THREADS), "
+                            "OMP_PROC_BIND=close, OMP_PLACES=cores",
+            "secondary": {"value": round(v3, 3), "unit": "Mcell-stage/s",
+                          "sample": f"periodic 256^3 TENO5 char + HLLC, {st3} CFL RK3 steps after 1 untimed"}}
 
 
 def main():
@@ -258,6 +295,7 @@ def main():
 
     cfl = 0.5
     n = 2048
+    head_cells = n * n
     log(f"[bench] 2D vortex {n}^2 setup")
     cons, h = cases.vortex(n)
     sol = Solver(2, (n, n), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, [BC.PERIODIC] * 4)
@@ -285,16 +323,16 @@ def main():
     large = None
     if not args.no_large:
         large = {}
-        n = 4096
-        log(f"[bench] 2D vortex {n}^2 setup")
-        cons, h = cases.vortex(n)
-        sol = Solver(2, (n, n), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, [BC.PERIODIC] * 4)
+        nl = 4096
+        log(f"[bench] 2D vortex {nl}^2 setup")
+        cons, h = cases.vortex(nl)
+        sol = Solver(2, (nl, nl), h, InterMethod.WCNSZ5, FluxMethod.ROE, Recon.CHAR, [BC.PERIODIC] * 4)
         sol.set_state(cons)
         del cons
         st = max(2, args.steps // 4)
-        v, ms, rf = measure(sol, 4, 2, n * n, st, max(1, args.warmup // 2), cfl, traffic=False)
+        v, ms, rf = measure(sol, 4, 2, nl * nl, st, max(1, args.warmup // 2), cfl, traffic=False)
         sol.close()
-        log(f"[bench] 2D {n}^2: {v:.1f} Mcell-stage/s, {ms:.3f} ms/step, roofline {rf['frac']:.3f}")
+        log(f"[bench] 2D {nl}^2: {v:.1f} Mcell-stage/s, {ms:.3f} ms/step, roofline {rf['frac']:.3f}")
         large["vortex2d_4096"] = {"value": round(v, 2), "unit": "Mcell-stage/s", "ms_per_step": round(ms, 4),
                                   "steps": st, "roofline": rf}
         n3 = 384
@@ -346,7 +384,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2D isentropic vortex 2048^2 periodic, WCNS5-Z char-wise + Roe + MND6, "
                                "SSP-RK3 with device CFL dt (CFL 0.5)",
-                   "cells": n * n, "parallelism": "single GPU"},
+                   "cells": head_cells, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "secondary": secondary,

@@ -25,6 +25,11 @@ using namespace cfd;
 // error slot and scheme table of the single-zone entry points (capi.hip)
 int cfd_set_error_from_batch(int code, const char* msg);
 int cfd_select_scheme(int inter, int dim, int* sch, double* ct);
+int cfd_check_faces(int dim, const int* n, const int* bc, std::string* msg);
+
+// Time bookkeeping of BlockSolver::RK3_SSP + solve (blockSolver.cpp:62,74,86,
+// 123), as the single-zone clock: t + dt - dt/2 - dt/2 + dt, operation by operation.
+__device__ __forceinline__ double zone_step_time(double t, double dt) { return ((t + dt) - dt / 2 - dt / 2) + dt; }
 
 #define BCHK(expr)                                                                                 \
   do {                                                                                             \
@@ -115,7 +120,7 @@ template <int D> __global__ void kb_finish_dt(double* zscal, double* dt, int nz,
   for (int d = 0; d < D; d++) lam += zscal[8 * z + 1 + d];
   double v = cfl / lam;
   zscal[8 * z] = v;
-  zscal[8 * z + 4] += v;
+  zscal[8 * z + 4] = zone_step_time(zscal[8 * z + 4], v);
   dt[z] = v;
 }
 
@@ -315,6 +320,8 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
   if (c->diff_method != CFD_DIFF_MND6) return cfd_set_error_from_batch(CFD_E_UNSUPP, "only DiffMethod MND6 (3) is implemented");
   if (c->flux_method != CFD_FLUX_HLLC && c->flux_method != CFD_FLUX_ROE)
     return cfd_set_error_from_batch(CFD_E_ARG, "flux_method must be HLLC (0) or ROE (1)");
+  if (c->recon != CFD_RECON_CHAR && c->recon != CFD_RECON_COMPONENT)
+    return cfd_set_error_from_batch(CFD_E_ARG, "recon must be CFD_RECON_CHAR or CFD_RECON_COMPONENT");
   int sch;
   double ct;
   if (c->precision != CFD_PRECISION_FP64) return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches run in FP64");
@@ -349,6 +356,13 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
         return cfd_set_error_from_batch(CFD_E_ARG, ("zone " + std::to_string(z) + ": bad n/h").c_str());
       }
       zd.seg_len[d] = zd.n[d];
+    }
+    {
+      std::string msg;
+      if (cfd_check_faces(c->dim, zd.n, zones[z].bc, &msg)) {
+        delete b;
+        return cfd_set_error_from_batch(CFD_E_ARG, ("zone " + std::to_string(z) + ": " + msg).c_str());
+      }
     }
     for (int d = 0; d < 3; d++) fill_research_consts(zd.aux[d].rc);
     for (int f = 0; f < 6; f++) {
@@ -475,9 +489,10 @@ int cfd_batch_times(cfd_batch* b, double* t_out) {
 
 int cfd_batch_autotune(cfd_batch* b, int reps) {
   if (!b || reps < 1) return cfd_set_error_from_batch(CFD_E_ARG, "bad argument");
-  // state is restored afterwards: work on a copy of U0
-  std::vector<double> save((size_t)b->total);
+  // state and zone clocks are restored afterwards
+  std::vector<double> save((size_t)b->total), zsave((size_t)8 * b->nz);
   BCHK(hipMemcpy(save.data(), b->U0, sizeof(double) * b->total, hipMemcpyDeviceToHost));
+  BCHK(hipMemcpy(zsave.data(), b->zscal, sizeof(double) * zsave.size(), hipMemcpyDeviceToHost));
   hipEvent_t e0, e1;
   BCHK(hipEventCreate(&e0));
   BCHK(hipEventCreate(&e1));
@@ -516,7 +531,7 @@ int cfd_batch_autotune(cfd_batch* b, int reps) {
   b->persist_blocks = bg;
   int rc = plan_zones(b);
   if (rc) return rc;
-  BCHK(hipMemset(b->zscal, 0, sizeof(double) * 8 * b->nz));
+  BCHK(hipMemcpy(b->zscal, zsave.data(), sizeof(double) * zsave.size(), hipMemcpyHostToDevice));
   return CFD_OK;
 }
 

@@ -90,6 +90,30 @@ static bool bc_known(int b) {
   return false;
 }
 
+// Face codes of a zone of n[0..dim) cells (single zones and every zone of a
+// batch): known codes, the double-Mach faces on the y faces of a 2D grid only,
+// and at least 4 cells along an axis with a mirroring face (the 4-cell ghost
+// layer of a mirror reads cells 0..3 of the zone).  0 or CFD_E_ARG with *msg set.
+int cfd_check_faces(int dim, const int* n, const int* bc, std::string* msg) {
+  for (int d = 0; d < dim; d++)
+    for (int s = 0; s < 2; s++) {
+      const int b = bc[2 * d + s];
+      if (!bc_known(b)) {
+        *msg = "unknown boundary code " + std::to_string(b);
+        return CFD_E_ARG;
+      }
+      if ((b == BC_DMR_TOP && !(dim == 2 && d == 1 && s == 1)) || (b == BC_DMR_WALL && !(dim == 2 && d == 1 && s == 0))) {
+        *msg = "the double-Mach boundaries belong on the y faces of a 2D grid (DoubleMachUp on y-high, the wall on y-low)";
+        return CFD_E_ARG;
+      }
+      if (b != BC_PERIODIC && b != BC_OUTFLOW && b != BC_DIRICHLET && n[d] < 4) {
+        *msg = "mirror boundaries need at least 4 cells along the axis";
+        return CFD_E_ARG;
+      }
+    }
+  return 0;
+}
+
 // ---- small kernels ---------------------------------------------------------------
 __global__ void k_set_scalar(double* p, double v) { *p = v; }
 
@@ -450,19 +474,11 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
     int want_active = d < cfg->dim;
     if (cfg->n[d] < 1) return cfd_fail(CFD_E_ARG, "n[] must be >= 1");
     if (!want_active && cfg->n[d] != 1) return cfd_fail(CFD_E_ARG, "n[d] must be 1 beyond dim");
-    if (want_active) {
-      if (!(cfg->h[d] > 0)) return cfd_fail(CFD_E_ARG, "h[d] must be > 0");
-      for (int s = 0; s < 2; s++) {
-        int b = cfg->bc[2 * d + s];
-        if (!bc_known(b)) return cfd_fail(CFD_E_ARG, "unknown boundary code " + std::to_string(b));
-        if ((b == BC_DMR_TOP && !(cfg->dim == 2 && d == 1 && s == 1)) ||
-            (b == BC_DMR_WALL && !(cfg->dim == 2 && d == 1 && s == 0)))
-          return cfd_fail(CFD_E_ARG, "the double-Mach boundaries belong on the y faces of a 2D grid "
-                                     "(DoubleMachUp on y-high, the wall on y-low)");
-        if (b != BC_PERIODIC && b != BC_OUTFLOW && b != BC_DIRICHLET && cfg->n[d] < 4)
-          return cfd_fail(CFD_E_ARG, "mirror boundaries need at least 4 cells along the axis");
-      }
-    }
+    if (want_active && !(cfg->h[d] > 0)) return cfd_fail(CFD_E_ARG, "h[d] must be > 0");
+  }
+  {
+    std::string msg;
+    if (cfd_check_faces(cfg->dim, cfg->n, cfg->bc, &msg)) return cfd_fail(CFD_E_ARG, msg);
   }
   cfd_handle* h = new cfd_handle();
   h->cfg = *cfg;
@@ -883,6 +899,9 @@ int cfd_get_time(cfd_handle* h, double* t) {
 
 int cfd_device_buffers(cfd_handle* h, double** state, double** rhs) {
   if (!h) return cfd_fail(CFD_E_ARG, "null handle");
+  // an FP32 handle keeps R in floats and sweeps an FP32 shadow of the state,
+  // so the FP64 planes promised here do not exist for it
+  if (h->fp32) return cfd_fail(CFD_E_UNSUPP, "cfd_device_buffers needs an FP64 handle");
   if (state) *state = h->U0;
   if (rhs) *rhs = static_cast<double*>(h->R);
   return CFD_OK;

@@ -251,7 +251,9 @@ int cfd_set_time(cfd_handle* h, double t);
 int cfd_get_time(cfd_handle* h, double* t);
 
 /* Device pointer of the current state (structure of arrays, nv planes of
- * n[0]*n[1]*n[2] doubles) and of the rhs buffer. */
+ * n[0]*n[1]*n[2] doubles) and of the rhs buffer.  FP64 handles only: an FP32
+ * handle (float rhs, sweeps reading an FP32 shadow of the state) returns
+ * CFD_E_UNSUPP. */
 int cfd_device_buffers(cfd_handle* h, double** state, double** rhs);
 
 #ifdef __cplusplus

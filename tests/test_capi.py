@@ -79,3 +79,47 @@ def test_fp32_rejected_for_1d_and_batches():
     assert "FP32" in lib.cfd_last_error().decode()
     cfg.precision = 7
     assert lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h)) < 0
+
+
+def _batch_cfg():
+    cfg = solver.cfd_config()
+    cfg.dim = 2
+    cfg.inter_method, cfg.flux_method, cfg.diff_method, cfg.recon = 2, 0, 3, 0
+    return cfg
+
+
+def _zones(specs):
+    arr = (solver.cfd_zone * len(specs))()
+    for i, (n, bc) in enumerate(specs):
+        arr[i].n[0], arr[i].n[1], arr[i].n[2] = n[0], n[1], 1
+        arr[i].h[0] = arr[i].h[1] = arr[i].h[2] = 0.1
+        for f in range(6):
+            arr[i].bc[f] = bc[f]
+    return arr
+
+
+@pytest.mark.parametrize("specs,msg", [
+    # a mirroring face on an axis of fewer than 4 cells (its ghosts would read outside the zone)
+    ([((16, 16), [7] * 6), ((3, 16), [2, 7, 7, 7, 7, 7])], "zone 1: mirror boundaries need at least 4 cells"),
+    ([((16, 16), [7, 7, 102, 7, 7, 7]), ((16, 2), [7, 7, 7, 11, 7, 7])], "zone 1: mirror boundaries"),
+    ([((16, 16), [7, 7, 7, 99, 7, 7])], "zone 0: unknown boundary code 99"),
+    ([((16, 16), [8, 7, 7, 7, 7, 7])], "double-Mach boundaries belong on the y faces"),
+])
+def test_batch_rejects_bad_faces_before_any_gpu_call(specs, msg):
+    lib = solver.load_library()
+    cfg = _batch_cfg()
+    arr = _zones(specs)
+    b = ctypes.c_void_p()
+    rc = lib.cfd_batch_create(ctypes.byref(cfg), len(specs), arr, ctypes.byref(b))
+    assert rc == -1
+    assert msg in lib.cfd_last_error().decode()
+    assert not b.value
+
+
+def test_batch_rejects_bad_recon():
+    lib = solver.load_library()
+    cfg = _batch_cfg()
+    cfg.recon = 5
+    b = ctypes.c_void_p()
+    assert lib.cfd_batch_create(ctypes.byref(cfg), 1, _zones([((16, 16), [7] * 6)]), ctypes.byref(b)) == -1
+    assert "recon" in lib.cfd_last_error().decode()

@@ -127,3 +127,78 @@ def test_batch_research_constants_and_whf_refusal():
         s.close()
     with pytest.raises(Exception, match="single zones only"):
         Batch(2, [z for z, _ in zs], InterMethod.WHFTCNSNLAD, FluxMethod.HLLC, Recon.CHAR)
+
+
+def _config_b_zones(nzones, lo, hi, seed):
+    """Config B's generator (the benchmark's): per-side sizes uniform in
+    [lo, hi], seeded four-quadrant states, outflow faces."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, size=(nzones, 2))
+    out = []
+    for i, (nx, ny) in enumerate(sizes):
+        cons, h = cases.riemann2d_quadrants(int(nx), int(ny), seed=seed + i)
+        out.append(({"n": (int(nx), int(ny)), "h": h, "bc": [BC.OUTFLOW] * 6}, cons))
+    return out
+
+
+def _single_zone_run(z, c, steps):
+    s = Solver(2, z["n"], z["h"], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, z["bc"])
+    s.set_state(c)
+    t = s.rk3_steps_cfl(0.5, steps)
+    out = s.get_state()
+    s.close()
+    return out, t
+
+
+def test_config_b_wcns5_hllc_uneven_zones_128_512():
+    """Config B's scheme (WCNS5 char-wise + HLLC) batched over 12 uneven
+    four-quadrant zones with sides in [128, 512]: every zone equals its
+    single-zone run bit for bit (state and clock), after an autotune, in the
+    graph-replayed persistent mode; the two smallest zones equal the oracle."""
+    zs = _config_b_zones(12, 128, 512, seed=4242)
+    print("config B zones", [z["n"] for z, _ in zs], flush=True)
+    steps = 2
+    b = Batch(2, [z for z, _ in zs], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR)
+    for i, (_, c) in enumerate(zs):
+        b.set_state(i, c)
+    b.autotune(1)
+    b.steps_cfl(0.5, steps, BATCH_GRAPH)
+    got = [b.get_state(i) for i in range(len(zs))]
+    times = b.times()
+    b.close()
+    for i, (z, c) in enumerate(zs):
+        single, t = _single_zone_run(z, c, steps)
+        assert np.array_equal(got[i], single), i
+        assert times[i] == t, (i, times[i], t)
+    small = sorted(range(len(zs)), key=lambda i: zs[i][0]["n"][0] * zs[i][0]["n"][1])[:2]
+    for i in small:
+        z, c = zs[i]
+        ref, tref = O.rk3(oracle_grid(2, z["n"], z["h"], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, z["bc"]),
+                          c, cfl=0.5, nsteps=steps)
+        print(f"config B zone {i} {z['n']} vs oracle: bitwise={np.array_equal(got[i], ref)}", flush=True)
+        assert np.array_equal(got[i], ref), i
+        assert times[i] == tref, i
+
+
+def test_config_b_96_zones_128_1024():
+    """Config B at its stated size: 96 zones, sides uniform in [128, 1024]
+    (the benchmark's workload, about 30 M cells), two graph-replayed steps;
+    a sample of zones (the largest, the smallest and six more) equals its
+    single-zone run bit for bit."""
+    zs = _config_b_zones(96, 128, 1024, seed=20261203)
+    steps = 2
+    b = Batch(2, [z for z, _ in zs], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR)
+    for i, (_, c) in enumerate(zs):
+        b.set_state(i, c)
+    b.steps_cfl(0.5, steps, BATCH_GRAPH)
+    cells = [z["n"][0] * z["n"][1] for z, _ in zs]
+    sample = sorted({int(np.argmax(cells)), int(np.argmin(cells)), 3, 17, 40, 58, 77, 95})
+    times = b.times()
+    for i in sample:
+        z, c = zs[i]
+        single, t = _single_zone_run(z, c, steps)
+        got = b.get_state(i)
+        print(f"96-zone batch, zone {i} {z['n']}: bitwise={np.array_equal(got, single)}", flush=True)
+        assert np.array_equal(got, single), i
+        assert times[i] == t, i
+    b.close()
