@@ -22,8 +22,36 @@
 
 namespace oracle {
 
-// zone_FD/src/include/macro.hpp:28 (#define GAMMA 1.4)
-constexpr double GAMMA = 1.4;
+// The ratio of specific heats.  The reference defines it as the macro GAMMA
+// (zone_FD/src/include/macro.hpp:28, CongFD1.0/src/include/macro.hpp:28-29):
+// `1.4` in the default build, the unparenthesised text `5.0/3.0` in the build
+// of its Rayleigh-Taylor runs.  The solver routines either copy it into a
+// variable (real gamma = GAMMA: the eigensystems, consToPrim, the HLLC wave
+// speeds) or expand the macro in place; only one expanded expression of the
+// path rounds differently from its variable form: the enthalpy term
+// p/r*GAMMA/(GAMMA-1) of the interface fluxes, which reads
+// p/r*5.0/3.0/(5.0/3.0-1) = (((p/r)*5.0)/3.0)/(5.0/3.0-1) in that build.
+// Every other expansion on the path -- (GAMMA-1)*x, GAMMA*p/r, GAMMA*(p/r),
+// GAMMA/(GAMMA-1)*p, x/(GAMMA-1) -- groups as the variable form does.
+// The oracle keeps the macro's text as (num, den): den == 0 for a single
+// literal.  Set by each oracle entry point before any parallel region.
+struct Gas {
+  double num = 1.4, den = 0.0;
+  double g = 1.4;  // the value of the macro
+};
+inline Gas g_gas;
+inline double gam() { return g_gas.g; }
+inline void set_gas(double num, double den) {
+  g_gas.num = num;
+  g_gas.den = den;
+  g_gas.g = den == 0.0 ? num : num / den;
+}
+// p/r*GAMMA/(GAMMA-1) as the flux routines write it (fluxSchemes.hpp:29,150,224,445;
+// CongFD1.0/src/include/fluxScheme.hpp:411-412, src/fluxScheme.cpp:82-83)
+template <class T> inline T enth_macro(const T& p, const T& r) {
+  if (g_gas.den == 0.0) return p / r * g_gas.g / (g_gas.g - 1);
+  return p / r * g_gas.num / g_gas.den / (g_gas.num / g_gas.den - 1);
+}
 
 // Interpolation schemes (zone_FD/src/include/interScheme.hpp).
 enum Scheme {
@@ -32,7 +60,11 @@ enum Scheme {
   S_TENOSA_F002 = 11, S_TENOSA_F203 = 12, S_TENOSA_F202 = 13,
   S_WHF_N = 14, S_WHF_A = 15, S_WHF_AS = 16, S_WHF_S = 17, S_WHF_LAD = 18,
   S_AI_S = 19, S_AI_A = 20, S_AI_AS = 21, S_AI_LADS = 22, S_AI_AZS = 23, S_AI_F002 = 24, S_WHF_COMPARE = 25,
-  S_CONGAA = 26, S_MUSCL5 = 27
+  S_CONGAA = 26, S_MUSCL5 = 27,
+  // CongFD1.0's own forms of two research schemes (its InterMethod enum,
+  // CongFD1.0/src/include/macro.hpp:45-87, differs from zone_FD's):
+  // whf_TCNS_AS_myF203_NoS with its own constants and whf_TCNS_A
+  S_C1_F203 = 28, S_C1_TENOA = 29
 };
 enum Flux { F_HLLC = 0, F_ROE = 1 };
 
@@ -143,7 +175,9 @@ template <class T> inline T tenos(const T q[5], const double CT = 0.157041780247
 // adaptive constant CT = 10^-beta_A, beta_A = floor(10 - 5 (1 - g)),
 // g = s^4 (5 - 4 s), s = min(1, 4.1666667 eta_min), eta the smoothness ratios
 // of adjacent differences.
-template <class T> inline T tenoa(const T q[5]) {
+// CongFD1.0's whf_TCNS_A (CongFD1.0/src/include/interScheme.hpp:1480-1594) is
+// the same function with s = min(1, 4.166667 eta_min).
+template <class T> inline T tenoa(const T q[5], double inv_cr = 4.1666667) {
   using std::abs;
   T b[3];
   betas(q, b);
@@ -156,7 +190,7 @@ template <class T> inline T tenoa(const T q[5]) {
   T emin = e0;
   if (e1 < emin) emin = e1;
   if (e2 < emin) emin = e2;
-  const T x = 4.1666667 * emin;
+  const T x = inv_cr * emin;
   const T mn = x < 1.0 ? x : T(1.0);
   const T gm = mn * mn * mn * mn * (5.0 - 4 * mn);
   const int betaA = (int)std::floor((double)(10.0 - 5.0 * (1.0 - gm)));
@@ -182,6 +216,8 @@ template <class T> inline T tenoa(const T q[5]) {
 //   law 2, whf_TCNS_AS_myF203_NoS (:3984-4091): m = 1 - std::min(0.24, eta_min)/0.24,
 //     C_T = -0.452662 m^3 + 0.590811 m^2 - 0.012189 m + 0.022010 (mm = m m, mmm = m mm)
 //   law 3, whf_TCNS_AS_myF202_NoS (:4093-4200): the same m, C_T = -0.088182 m m + 0.259395 m - 0.000616
+//   law 4, CongFD1.0's whf_TCNS_AS_myF203_NoS (CongFD1.0/src/include/interScheme.hpp:3170-3279):
+//     law 2 with the constants -0.45, 0.59, -0.012, 0.022
 // then TENO-S's cut-off with that C_T.  The zyc form writes the differences
 // with the opposite sign (q_{i} - q_{i-1}); |2 a b| and a^2 + b^2 are the
 // same doubles either way.
@@ -211,6 +247,9 @@ template <class T> inline T tenosa(const T q[5], int law) {
     if (law == 2) {
       const T mm = m * m, mmm = m * mm;
       CT = -0.452662 * mmm + 0.590811 * mm - 0.012189 * m + 0.022010;
+    } else if (law == 4) {
+      const T mm = m * m, mmm = m * mm;
+      CT = -0.45 * mmm + 0.59 * mm - 0.012 * m + 0.022;
     } else {
       CT = -0.088182 * m * m + 0.259395 * m - 0.000616;
     }
@@ -600,6 +639,8 @@ template <class T> inline T inter5(int s, const T q[5]) {
   case S_WHF_COMPARE: return whf_compare(q);
   case S_CONGAA: return congaa(q);
   case S_MUSCL5: return muscl5(q);
+  case S_C1_F203: return tenosa(q, 4);
+  case S_C1_TENOA: return tenoa(q, 4.166667);
   default: return linear5(q);
   }
 }
@@ -618,14 +659,14 @@ template <class T, int D> inline void cons_to_prim(const T* c, T* p) {
   if (D == 1) {
     T E = c[2] / r;
     T e = -u * u / 2 + E;
-    T RT = (GAMMA - 1) * e;
+    T RT = (gam() - 1) * e;
     p[0] = r; p[1] = u; p[2] = r * RT;
   } else if (D == 2) {
     T v = c[2] / r;
     T E = c[3] / r;
     T q2 = (u * u + v * v) / 2;
     T e = -q2 + E;
-    T RT = (GAMMA - 1) * e;
+    T RT = (gam() - 1) * e;
     p[0] = r; p[1] = u; p[2] = v; p[3] = r * RT;
   } else {
     T v = c[2] / r;
@@ -633,7 +674,7 @@ template <class T, int D> inline void cons_to_prim(const T* c, T* p) {
     T E = c[4] / r;
     T q2 = (u * u + v * v + w * w) / 2;
     T e = -q2 + E;
-    T RT = (GAMMA - 1) * e;
+    T RT = (gam() - 1) * e;
     p[0] = r; p[1] = u; p[2] = v; p[3] = w; p[4] = r * RT;
   }
 }
@@ -645,14 +686,14 @@ template <class T, int D> inline void prim_to_cons(const T* p, T* c) {
   T r = p[0];
   if (D == 1) {
     T ek = (p[1] * p[1]) / 2;
-    c[0] = r; c[1] = r * p[1]; c[2] = p[2] / (GAMMA - 1) + r * ek;
+    c[0] = r; c[1] = r * p[1]; c[2] = p[2] / (gam() - 1) + r * ek;
   } else if (D == 2) {
     T ek = (p[1] * p[1] + p[2] * p[2]) / 2;
-    c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = p[3] / (GAMMA - 1) + r * ek;
+    c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = p[3] / (gam() - 1) + r * ek;
   } else {
     T ek = (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]) / 2;
     c[0] = r; c[1] = r * p[1]; c[2] = r * p[2]; c[3] = r * p[3];
-    c[4] = p[4] / (GAMMA - 1) + r * ek;
+    c[4] = p[4] / (gam() - 1) + r * ek;
   }
 }
 
@@ -672,16 +713,16 @@ template <class T, int D> struct Eigen {
     if (D == 1) {
       T rl = pl[0], ul = pl[1], ppl = pl[2];
       T rr = pr[0], ur = pr[1], ppr = pr[2];
-      T HL = (ul * ul) / 2 + ppl / rl * GAMMA / (GAMMA - 1);
-      T HR = (ur * ur) / 2 + ppr / rr * GAMMA / (GAMMA - 1);
+      T HL = (ul * ul) / 2 + ppl / rl * gam() / (gam() - 1);
+      T HR = (ur * ur) / 2 + ppr / rr * gam() / (gam() - 1);
       T c1 = sqrt(rl), c2 = sqrt(rr);
       T divisor = 1.0 / (sqrt(rl) + sqrt(rr));
       T u = (c1 * ul + c2 * ur) * divisor;
       T ht = (c1 * HL + c2 * HR) * divisor;
       T ek = (u * u) / 2;
       T h = ht - ek;
-      T c = sqrt((GAMMA - 1) * h);
-      const double g1 = GAMMA - 1;
+      T c = sqrt((gam() - 1) * h);
+      const double g1 = gam() - 1;
       T l[9] = {ht + c * (u - c) / g1, -u - c / g1, T(1.0),
                 -2.0 * ht + 4.0 * c * c / g1, 2.0 * u, T(-2.0),
                 ht - c * (u + c) / g1, -u + c / g1, T(1.0)};
@@ -698,11 +739,11 @@ template <class T, int D> struct Eigen {
     T ppl = pl[NV - 1], ppr = pr[NV - 1];
     T HL, HR;
     if (D == 2) {
-      HL = (ul * ul + vl * vl) / 2 + ppl / rl * GAMMA / (GAMMA - 1);
-      HR = (ur * ur + vr * vr) / 2 + ppr / rr * GAMMA / (GAMMA - 1);
+      HL = (ul * ul + vl * vl) / 2 + ppl / rl * gam() / (gam() - 1);
+      HR = (ur * ur + vr * vr) / 2 + ppr / rr * gam() / (gam() - 1);
     } else {
-      HL = (ul * ul + vl * vl + wl * wl) / 2 + ppl / rl * GAMMA / (GAMMA - 1);
-      HR = (ur * ur + vr * vr + wr * wr) / 2 + ppr / rr * GAMMA / (GAMMA - 1);
+      HL = (ul * ul + vl * vl + wl * wl) / 2 + ppl / rl * gam() / (gam() - 1);
+      HR = (ur * ur + vr * vr + wr * wr) / 2 + ppr / rr * gam() / (gam() - 1);
     }
     T c1 = sqrt(rl), c2 = sqrt(rr);
     T divisor = 1.0 / (c1 + c2);
@@ -720,7 +761,7 @@ template <class T, int D> struct Eigen {
       ek = (u * u + v * v + w * w) / 2;
     }
     T h = ht - ek;
-    T c = sqrt((GAMMA - 1) * h);
+    T c = sqrt((gam() - 1) * h);
     if (D == 2) {
       // leftEig / rightEig (eigenSystem.cpp:53-85)
       T l[16] = {(-ny * u + nx * v), T(ny), T(-nx), T(0.0),
@@ -787,15 +828,15 @@ template <class T, int D> inline void eig_cons_to_prim(const T* c, T* p) {
   if (D == 1) {
     T ut = c[1] / rt;
     T ekt = (ut * ut) / 2;
-    p[0] = rt; p[1] = ut; p[2] = (GAMMA - 1) * (c[2] - rt * ekt);
+    p[0] = rt; p[1] = ut; p[2] = (gam() - 1) * (c[2] - rt * ekt);
   } else if (D == 2) {
     T ut = c[1] / rt, vt = c[2] / rt;
     T rekt = (c[1] * c[1] + c[2] * c[2]) / rt / 2;
-    p[0] = rt; p[1] = ut; p[2] = vt; p[3] = (GAMMA - 1) * (c[3] - rekt);
+    p[0] = rt; p[1] = ut; p[2] = vt; p[3] = (gam() - 1) * (c[3] - rekt);
   } else {
     T ut = c[1] / rt, vt = c[2] / rt, wt = c[3] / rt;
     T rekt = (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) / rt / 2;
-    p[0] = rt; p[1] = ut; p[2] = vt; p[3] = wt; p[4] = (GAMMA - 1) * (c[4] - rekt);
+    p[0] = rt; p[1] = ut; p[2] = vt; p[3] = wt; p[4] = (gam() - 1) * (c[4] - rekt);
   }
 }
 
@@ -806,8 +847,8 @@ template <class T> inline T lim_rho(const T* W) { return W[0]; }
 template <class T> inline T lim_rhoc2(const T* W) {
   T u = W[1] / W[0];
   T ekin = 0.5 * u * u;
-  T p = (GAMMA - 1) * (W[2] - W[0] * ekin);
-  return GAMMA * p;
+  T p = (gam() - 1) * (W[2] - W[0] * ekin);
+  return gam() * p;
 }
 template <class T> inline void positive_limit(const T* W, T* Wi, int which) {
   T qi = which == 0 ? lim_rho(Wi) : lim_rhoc2(Wi);
@@ -825,9 +866,15 @@ template <class T> inline void positive_limit(const T* W, T* Wi, int which) {
 // w[0..5] = cells i-3 .. i+2 (interface between w[2] and w[3]).
 // Component-wise: Recon5OrderFaceCenter::reconI (reconstructor5order.hpp:128-151).
 // Characteristic: Recon5Order{1D,2D}EulerEig::reconI (:154-244, :247-356).
+// variant 1, CongFD1.0's Euler path (SpaceDis::recon1DFaceCenter /
+// recon2DFaceCenter, CongFD1.0/src/src/sp_recon.cpp:162-267, with the checks
+// of calFluxEuler1D / 2D, sp_flux.cpp:123-252): the same characteristic
+// reconstruction with no 1D limiter, and a state with negative density or
+// pressure replaced by its own cell (cell i-1 for the left state, cell i for
+// the right one) in 1D and 2D alike.
 template <class T, int D>
 inline void recon_interface(int scheme, bool charwise, const T* const* wp, const T* const* wc, int ax,
-                            T* outL, T* outR) {
+                            T* outL, T* outR, int variant = 0) {
   constexpr int NV = D + 2;
   if (!charwise) {
     for (int i = 0; i < NV; i++) {
@@ -854,6 +901,15 @@ inline void recon_interface(int scheme, bool charwise, const T* const* wp, const
   T cL[NV], cR[NV];
   eig.to_cons(QL, cL);
   eig.to_cons(QR, cR);
+  if (variant == 1) {
+    eig_cons_to_prim<T, D>(cL, outL);
+    eig_cons_to_prim<T, D>(cR, outR);
+    if (outL[0] < 0 || outL[NV - 1] < 0)
+      for (int i = 0; i < NV; i++) outL[i] = pL[i];
+    if (outR[0] < 0 || outR[NV - 1] < 0)
+      for (int i = 0; i < NV; i++) outR[i] = pR[i];
+    return;
+  }
   if (D == 1) {
     T consL[3], consR[3];
     prim_to_cons<T, 1>(pL, consL);
@@ -883,10 +939,10 @@ inline void recon_interface(int scheme, bool charwise, const T* const* wp, const
 template <class T> inline void hllc1d(const T* L, const T* Rr, T* res) {
   using std::sqrt;
   T rl = L[0], ul = L[1], pl = L[2], rr = Rr[0], ur = Rr[1], pr = Rr[2];
-  T HL = pl / rl * GAMMA / (GAMMA - 1) + (ul * ul) / 2;
-  T HR = pr / rr * GAMMA / (GAMMA - 1) + (ur * ur) / 2;
-  T cl = sqrt(GAMMA * (pl / rl));
-  T cr = sqrt(GAMMA * (pr / rr));
+  T HL = enth_macro(pl, rl) + (ul * ul) / 2;
+  T HR = enth_macro(pr, rr) + (ur * ur) / 2;
+  T cl = sqrt(gam() * (pl / rl));
+  T cr = sqrt(gam() * (pr / rr));
   T SL = smin(ur - cr, ul - cl);
   T SR = smax(ul + cl, ur + cr);
   T Sstar = ((pr - pl) + (rl * ul * (SL - ul) - rr * ur * (SR - ur))) / (rl * (SL - ul) - rr * (SR - ur));
@@ -894,13 +950,13 @@ template <class T> inline void hllc1d(const T* L, const T* Rr, T* res) {
     res[0] = rl * ul; res[1] = rl * ul * ul + pl; res[2] = rl * HL * ul;
   } else if (Sstar >= 0) {
     T pStar = pl + rl * (SL - ul) * (Sstar - ul);
-    T U[3] = {rl, rl * ul, pl / (GAMMA - 1) + rl * ul * ul / 2};
+    T U[3] = {rl, rl * ul, pl / (gam() - 1) + rl * ul * ul / 2};
     T F[3] = {rl * ul, rl * ul * ul + pl, rl * HL * ul};
     T Dd[3] = {T(0.0), T(1.0), Sstar};
     for (int i = 0; i < 3; i++) res[i] = (Sstar * (SL * U[i] - F[i]) + SL * pStar * Dd[i]) / (SL - Sstar);
   } else if (SR >= 0) {
     T pStar = pr + rr * (SR - ur) * (Sstar - ur);
-    T U[3] = {rr, rr * ur, pr / (GAMMA - 1) + rr * ur * ur / 2};
+    T U[3] = {rr, rr * ur, pr / (gam() - 1) + rr * ur * ur / 2};
     T F[3] = {rr * ur, rr * ur * ur + pr, rr * HR * ur};
     T Dd[3] = {T(0.0), T(1.0), Sstar};
     for (int i = 0; i < 3; i++) res[i] = (Sstar * (SR * U[i] - F[i]) + SR * pStar * Dd[i]) / (SR - Sstar);
@@ -914,13 +970,13 @@ template <class T> inline void roe1d(const T* L, const T* Rr, T* res) {
   using std::abs;
   using std::sqrt;
   T rl = L[0], ul = L[1], pl = L[2], rr = Rr[0], ur = Rr[1], pr = Rr[2];
-  T HL = pl / rl * GAMMA / (GAMMA - 1) + (ul * ul) / 2;
-  T HR = pr / rr * GAMMA / (GAMMA - 1) + (ur * ur) / 2;
+  T HL = enth_macro(pl, rl) + (ul * ul) / 2;
+  T HR = enth_macro(pr, rr) + (ur * ur) / 2;
   T rBar = sqrt(rl * rr);
   T uBar = (ul * sqrt(rl) + ur * sqrt(rr)) / (sqrt(rl) + sqrt(rr));
   T HBar = (HL * sqrt(rl) + HR * sqrt(rr)) / (sqrt(rl) + sqrt(rr));
-  T cBar = sqrt((GAMMA - 1) * (HBar - uBar * uBar / 2));
-  T cBar2 = (GAMMA - 1) * (HBar - uBar * uBar / 2);
+  T cBar = sqrt((gam() - 1) * (HBar - uBar * uBar / 2));
+  T cBar2 = (gam() - 1) * (HBar - uBar * uBar / 2);
   T dr = rr - rl, du = ur - ul, dp = pr - pl;
   T lam[3] = {abs(uBar), abs(uBar + cBar), abs(uBar - cBar)};
   T eps = 0.1 * (abs(uBar) + cBar);
@@ -936,7 +992,7 @@ template <class T> inline void roe1d(const T* L, const T* Rr, T* res) {
   for (int i = 0; i < 3; i++) res[i] = (FL[i] + FR[i]) / 2.0;
   res[0] -= 0.5 * a3;
   res[1] -= 0.5 * (a3 * uBar + a4);
-  res[2] -= 0.5 * (HBar * a3 + uBar * a4 - cBar2 * a0 / (GAMMA - 1));
+  res[2] -= 0.5 * (HBar * a3 + uBar * a4 - cBar2 * a0 / (gam() - 1));
 }
 
 // HLLCFlux2D2 (fluxSchemes.hpp:202-327); EXTENSION to 3D with the tangential
@@ -951,19 +1007,19 @@ template <class T, int D> inline void hllc_md(const T* Lp, const T* Rp, int ax, 
   for (int k = 0; k < D; k++) { vel_l[k] = Lp[1 + k]; vel_r[k] = Rp[1 + k]; }
   T HL, HR;
   if (D == 2) {
-    HL = (vel_l[0] * vel_l[0] + vel_l[1] * vel_l[1]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
-    HR = (vel_r[0] * vel_r[0] + vel_r[1] * vel_r[1]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+    HL = (vel_l[0] * vel_l[0] + vel_l[1] * vel_l[1]) / 2 + enth_macro(pl, rl);
+    HR = (vel_r[0] * vel_r[0] + vel_r[1] * vel_r[1]) / 2 + enth_macro(pr, rr);
   } else {
-    HL = (vel_l[0] * vel_l[0] + vel_l[1] * vel_l[1] + vel_l[2] * vel_l[2]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
-    HR = (vel_r[0] * vel_r[0] + vel_r[1] * vel_r[1] + vel_r[2] * vel_r[2]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+    HL = (vel_l[0] * vel_l[0] + vel_l[1] * vel_l[1] + vel_l[2] * vel_l[2]) / 2 + enth_macro(pl, rl);
+    HR = (vel_r[0] * vel_r[0] + vel_r[1] * vel_r[1] + vel_r[2] * vel_r[2]) / 2 + enth_macro(pr, rr);
   }
   T VnL = vel_l[ax], VnR = vel_r[ax];
-  T cl = sqrt(pl / rl * GAMMA);
-  T cr = sqrt(pr / rr * GAMMA);
+  T cl = sqrt(pl / rl * gam());
+  T cr = sqrt(pr / rr * gam());
   T rBar = (rl + rr) / 2, cBar = (cl + cr) / 2;
   T ps = smax(T(0.0), (pl + pr) / 2 - rBar * cBar * (VnR - VnL) / 2);
-  T ql = (ps <= pl) ? T(1.0) : sqrt(1.0 + (GAMMA + 1.0) / (2.0 * GAMMA) * (ps / pl - 1.0));
-  T qr = (ps <= pr) ? T(1.0) : sqrt(1.0 + (GAMMA + 1.0) / (2.0 * GAMMA) * (ps / pr - 1.0));
+  T ql = (ps <= pl) ? T(1.0) : sqrt(1.0 + (gam() + 1.0) / (2.0 * gam()) * (ps / pl - 1.0));
+  T qr = (ps <= pr) ? T(1.0) : sqrt(1.0 + (gam() + 1.0) / (2.0 * gam()) * (ps / pr - 1.0));
   T SL = VnL - cl * ql, SR = VnR + cr * qr;
   T Sstar = ((pr - pl) + (rl * VnL * (SL - VnL) - rr * VnR * (SR - VnR))) / (rl * (SL - VnL) - rr * (SR - VnR));
   auto side_flux = [&](T r, const T* vel, T p, T H, T Vn, T* F) {
@@ -1019,11 +1075,11 @@ template <class T, int D> inline void roe_md(const T* Lp, const T* Rp, int ax, T
   for (int k = 0; k < D; k++) { vl[k] = Lp[1 + k]; vr[k] = Rp[1 + k]; }
   T HL, HR;
   if (D == 2) {
-    HL = (vl[0] * vl[0] + vl[1] * vl[1]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
-    HR = (vr[0] * vr[0] + vr[1] * vr[1]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+    HL = (vl[0] * vl[0] + vl[1] * vl[1]) / 2 + enth_macro(pl, rl);
+    HR = (vr[0] * vr[0] + vr[1] * vr[1]) / 2 + enth_macro(pr, rr);
   } else {
-    HL = (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) / 2 + pl / rl * GAMMA / (GAMMA - 1);
-    HR = (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) / 2 + pr / rr * GAMMA / (GAMMA - 1);
+    HL = (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) / 2 + enth_macro(pl, rl);
+    HR = (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]) / 2 + enth_macro(pr, rr);
   }
   T VnL = vl[ax], VnR = vr[ax];
   T FcL[NV], FcR[NV];
@@ -1044,7 +1100,7 @@ template <class T, int D> inline void roe_md(const T* Lp, const T* Rp, int ax, T
   T q2;
   if (D == 2) q2 = (va[0] * va[0] + va[1] * va[1]) / 2;
   else q2 = (va[0] * va[0] + va[1] * va[1] + va[2] * va[2]) / 2;
-  T cAvg = sqrt((GAMMA - 1) * (HAvg - q2));
+  T cAvg = sqrt((gam() - 1) * (HAvg - q2));
   T VnAvg = va[ax];
   T lam[3] = {abs(VnAvg - cAvg), abs(VnAvg), abs(VnAvg + cAvg)};
   T eps = 0.05 * (abs(VnAvg) + cAvg);
@@ -1083,7 +1139,7 @@ template <class T, int D> inline void node_flux(const T* p, int ax, T* F) {
     T r = p[0], u = p[1], pp = p[2];
     F[0] = r * u;
     F[1] = r * u * u + pp;
-    F[2] = (GAMMA / (GAMMA - 1) * pp + r * (u * u) / 2) * u;
+    F[2] = (gam() / (gam() - 1) * pp + r * (u * u) / 2) * u;
     return;
   }
   constexpr int NV = D + 2;
@@ -1100,7 +1156,7 @@ template <class T, int D> inline void node_flux(const T* p, int ax, T* F) {
   }
   F[0] = r * Vn;
   for (int k = 0; k < D; k++) F[1 + k] = r * p[1 + k] * Vn + n[k] * pp;
-  F[E] = (k2 / 2 * r + GAMMA / (GAMMA - 1) * pp) * Vn;
+  F[E] = (k2 / 2 * r + gam() / (gam() - 1) * pp) * Vn;
 }
 
 template <class T, int D> inline void riemann(int flux, const T* L, const T* R, int ax, T* res) {
@@ -1197,6 +1253,9 @@ struct Grid {
   double x0 = 0.0;
   double t = 0.0;
   double gravity[3] = {0.0, 0.0, 0.0};
+  // 0: zone_FD's path; 1: CongFD1.0's Euler path (recon_interface's variant,
+  // and SpaceDis::difMND6's three divisions by h, CongFD1.0/src/src/sp_difference.cpp:130-132)
+  int variant = 0;
 };
 
 // One direction sweep: SpDistributor::rhsSolve body for direction `ax`
@@ -1252,12 +1311,19 @@ void sweep(const Grid& g, int ax, const T* prim, T* rhs, int nthreads) {
           wc[j] = &cc[(cell + 4) * NV];
         }
         T L[NV], R[NV];
-        recon_interface<T, D>(g.scheme, g.charwise != 0, wp, wc, ax, L, R);
+        recon_interface<T, D>(g.scheme, g.charwise != 0, wp, wc, ax, L, R, g.variant);
         riemann<T, D>(g.flux, L, R, ax, &fh[m * NV]);
       }
       for (int m = 0; m < n + 2; m++) node_flux<T, D>(&pp[(m - 1 + 4) * NV], ax, &fn[m * NV]);
       for (int i = 0; i < n; i++) {
         T* r = &rhs[(base + i * sa) * NV];
+        if (g.variant == 1) {
+          for (int v = 0; v < NV; v++)
+            r[v] += w1 * (fn[(i + 2) * NV + v] - fn[i * NV + v]) / hh +
+                    w0 * (fh[(i + 2) * NV + v] - fh[(i + 1) * NV + v]) / hh +
+                    w2 * (fh[(i + 3) * NV + v] - fh[i * NV + v]) / hh;
+          continue;
+        }
         for (int v = 0; v < NV; v++) {
           r[v] += (w1 * (fn[(i + 2) * NV + v] - fn[i * NV + v]) + w0 * (fh[(i + 2) * NV + v] - fh[(i + 1) * NV + v]) +
                    w2 * (fh[(i + 3) * NV + v] - fh[i * NV + v])) /
@@ -1328,7 +1394,7 @@ template <class T, int D> double cfl_dt(const Grid& g, const T* cons, double cfl
     for (long i = 0; i < N; i++) {
       T p[NV];
       cons_to_prim<T, D>(&cons[i * NV], p);
-      double li = (std::sqrt(GAMMA * double(p[NV - 1]) / double(p[0])) + std::abs(double(p[ax + 1]))) / g.h[ax];
+      double li = (std::sqrt(gam() * double(p[NV - 1]) / double(p[0])) + std::abs(double(p[ax + 1]))) / g.h[ax];
       mx = std::max(mx, li);
     }
     lam += mx;

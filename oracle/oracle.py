@@ -24,6 +24,8 @@ S_TENOSA_F002, S_TENOSA_F203, S_TENOSA_F202 = range(11, 14)
 S_WHF_N, S_WHF_A, S_WHF_AS, S_WHF_S, S_WHF_LAD = range(14, 19)
 S_AI_S, S_AI_A, S_AI_AS, S_AI_LADS, S_AI_AZS, S_AI_F002, S_WHF_COMPARE = range(19, 26)
 S_CONGAA, S_MUSCL5 = range(26, 28)
+# CongFD1.0's whf_TCNS_AS_myF203_NoS and whf_TCNS_A
+S_C1_F203, S_C1_TENOA = range(28, 30)
 # oracle::Flux
 F_HLLC, F_ROE = 0, 1
 
@@ -41,7 +43,17 @@ class OracleGrid(ctypes.Structure):
         ("x0", ctypes.c_double),
         ("t", ctypes.c_double),
         ("gravity", ctypes.c_double * 3),
+        ("variant", ctypes.c_int),
+        ("gamma_num", ctypes.c_double),
+        ("gamma_den", ctypes.c_double),
     ]
+
+
+# oracle Grid.variant
+V_ZONE_FD, V_CONGFD1 = 0, 1
+# the GAMMA macro's text: (num, den) for "num/den", (num, 0) for one literal
+GAMMA_14 = (1.4, 0.0)
+GAMMA_5_3 = (5.0, 3.0)
 
 
 def build():
@@ -76,6 +88,7 @@ def lib():
         _lib.oracle_recon.argtypes = [i, i, i, pd, i, pd, pd]
         _lib.oracle_accuracy_rhs.argtypes = [i, i, pd, pd, pd, d, pd]
         _lib.oracle_max_threads.restype = i
+        _lib.oracle_set_gas.argtypes = [d, d]
         _lib.oracle_dual_rhs.argtypes = [pg, pd, pd, pd, pd, i]
         _lib.oracle_dual_rk3.argtypes = [pg, pd, pd, d, i, i]
     return _lib
@@ -89,8 +102,11 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def make_grid(dim, n, h, scheme, flux, charwise, bc, bc_prim=None, x0=0.0, t=0.0, gravity=(0.0, 0.0, 0.0)):
+def make_grid(dim, n, h, scheme, flux, charwise, bc, bc_prim=None, x0=0.0, t=0.0, gravity=(0.0, 0.0, 0.0),
+              variant=V_ZONE_FD, gamma=GAMMA_14):
     g = OracleGrid()
+    g.variant = int(variant)
+    g.gamma_num, g.gamma_den = float(gamma[0]), float(gamma[1])
     g.x0 = float(x0)
     g.t = float(t)
     for i in range(3):
@@ -160,19 +176,22 @@ def cfl_dt(grid, cons, cfl, nthreads=None):
     return lib().oracle_cfl_dt(ctypes.byref(grid), _p(c), float(cfl), nthreads or nthreads_default())
 
 
-def inter5(scheme, q):
+def inter5(scheme, q, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     q = _f64(q)
     return lib().oracle_inter5(scheme, _p(q))
 
 
-def cons_to_prim(dim, c):
+def cons_to_prim(dim, c, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     c = _f64(c)
     p = np.empty(dim + 2)
     lib().oracle_cons_to_prim(dim, _p(c), _p(p))
     return p
 
 
-def eigen(dim, pl, pr, ax):
+def eigen(dim, pl, pr, ax, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     nv = dim + 2
     L = np.empty((nv, nv))
     R = np.empty((nv, nv))
@@ -180,7 +199,8 @@ def eigen(dim, pl, pr, ax):
     return L, R
 
 
-def prim_char_roundtrip(dim, pl, pr, ax, prim):
+def prim_char_roundtrip(dim, pl, pr, ax, prim, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     nv = dim + 2
     ch = np.empty(nv)
     back = np.empty(nv)
@@ -188,19 +208,22 @@ def prim_char_roundtrip(dim, pl, pr, ax, prim):
     return ch, back
 
 
-def riemann(dim, flux, L, R, ax):
+def riemann(dim, flux, L, R, ax, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     out = np.empty(dim + 2)
     lib().oracle_riemann(dim, flux, _p(_f64(L)), _p(_f64(R)), ax, _p(out))
     return out
 
 
-def node_flux(dim, p, ax):
+def node_flux(dim, p, ax, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     out = np.empty(dim + 2)
     lib().oracle_node_flux(dim, _p(_f64(p)), ax, _p(out))
     return out
 
 
-def recon(dim, scheme, charwise, window, ax):
+def recon(dim, scheme, charwise, window, ax, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     nv = dim + 2
     w = _f64(window).reshape(6 * nv)
     L = np.empty(nv)
@@ -209,7 +232,8 @@ def recon(dim, scheme, charwise, window, ax):
     return L, R
 
 
-def accuracy_rhs(scheme, cells, ghost_l, ghost_r, h):
+def accuracy_rhs(scheme, cells, ghost_l, ghost_r, h, gamma=GAMMA_14):
+    lib().oracle_set_gas(*gamma)
     cells = _f64(cells)
     out = np.empty_like(cells)
     lib().oracle_accuracy_rhs(scheme, len(cells), _p(cells), _p(_f64(ghost_l)), _p(_f64(ghost_r)), float(h), _p(out))

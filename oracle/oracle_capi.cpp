@@ -22,6 +22,8 @@ struct oracle_grid {
   double x0;          // cell centres at x0 + (i + 0.5) h[0]
   double t;           // time at the start (boundaries that depend on time)
   double gravity[3];  // acceleration of the gravity source (0: none)
+  int variant;        // 0 zone_FD's path, 1 CongFD1.0's Euler path
+  double gamma_num, gamma_den;  // the text of the GAMMA macro: num (den = 0) or num/den
 };
 
 static Grid to_grid(const oracle_grid* c) {
@@ -38,10 +40,17 @@ static Grid to_grid(const oracle_grid* c) {
   g.x0 = c->x0;
   g.t = c->t;
   for (int i = 0; i < 3; i++) g.gravity[i] = c->gravity[i];
+  g.variant = c->variant;
+  if (c->gamma_num != 0.0) set_gas(c->gamma_num, c->gamma_den);
+  else set_gas(1.4, 0.0);
   return g;
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+// The GAMMA macro's text for the point entry points below (the grid entry
+// points take it from their grid): num alone (den = 0) or num/den.
+void oracle_set_gas(double num, double den) { set_gas(num, den); }
 
 // rhs (AoS [cell][nv], x fastest) of the state cons; prim receives consToPrim.
 int oracle_rhs(const oracle_grid* c, const double* cons, double* prim, double* rhs, int nthreads) {

@@ -13,7 +13,11 @@ package's CGNS reader (cfd_inwsl_amd/cgns.py) and writes:
 * reference_outputs.json: the run each array comes from (the reference file,
   the driver's case number and info.txt values), and SHA-256 digests of the
   fields and coordinates of initial-state files, which the driver's
-  --initial-only output must reproduce bit for bit.
+  --initial-only output must reproduce bit for bit;
+* CongFD1.0's runs: the two 200-cell Lax series (every output's digests, the
+  last output's fields) and the two 64x256 Rayleigh-Taylor runs of its
+  gamma = 5/3 build (digests of the t = 0 and t = 1.95 fields, the t = 1.95
+  density as float32).
 
 Run in a checkout that has the reference tree (default /root/reference):
     python tests/golden/make_reference_outputs.py [REFERENCE_ROOT]
@@ -57,6 +61,17 @@ SOD_LOG = "zone_FD/cases/Sod - congTCNS5CT5 - 101"
 # whf_zyc_TcnsN_myASF002_1 (InterMethod whfzycTCNSNmyASF002_1 = 33), outputs at t = 0.1 and 0.2
 SOD_ZYC = "zone_FD/cases/Sod - whfzycTCNSNmyASF002_1 - 101"
 DMR = "zone_FD/cases/Double Mach    t=0.2000/512, 128 - TENO   t=0.2000.cgns"
+# CongFD1.0's own runs (its Euler path: Roe flux hard-wired, no 1D limiter,
+# three-division MND6): two 200-cell Lax series with outputs every 0.01 up to
+# 0.14, and two 64x256 Rayleigh-Taylor runs of the gamma = 5/3 build at t = 1.95
+C1_LAX = {
+    "c1_lax_f203": "CongFD1.0/cases/Lax - TENO-AS-myF203_NoS - 201 - t={t}.cgns",
+    "c1_lax_tenos": "CongFD1.0/cases/EULERWCNS5t={t}.cgns",
+}
+C1_RTI = {
+    "c1_rti_whftcnsa": "CongFD1.0/cases/RTI - WHFTCNSA - 65x257 - t={t}.cgns",
+    "c1_rti_whftcnsaf002": "CongFD1.0/cases/RTI - WHFTCNSAF002 - 65x257 - t={t}.cgns",
+}
 FIELDS = ("Density", "XVelocity", "YVelocity", "ZVelocity", "Pressure")
 
 
@@ -137,6 +152,28 @@ def main(root):
     zone, imax = zone_of(sol)
     arrays["dmr/Density_f32"] = sol["Density"].astype(np.float32)
     meta["dmr"] = dict(file=DMR, case=4, dim=2, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5, t_end=0.2)
+    # CongFD1.0 Lax series: the t = 0.14 fields in full, SHA-256 digests of
+    # every output's fields (t = 0.01 .. 0.14)
+    for key, pattern in C1_LAX.items():
+        meta[key] = dict(file=pattern, case=2, dim=1, cfl=0.5, output_dt=0.01, n_out=14, sha256=[])
+        for k in range(1, 15):
+            sol = cgns.read_solution(os.path.join(root, pattern.format(t=f"{0.01 * k:.4f}")))
+            meta[key]["sha256"].append({f: digest(sol[f]) for f in ("Density", "XVelocity", "Pressure")})
+        zone, imax = zone_of(sol)
+        meta[key].update(zone=zone, imax=imax)
+        for f in ("Density", "XVelocity", "Pressure"):
+            arrays[f"{key}/t0.1400/{f}"] = sol[f]
+    # CongFD1.0 Rayleigh-Taylor runs: digests of the t = 0 and t = 1.95 fields
+    # (cells x fastest), the t = 1.95 density as float32 for error reports
+    for key, pattern in C1_RTI.items():
+        meta[key] = dict(file=pattern, case=3, dim=2, cfl=0.5, output_dt=1.95)
+        for t in ("0.0000", "1.9500"):
+            sol = cgns.read_solution(os.path.join(root, pattern.format(t=t)))
+            meta[key][f"sha256_t{t}"] = {f: digest(sol[f].T.reshape(-1))
+                                        for f in ("Density", "XVelocity", "YVelocity", "Pressure")}
+        zone, imax = zone_of(sol)
+        meta[key].update(zone=zone, imax=imax)
+        arrays[f"{key}/t1.9500/Density_f32"] = sol["Density"].T.reshape(-1).astype(np.float32)
     tree = cgns.read_tree(os.path.join(root, CASES["sod"]["file"]))
     meta["tree_1d"] = tree_listing(tree)
     meta["root_attrs"] = {k: v for k, v in tree.attrs.items() if isinstance(v, str)}
