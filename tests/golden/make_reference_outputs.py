@@ -151,7 +151,14 @@ def main(root):
     sol = cgns.read_solution(os.path.join(root, DMR))
     zone, imax = zone_of(sol)
     arrays["dmr/Density_f32"] = sol["Density"].astype(np.float32)
-    meta["dmr"] = dict(file=DMR, case=4, dim=2, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5, t_end=0.2)
+    # the double-Mach run wrote an output every 0.01 (zoneMain's Double Mach
+    # preset: endStep 20, outputDt 0.01); digests of the four schemes' t = 0.2 fields
+    meta["dmr"] = dict(file=DMR, case=4, dim=2, zone=zone, imax=imax, inter=11, flux="hllc", cfl=0.5, t_end=0.2,
+                       output_dt=0.01, sha256={})
+    for variant in ("TENO", "TENOS", "TENOA", "TENOAS"):
+        sol = cgns.read_solution(os.path.join(root, DMR.replace("- TENO ", f"- {variant} ")))
+        meta["dmr"]["sha256"][variant] = {f: digest(sol[f].T.reshape(-1))
+                                          for f in ("Density", "XVelocity", "YVelocity", "Pressure")}
     # CongFD1.0 Lax series: the t = 0.14 fields in full, SHA-256 digests of
     # every output's fields (t = 0.01 .. 0.14)
     for key, pattern in C1_LAX.items():

@@ -235,26 +235,31 @@ def test_driver_reproduces_reference_research_files(tmp_path, key, variant, inte
         assert np.array_equal(sol[f], want), (f, np.abs(sol[f] - want).max())
 
 
+def _digest(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
 @pytest.mark.gpu
-def test_driver_double_mach_near_reference_solution_file(tmp_path):
-    """Double Mach reflection, 512 x 128 cells to t = 0.2, against the
-    reference's "512, 128 - TENO t=0.2000" file.  That run's settings are not
-    recorded and no setting tried reproduces it exactly (the 1D TENO files
-    are reproduced bit for bit), so the comparison is loose: the same waves at
-    the same places, density within a mean 0.03 (measured 0.018; through
-    the oracle, a reflecting wall over x < 1/6 gives 0.34 and a DoubleMachUp
-    top frozen at t = 0 does not reach t = 0.2)."""
+@pytest.mark.parametrize("variant,inter", [("TENO", 11), ("TENOAS", 26)])
+def test_driver_reproduces_reference_double_mach_files(tmp_path, variant, inter):
+    """Double Mach reflection, 512 x 128 cells, TENO5 (inter 11) and
+    whf_ai_TcnsN_AS_1 (inter 26) char-wise + HLLC, CFL 0.5, outputs every 0.01
+    to t = 0.2 (zoneMain's Double Mach preset: endStep 20, outputDt 0.01):
+    the driver's t = 0.2 file equals the reference's "512, 128 - TENO" /
+    "TENOAS" file bit for bit (all four fields).  This run exercises the
+    time-dependent DoubleMachUp top, the x < 1/6 wall wiring, HLLC's star
+    branches and the 2D positivity fallback at the Mach-10 shock."""
     meta, arr = _reference_fixtures()
     m = meta["dmr"]
-    _info_txt(tmp_path / "info.txt", 11, 1, m["t_end"], m["cfl"], m["case"], m["zone"], m["imax"], 2)
-    r = _run(["--flux", "hllc"], tmp_path)
+    nout = int(round(m["t_end"] / m["output_dt"]))
+    _info_txt(tmp_path / "info.txt", 11, nout, m["output_dt"], m["cfl"], m["case"], m["zone"], m["imax"], 2)
+    r = _run(["--flux", "hllc", "--inter", str(inter)], tmp_path)
     print(r.stdout[-200:], r.stderr[-300:], flush=True)
     assert r.returncode == 0
-    rho = cgns.read_solution(_last_output(tmp_path))["Density"]
-    ref = arr["dmr/Density_f32"].astype(np.float64)
-    d = np.abs(rho - ref)
-    print(f"double Mach vs reference file: mean |drho| {d.mean():.4f}, max {d.max():.3f}, "
-          f"equal to float32 {np.mean(rho.astype(np.float32) == arr['dmr/Density_f32']):.3f}", flush=True)
-    assert d.mean() < 0.03
-    # the undisturbed pre- and post-shock regions are the reference's exactly
-    assert np.mean(rho.astype(np.float32) == arr["dmr/Density_f32"]) > 0.25
+    sol = cgns.read_solution(_last_output(tmp_path))
+    if variant == "TENO":
+        d = np.abs(sol["Density"] - arr["dmr/Density_f32"].astype(np.float64))
+        print(f"double Mach {variant}: max |drho| vs the file's float32 copy {d.max():.2e}", flush=True)
+    for f in ("Density", "XVelocity", "YVelocity", "Pressure"):
+        assert _digest(sol[f].T.reshape(-1)) == m["sha256"][variant][f], (variant, f)
