@@ -152,6 +152,28 @@ def adjoint_shock(n, seed=20261204, amp=1e-2, kmax=8):
             "h": h, "window": (slice(j0, j0 + w), slice(i0, i0 + w))}
 
 
+def adjoint_smooth(n, seed=20261205, amp=1e-2, kmax=8, u0=0.5, v0=0.3):
+    """Smooth companion of adjoint_shock on the periodic unit square: a
+    uniform subsonic stream (u0, v0) at p = 1/gamma, rho = 1 (Mach 0.58)
+    carrying seeded random-phase perturbations of amplitude amp (|k| <= kmax)
+    in all four primitive variables -- acoustic and entropy waves with no
+    discontinuity, so the discrete map has no branch switch a small
+    perturbation could cross (no shock, no TENO flags with WENO-JS, HLLC
+    always in its subsonic star region with S* > 0).  Same dict layout and
+    objective window as adjoint_shock; rho0 holds the density perturbation
+    (rho_factor = 1)."""
+    rng = np.random.default_rng(seed)
+    rho0 = 1.0 + amp * _random_phase_field_2d(n, kmax, rng)
+    u = u0 + amp * _random_phase_field_2d(n, kmax, rng)
+    v = v0 + amp * _random_phase_field_2d(n, kmax, rng)
+    p = 1.0 / GAMMA * (1.0 + amp * _random_phase_field_2d(n, kmax, rng))
+    w = max(n // 8, 1)
+    i0 = n // 2 + max(n // 128, 1)
+    j0 = n // 2 - w // 2
+    return {"rho0": rho0, "rho_factor": np.ones((n, n)), "u": u, "v": v, "p": p, "h": 1.0 / n,
+            "window": (slice(j0, j0 + w), slice(i0, i0 + w))}
+
+
 def adjoint_state(case, rho0=None):
     """Conservative state [cell][4] of an adjoint_shock case for the density field rho0."""
     r = (case["rho0"] if rho0 is None else rho0) * case["rho_factor"]

@@ -150,11 +150,14 @@ __device__ __forceinline__ void beta3(T q0, T q1, T q2, T q3, T q4, T& b0, T& b1
   b2 = sq(q2 - T(2.0) * q3 + q4) + T(0.25) * sq(T(3.0) * q2 - T(4.0) * q3 + q4);
 }
 
-// TENO flag -> fixed stencil (interScheme.hpp:417-451)
+// TENO flag -> fixed stencil (interScheme.hpp:417-451).  Flag 0 (every
+// stencil kept: smooth data, the common case) is tested first, so that a
+// wave whose lanes all keep every stencil skips the other cases' blocks.
 template <class T>
 __device__ __forceinline__ T teno_pick(unsigned flag, T q0, T q1, T q2, T q3, T q4) {
+  if (flag == 0u)
+    return T(0.0234375) * q0 - T(0.15625) * q1 + T(0.703125) * q2 + T(0.46875) * q3 - T(0.0390625) * q4;
   switch (flag) {
-  case 0: return T(0.0234375) * q0 - T(0.15625) * q1 + T(0.703125) * q2 + T(0.46875) * q3 - T(0.0390625) * q4;
   case 1: return -T(0.0625) * q1 + T(0.5625) * q2 + T(0.5625) * q3 - T(0.0625) * q4;
   case 2:
   case 3: return T(0.375) * q2 + T(0.75) * q3 - T(0.125) * q4;
@@ -165,6 +168,20 @@ __device__ __forceinline__ T teno_pick(unsigned flag, T q0, T q1, T q2, T q3, T 
   }
 }
 
+// Every stencil of the gamma-ratio cut-off (Teno5_Z and the schemes that
+// share its test) is kept whenever tau <= min_k (beta_k + 1e-40): then
+// tau / (beta_k + 1e-40) <= 1, so 1 <= t_k = 1 + tau / (beta_k + 1e-40) <= 2
+// and 1 <= gamma_k = t_k^6 <= 64 -- in exact arithmetic and in every
+// correctly rounded evaluation alike, rounding being monotone and exact at
+// 1, 2, 4, 64 -- hence ct * sum(gamma) <= 192 ct < 1 <= gamma_k for every
+// cut-off constant ct <= 1e-4 of the family, and no test gamma_k < ct * sum
+// succeeds: the flag is 0, the value the full evaluation returns.  Smooth
+// data (tau = O(h^5), beta = O(h^2)) takes this path without any division.
+// min_k (beta_k + 1e-40) is (min_k beta_k) + 1e-40 (rounding is monotone).
+template <class T> __device__ __forceinline__ bool teno_keeps_all(T tau, T b0, T b1, T b2) {
+  return tau <= smin(smin(b0, b1), b2) + eps_tiny<T>();
+}
+
 // Teno5_Z's cut-off flags (interScheme.hpp:384-452; ct = 1e-5, CT4 1e-4,
 // CT7 1e-7): bit k set where stencil k is cut.
 template <class T>
@@ -172,6 +189,7 @@ __device__ __forceinline__ unsigned teno5_flag(T q0, T q1, T q2, T q3, T q4, T c
   T b0, b1, b2;
   beta3(q0, q1, q2, q3, q4, b0, b1, b2);
   T tau = fabs(b2 - b0);
+  if (teno_keeps_all(tau, b0, b1, b2)) return 0u;
   T t0, t1, t2;
   if (sizeof(T) == sizeof(double)) {
     t0 = T(1.0) + fdv(tau, b0 + eps_tiny<T>());
@@ -215,6 +233,9 @@ __device__ __forceinline__ unsigned cut_flag(T q0, T q1, T q2, T q3, T q4, T ct,
     // the adaptive constant 10^-beta_A, beta_A = floor(10 - 5 (1 - g)),
     // g = s^4 (5 - 4 s), s = min(1, 4.1666667 eta_min)
     T tau = fabs(b2 - b0);
+    // the same cut-off test with a constant 10^-beta_A <= 1e-5: all kept
+    // whatever beta_A is, so the smoothness ratios are not needed
+    if (teno_keeps_all(tau, b0, b1, b2)) return 0u;
     const T epsA = T(2.7551 * 1e-7);
     const T d0 = q0 - q1, d1 = q1 - q2, d2 = q2 - q3, d3 = q3 - q4;
     const T e0 = fdv(fabs(T(2.0) * d0 * d1) + epsA, d0 * d0 + d1 * d1 + epsA);

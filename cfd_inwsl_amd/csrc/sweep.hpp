@@ -118,11 +118,22 @@ template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T
 template <int D, int AX, int SCH, int FLX, class T> struct SweepOcc {
   static constexpr bool dbl = sizeof(T) == sizeof(double);
   static constexpr int waves = (D == 2 && AX == 0 && FLX == FLX_ROE && dbl)                               ? 3
-                               : (D == 3 && dbl && (SCH == SCH_JS || SCH == SCH_Z || SCH == SCH_TENOA)) ? 2
+                               : (D == 3 && dbl && (SCH == SCH_JS || SCH == SCH_Z || SCH == SCH_TENOA || SCH == SCH_TENO)) ? 2
                                                                                                           : 1;
+  // the cell phase's operands are prefetched where the registers allow it
+  static constexpr bool prefetch = waves == 1 || (D == 3 && dbl && SCH == SCH_TENO);
 };
 
 // One workgroup's share of a sweep: pencil group bx, segment by.
+// x mod R for 0 <= x < 2R: min(x, x - R) as unsigned (x - R wraps above x
+// when x < R) -- two VALU instructions where a modulo takes a multiply-high
+// sequence
+template <int R> __device__ __forceinline__ int ring_wrap(int x) {
+  return (int)min((unsigned)x, (unsigned)(x - R));
+}
+// c mod R of a non-negative-or-small signed (wave-uniform) position
+template <int R> __device__ __forceinline__ int ring_mod(int c) { return ((c % R) + R) % R; }
+
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T, bool PF>
 __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, const int by) {
   constexpr int NV = D + 2;
@@ -218,7 +229,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       for (int v = 0; v < NV; v++) pr[v] = Ut[v * a.N + idx];
     }
   };
-  auto commit_cell = [&](int g, const T* raw, bool raw_prim, int flip) {
+  auto commit_cell = [&](int slot, const T* raw, bool raw_prim, int flip) {
     T pr[NV];
     if (raw_prim) {
       for (int v = 0; v < NV; v++) pr[v] = raw[v];
@@ -227,7 +238,6 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       for (int v = 1; v < NV - 1; v++)
         if (v == flip) pr[v] = -pr[v];
     }
-    const int slot = (g + 8 * RC) % RC;
     for (int v = 0; v < NV; v++) cell[cidx(v, slot, lp)] = pr[v];
     if (CHARW) {
       T cc[NV];
@@ -235,16 +245,17 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       for (int v = 0; v < NV; v++) cell[cidx(NV + v, slot, lp)] = cc[v];
     }
   };
-  auto load_cell = [&](int g) {
+  // ring slot of cell g: ring_wrap(base + la) with a wave-uniform base
+  auto load_cell = [&](int g, int slot) {
     T raw[NV];
     bool raw_prim;
     int flip;
     fetch_cell(g, raw, raw_prim, flip);
-    commit_cell(g, raw, raw_prim, flip);
+    commit_cell(slot, raw, raw_prim, flip);
   };
 
   // preload cells c0-4 .. c0
-  if (la < 5 && pvalid) load_cell(c0 - 4 + la);
+  if (la < 5 && pvalid) load_cell(c0 - 4 + la, ring_wrap<RC>(ring_mod<RC>(c0 - 4) + la));
 
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
   const int m_end = c1 + 3;  // interfaces [c0, c1+3)
@@ -254,10 +265,15 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
   if (a.flags & SW_WAVE)
     for (int d = 0; d < D; d++) wmax[d][threadIdx.x] = 0.0;
   for (int m0 = c0; m0 < m_end; m0 += CA) {
+    // wave-uniform ring bases of this chunk: the cell slot of cell m0-4
+    // (interface windows m0-4.., the cell phase's neighbours, the cells
+    // loaded from m0+1 = m0-4+5) and the flux slot of interface m0-3 (the
+    // cell phase's interfaces m0-3.., the interfaces m0.. computed)
+    const int bI = ring_mod<RC>(m0 - 4), bF3 = ring_mod<FR>(m0 - 3);
     // 1. load cells m0+1 .. m0+CA (up to c1+3)
     {
       const int g = m0 + 1 + la;
-      if (pvalid && g <= c1 + 3) load_cell(g);
+      if (pvalid && g <= c1 + 3) load_cell(g, ring_wrap<RC>(bI + 5 + la));
     }
     // operands of this chunk's cell phase (cell m0-3+la), in flight during phase 2
     T pf_r[NV];
@@ -280,7 +296,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       if (pvalid && m < m_end) {
         // window cells m-4+j, j = 0..5; the interface lies between m-2 and m-1
         int sl[6];
-        for (int j = 0; j < 6; j++) sl[j] = (m - 4 + j + 8 * RC) % RC;
+        for (int j = 0; j < 6; j++) sl[j] = ring_wrap<RC>(bI + la + j);
         T outL[NV], outR[NV];
         if (!CHARW) {
           for (int v = 0; v < NV; v++) {
@@ -375,7 +391,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
           if (FLX == FLX_HLLC) hllc_md<D, AX>(outL, outR, F);
           else roe_md<D, AX>(outL, outR, F);
         }
-        const int fs = m % FR;
+        const int fs = ring_wrap<FR>(bF3 + 3 + la);
         for (int v = 0; v < NV; v++) flux[fidx(v, fs, lp)] = F[v];
       }
     }
@@ -385,14 +401,15 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
       const int i = m0 - 3 + la;
       if (pvalid && i >= c0 && i < c1) {
         T pm[NV], pp[NV], fm[NV], fp[NV];
-        const int sm = (i - 1 + 8 * RC) % RC, sp = (i + 1 + 8 * RC) % RC;
+        const int sm = ring_wrap<RC>(bI + la), sp = ring_wrap<RC>(bI + la + 2);
         for (int v = 0; v < NV; v++) {
           pm[v] = cell[cidx(v, sm, lp)];
           pp[v] = cell[cidx(v, sp, lp)];
         }
         node_flux<D, AX>(pm, fm);
         node_flux<D, AX>(pp, fp);
-        const int f0 = i % FR, f1 = (i + 1) % FR, f2 = (i + 2) % FR, f3 = (i + 3) % FR;
+        const int f0 = ring_wrap<FR>(bF3 + la), f1 = ring_wrap<FR>(bF3 + la + 1);
+        const int f2 = ring_wrap<FR>(bF3 + la + 2), f3 = ring_wrap<FR>(bF3 + la + 3);
         const long idx = base + (long)i * sa;
         double dtv = 0.0;
         if (a.flags & SW_RK) dtv = *a.dtp;
@@ -403,7 +420,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
         T src[NV];
         const bool source = AX == D - 1 && (a.flags & SW_SOURCE);
         if (source) {
-          const int si = (i + 8 * RC) % RC;
+          const int si = ring_wrap<RC>(bI + la + 1);
           T pc[NV];
           for (int v = 0; v < NV; v++) pc[v] = cell[cidx(v, si, lp)];
           const double* grav = a.aux->grav;
@@ -486,7 +503,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
 __global__ __launch_bounds__(256, (SweepOcc<D, AX, SCH, FLX, T>::waves)) void sweep_kernel(SweepArgs a) {
   if (a.halt && *a.halt != 0.0) return;
-  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value && SweepOcc<D, AX, SCH, FLX, T>::waves == 1>(
+  sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value && SweepOcc<D, AX, SCH, FLX, T>::prefetch>(
       a, blockIdx.x, blockIdx.y);
 }
 

@@ -205,3 +205,42 @@ def test_entropy_fix_acts_in_the_reference_roe_runs():
     c = np.sqrt(g * p / r)
     frac_y = np.mean(np.abs(v) < 0.05 * (np.abs(v) + c))
     assert frac_y > 0.9
+
+
+def _betas(q):
+    b0 = (q[0] - 2 * q[1] + q[2]) ** 2 + 0.25 * (q[0] - 4 * q[1] + 3 * q[2]) ** 2
+    b1 = (q[1] - 2 * q[2] + q[3]) ** 2 + 0.25 * (q[1] - q[3]) ** 2
+    b2 = (q[2] - 2 * q[3] + q[4]) ** 2 + 0.25 * (3 * q[2] - 4 * q[3] + q[4]) ** 2
+    return b0, b1, b2
+
+
+@pytest.mark.parametrize("scheme", [O.S_TENO5, O.S_TENO5_CT4, O.S_TENO5_CT7, O.S_TENO5_CT10, O.S_TENOA])
+def test_teno_keeps_every_stencil_when_tau_is_below_the_smallest_beta(scheme):
+    """The GPU kernels skip the gamma-ratio cut-off (three divisions and the
+    smoothness ratios of TENO-A) where tau = |beta_2 - beta_0| <= min beta_k +
+    1e-40, returning flag 0: 1 <= gamma_k <= 64 there and ct * sum(gamma) <=
+    192 ct < 1 for every constant of the family.  The oracle's full
+    evaluation agrees on smooth, scaled, tied (tau == min beta) and random
+    windows: the value is the 5th-order linear stencil's, bit for bit."""
+    rng = np.random.default_rng(int(scheme))
+    hits = 0
+    for trial in range(20000):
+        kind = trial % 4
+        if kind == 0:  # smooth samples of random functions
+            h = 10 ** rng.uniform(-4, -0.5)
+            a, b, c, d = rng.normal(size=4)
+            x0 = rng.uniform(-1, 1)
+            q = np.array([a + b * math.sin(c * (x0 + k * h)) + d * (x0 + k * h) ** 3 for k in range(5)])
+        elif kind == 1:  # random windows scaled over many decades
+            q = rng.normal(size=5) * 10 ** rng.uniform(-30, 30)
+        elif kind == 2:  # near-constant data (beta ~ 0)
+            q = 1.0 + rng.normal(size=5) * 10 ** rng.uniform(-300, -8)
+        else:  # mirrored data: beta_0 == beta_2, tau == 0
+            a, b, c = rng.normal(size=3)
+            q = np.array([a, b, c, b, a])
+        b0, b1, b2 = _betas(q)
+        tau = abs(b2 - b0)
+        if tau <= min(b0, b1, b2) + 1e-40:
+            hits += 1
+            assert O.inter5(scheme, q) == O.inter5(O.S_LINEAR, q), q
+    assert hits > 5000
