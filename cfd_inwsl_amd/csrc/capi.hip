@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,43 @@ int cfd_set_error_from_batch(int code, const char* msg) { return cfd_fail(code, 
   } while (0)
 
 #include "handle.hpp"
+
+// ---- scratch of the entry point that allocates nothing ----------------------------
+// cfd_rk3_steps_cfl_noalloc keeps the per-workgroup CFL maxima of each step's
+// last sweep (the partial results cfd_rk3_steps_cfl keeps in a buffer it
+// allocates on demand) in this array: at most CFD_NOALLOC_WAVE_DOUBLES doubles
+// (256 KiB) per GPU, held from the library's first use on that GPU until the
+// process ends.  Host code uses only the device address hipGetSymbolAddress
+// returns for each device (the array's name in host code is not that address).
+#define CFD_NOALLOC_WAVE_DOUBLES 32768
+static __device__ __align__(256) double scratch[CFD_NOALLOC_WAVE_DOUBLES];
+static double* g_scratch_ptr[64];
+static bool g_scratch_ready[64];
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on `device` (looked up once per device,
+// with that device current), or null with the error recorded.
+static double* scratch_on(int device) {
+  if (device < 0 || device >= 64) {
+    cfd_fail(CFD_E_ARG, "device ordinal out of range for the no-allocation scratch");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ready[device]) return g_scratch_ptr[device];
+  int cur = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+  void* p = nullptr;
+  if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != device) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !p) {
+    cfd_fail(CFD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return nullptr;
+  }
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  g_scratch_ready[device] = true;
+  return g_scratch_ptr[device];
+}
 
 // ---- scheme selection ---------------------------------------------------------
 // SolverType::initReconer for EULER (zone_FD/src/src/solverType.cpp:217-594).
@@ -366,7 +404,10 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
   if (ax == h->D - 1 && has_source(h)) a.flags |= SW_SOURCE;
   const long nb = plan_pencil_blocks(h->cfg.n, ax, tile);
   const int nseg = (h->cfg.n[ax] + seg - 1) / seg;
-  if (flags & SW_WAVE) {
+  if ((flags & SW_WAVE) && h->wave_scratch) {  // cfd_rk3_steps_cfl_noalloc (capacity checked there)
+    h->wpart_blocks = nb * nseg;
+    a.wave = h->wave_scratch;
+  } else if (flags & SW_WAVE) {
     const long need = (long)h->D * nb * nseg;
     if (need > h->wpart_cap) {  // once per plan change; the step loops are not captured
       if (h->wpart) (void)hipFree(h->wpart);
@@ -410,10 +451,11 @@ static int rk3(cfd_handle* h) {
 // maxima from the partials of the previous step's last sweep (SW_WAVE)
 static void wave_from_parts(cfd_handle* h) {
   const double* hh = h->cfg.h;
+  const double* part = h->wave_scratch ? h->wave_scratch : h->wpart;
   switch (h->D) {
-  case 1: hipLaunchKernelGGL(k_wave_parts<1>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
-  case 2: hipLaunchKernelGGL(k_wave_parts<2>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
-  default: hipLaunchKernelGGL(k_wave_parts<3>, dim3(1), dim3(256), 0, h->stream, h->wpart, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
+  case 1: hipLaunchKernelGGL(k_wave_parts<1>, dim3(1), dim3(256), 0, h->stream, part, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
+  case 2: hipLaunchKernelGGL(k_wave_parts<2>, dim3(1), dim3(256), 0, h->stream, part, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
+  default: hipLaunchKernelGGL(k_wave_parts<3>, dim3(1), dim3(256), 0, h->stream, part, h->wpart_blocks, hh[0], hh[1], hh[2], h->scal); break;
   }
 }
 
@@ -642,6 +684,32 @@ int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out) {
     HIPCHK(hipStreamSynchronize(h->stream));
   }
   return CFD_OK;
+}
+
+int cfd_rk3_steps_cfl_noalloc(cfd_handle* h, double cfl, int nsteps) {
+  if (!h) return cfd_fail(CFD_E_ARG, "null handle");
+  if (!(cfl > 0) || nsteps < 0) return cfd_fail(CFD_E_ARG, "cfl must be > 0 and nsteps >= 0");
+  // the last sweep of each step writes one maximum per workgroup and direction
+  const int ax = h->D - 1;
+  const long need = (long)h->D * plan_pencil_blocks(h->cfg.n, ax, h->tile[ax]) *
+                    ((h->cfg.n[ax] + h->seg[ax] - 1) / h->seg[ax]);
+  if (need > CFD_NOALLOC_WAVE_DOUBLES)
+    return cfd_fail(CFD_E_ARG, "cfd_rk3_steps_cfl_noalloc: the launch plan's last sweep has " + std::to_string(need) +
+                                   " CFL partial maxima, above the scratch capacity of " +
+                                   std::to_string(CFD_NOALLOC_WAVE_DOUBLES) + " doubles (use longer segments)");
+  double* sc = scratch_on(h->cfg.device);
+  if (!sc) return CFD_E_HIP;
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, h->stream, h->scal + 4, 0.0);
+  h->fuse_wave = true;
+  h->wave_scratch = sc;
+  int rc = 0;
+  for (int s = 0; s < nsteps && rc == 0; s++) {
+    rc = s == 0 ? wave_speed(h, cfl, 1) : finish_dt(h, cfl, 1);
+    if (rc == 0) rc = rk3(h);
+  }
+  h->wave_scratch = nullptr;
+  h->fuse_wave = false;
+  return rc;
 }
 
 int cfd_set_keep_rhs(cfd_handle* h, int keep) {

@@ -36,6 +36,7 @@ struct cfd_handle {
   double* wpart;         // those maxima: D x wpart_blocks
   long wpart_cap;        // doubles allocated
   long wpart_blocks;     // workgroups of the last launch that wrote them
+  double* wave_scratch;  // non-null during cfd_rk3_steps_cfl_noalloc: the maxima go there instead
   double* partial;       // block partial sums for totals
   int npartial;
   int keep_rhs;

@@ -112,7 +112,8 @@ BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_ZONE = 0, 1, 2
 # Every entry point declared in include/congfd.h.
 EXPORTS = [
     "cfd_last_error", "cfd_version", "cfd_create", "cfd_destroy", "cfd_set_state", "cfd_get_state",
-    "cfd_eval_rhs", "cfd_rk3_step", "cfd_rk3_steps_cfl", "cfd_cfl_dt", "cfd_conserved_totals",
+    "cfd_eval_rhs", "cfd_rk3_step", "cfd_rk3_steps_cfl", "cfd_rk3_steps_cfl_noalloc", "cfd_cfl_dt",
+    "cfd_conserved_totals",
     "cfd_steps_to_output", "cfd_synchronize", "cfd_device_buffers", "cfd_time_sweeps", "cfd_copy_probe", "cfd_set_keep_rhs", "cfd_rhs_linf", "cfd_autotune", "cfd_launch_plan", "cfd_set_launch_plan",
     "cfd_batch_create", "cfd_batch_destroy", "cfd_batch_set_state", "cfd_batch_get_state", "cfd_batch_steps_cfl",
     "cfd_batch_times", "cfd_batch_autotune", "cfd_batch_synchronize",
@@ -149,6 +150,7 @@ def load_library():
     lib.cfd_eval_rhs.argtypes = [P, pd]
     lib.cfd_rk3_step.argtypes = [P, ctypes.c_double]
     lib.cfd_rk3_steps_cfl.argtypes = [P, ctypes.c_double, ctypes.c_int, pd]
+    lib.cfd_rk3_steps_cfl_noalloc.argtypes = [P, ctypes.c_double, ctypes.c_int]
     lib.cfd_cfl_dt.argtypes = [P, ctypes.c_double, pd]
     lib.cfd_conserved_totals.argtypes = [P, pd]
     lib.cfd_synchronize.argtypes = [P]

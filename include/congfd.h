@@ -136,6 +136,18 @@ int cfd_rk3_steps_cfl(cfd_handle* h, double cfl, int nsteps, double* t_out);
 int cfd_steps_to_output(cfd_handle* h, double cfl, double output_dt, double* output_t, int max_steps,
                         double* log, int* nsteps, int* reached);
 
+/* cfd_rk3_steps_cfl without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The same steps, dt
+ * and clock as cfd_rk3_steps_cfl(h, cfl, nsteps, NULL), asynchronous; the
+ * per-workgroup CFL maxima that cfd_rk3_steps_cfl keeps in a buffer it
+ * allocates on demand go to a fixed device array of the library instead:
+ * 32768 doubles (256 KiB) per GPU, held until the process ends.  A launch
+ * plan whose last sweep needs more (D x workgroups above 32768) is refused
+ * with CFD_E_ARG.  The array is shared by every call on one GPU: calls that
+ * overlap on the same GPU (other handles, other streams) race. */
+int cfd_rk3_steps_cfl_noalloc(cfd_handle* h, double cfl, int nsteps);
+
 /* When keep != 0 the RK3 calls also store the rhs of each stage (the last
  * stage's rhs stays readable, as the reference's Equation::rhs does after
  * RK3_SSP); off by default, which saves one rhs write per step. */
