@@ -177,9 +177,12 @@ __device__ __forceinline__ T teno_pick(unsigned flag, T q0, T q1, T q2, T q3, T 
 // cut-off constant ct <= 1e-4 of the family, and no test gamma_k < ct * sum
 // succeeds: the flag is 0, the value the full evaluation returns.  Smooth
 // data (tau = O(h^5), beta = O(h^2)) takes this path without any division.
-// min_k (beta_k + 1e-40) is (min_k beta_k) + 1e-40 (rounding is monotone).
+// min_k (beta_k + 1e-40) is (min_k beta_k) + 1e-40 (rounding is monotone);
+// the hardware minimum (NaN-ignoring) gives the same decision as std::min
+// here: a NaN in beta_0 or beta_2 makes tau NaN and the test false either
+// way, and a NaN beta_1 is skipped by both.
 template <class T> __device__ __forceinline__ bool teno_keeps_all(T tau, T b0, T b1, T b2) {
-  return tau <= smin(smin(b0, b1), b2) + eps_tiny<T>();
+  return tau <= fmin(fmin(b0, b1), b2) + eps_tiny<T>();
 }
 
 // Teno5_Z's cut-off flags (interScheme.hpp:384-452; ct = 1e-5, CT4 1e-4,
