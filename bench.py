@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 from cfd_inwsl_amd import (BATCH_GRAPH, BATCH_PER_ZONE, BATCH_PERSISTENT, BC, PRECISION_FP32,  # noqa: E402
-                           PRECISION_FP64, Batch, FluxMethod, InterMethod, Recon, Solver, cases)
+                           PRECISION_FP64, PRECISION_FP64_FMA, Batch, FluxMethod, InterMethod, Recon, Solver, cases)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 
@@ -104,11 +104,13 @@ def bench_batch(nzones=96, seed=20261203, steps=3):
             "speedup_graph_vs_per_zone": round(out["per_zone"]["ms_per_step"] / out["graph"]["ms_per_step"], 2)}
 
 
-def bench_fp32(steps_err=100):
-    """FP32 storage + arithmetic in the sweeps, FP64 U^n and stage combination:
-    throughput on both headline grids, and the L1 / Linf difference to the
-    FP64 path (equal to the oracle bit for bit) after `steps_err` RK3 steps of
-    a fixed dt (config T: 3D 256^3, 100 steps)."""
+def bench_precisions(steps_err=100):
+    """The two opt-in arithmetic modes against the exact FP64 path (equal to
+    the oracle bit for bit): FP32 storage + arithmetic in the sweeps (FP64 U^n
+    and stage combination), and FP64 with fused multiply-adds contracted.
+    Throughput on both headline grids, and the per-variable L1 / Linf
+    difference to the FP64 path after `steps_err` RK3 steps of a fixed dt."""
+    names = {PRECISION_FP64: "fp64", PRECISION_FP32: "fp32", PRECISION_FP64_FMA: "fp64_fma"}
     out = {}
     for name, dim, n, mk, inter, flux in (
             ("2d_vortex_2048", 2, (2048, 2048), lambda: cases.vortex(2048), InterMethod.WCNSZ5, FluxMethod.ROE),
@@ -117,7 +119,7 @@ def bench_fp32(steps_err=100):
         ncell = int(np.prod(n))
         res = {}
         finals = {}
-        for prec in (PRECISION_FP64, PRECISION_FP32):
+        for prec in (PRECISION_FP64, PRECISION_FP32, PRECISION_FP64_FMA):
             s = Solver(dim, n, h, inter, flux, Recon.CHAR, [BC.PERIODIC] * 6, precision=prec)
             s.set_state(cons)
             s.autotune(reps=2)
@@ -130,15 +132,18 @@ def bench_fp32(steps_err=100):
                 s.rk3_step(dt)
             s.synchronize()
             sec = time.perf_counter() - t0
-            res["fp32" if prec else "fp64"] = round(ncell * 3 * steps_err / sec / 1e6, 1)
+            res[names[prec]] = round(ncell * 3 * steps_err / sec / 1e6, 1)
             finals[prec] = s.get_state()
             s.close()
-        a, b = finals[PRECISION_FP32], finals[PRECISION_FP64]
+        b = finals[PRECISION_FP64]
         scale = np.abs(b).max(axis=0)
-        out[name] = {"mcell_stage_s": res, "steps": steps_err,
-                     "L1_rel": [float(x) for x in np.abs(a - b).mean(axis=0) / scale],
-                     "Linf_rel": [float(x) for x in np.abs(a - b).max(axis=0) / scale]}
-        log(f"[bench] fp32 {name}: {out[name]}")
+        diff = {}
+        for prec in (PRECISION_FP32, PRECISION_FP64_FMA):
+            a = finals[prec]
+            diff[names[prec]] = {"L1_rel": [float(x) for x in np.abs(a - b).mean(axis=0) / scale],
+                                 "Linf_rel": [float(x) for x in np.abs(a - b).max(axis=0) / scale]}
+        out[name] = {"mcell_stage_s": res, "steps": steps_err, "vs_fp64": diff}
+        log(f"[bench] precisions {name}: {out[name]}")
     return out
 
 
@@ -284,7 +289,8 @@ def main():
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D 256^3 secondary figure")
     ap.add_argument("--no-large", action="store_true", help="skip the 2D 4096^2 / 3D 384^3 figures")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-zones figure")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage figures")
+    ap.add_argument("--no-precisions", "--no-fp32", dest="no_precisions", action="store_true",
+                    help="skip the FP32-storage and FP64-FMA figures")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint figures")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -353,10 +359,10 @@ def main():
         log("[bench] batched zones")
         batch = bench_batch()
 
-    fp32 = None
-    if not args.no_fp32:
-        log("[bench] FP32 storage variant")
-        fp32 = bench_fp32()
+    precisions = None
+    if not args.no_precisions:
+        log("[bench] FP32-storage and FP64-FMA variants")
+        precisions = bench_precisions()
 
     adj = None
     if not args.no_adjoint:
@@ -390,7 +396,7 @@ def main():
         "secondary": secondary,
         "large": large,
         "batch": batch,
-        "fp32": fp32,
+        "precisions": precisions,
         "adjoint": adj,
     }
     print(json.dumps(line), flush=True)

@@ -7,6 +7,6 @@ selection, test cases).
 """
 from .solver import (  # noqa: F401
     BC, Batch, FluxMethod, InterMethod, Recon, Solver, library_path, load_library, cfd_config,
-    BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_ZONE, PRECISION_FP32, PRECISION_FP64,
+    BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_ZONE, PRECISION_FP32, PRECISION_FP64, PRECISION_FP64_FMA,
 )
 from . import cases  # noqa: F401

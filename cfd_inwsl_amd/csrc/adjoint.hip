@@ -487,6 +487,8 @@ void cfd_adjoint_release(cfd_handle* h) {
 static int adj_supported(cfd_handle* h) {
   const bool teno_family = h->sch == cfd::SCH_TENO || h->sch == cfd::SCH_TENOS || h->sch == cfd::SCH_TENOA ||
                            h->sch == cfd::SCH_TENOSA || (h->sch == cfd::SCH_WHF && (int)h->ct != 15);
+  if (h->fma)
+    return cfd_fail(CFD_E_UNSUPP, "the adjoint differentiates the exact FP64 arithmetic (not CFD_PRECISION_FP64_FMA)");
   if (h->D != 2 || h->fp32 || !h->cw || h->flx != 0 || (h->sch != cfd::SCH_JS && !teno_family))
     return cfd_fail(CFD_E_UNSUPP,
                     "the adjoint covers 2D FP64 characteristic-wise WCNS5 (WENO-JS) or TENO-family schemes "

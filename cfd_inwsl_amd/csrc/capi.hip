@@ -420,6 +420,7 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
     h->wpart_blocks = nb * nseg;
     a.wave = h->wpart;
   }
+  if (h->fma) return launch_sweep_fma(h->D, ax, h->sch, h->flx, h->cw, tile, &a, dim3((unsigned)nb, (unsigned)nseg), h->stream);
   return launch_sweep_dispatch(h->D, ax, h->sch, h->flx, h->cw, tile, h->fp32, &a, nullptr,
                                dim3((unsigned)nb, (unsigned)nseg), h->stream);
 }
@@ -531,11 +532,17 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
   h->ct = ct;
   h->flx = cfg->flux_method;
   h->cw = cfg->recon == CFD_RECON_CHAR;
-  if (cfg->precision != CFD_PRECISION_FP64 && cfg->precision != CFD_PRECISION_FP32) {
+  if (cfg->precision != CFD_PRECISION_FP64 && cfg->precision != CFD_PRECISION_FP32 &&
+      cfg->precision != CFD_PRECISION_FP64_FMA) {
     delete h;
-    return cfd_fail(CFD_E_ARG, "precision must be CFD_PRECISION_FP64 or CFD_PRECISION_FP32");
+    return cfd_fail(CFD_E_ARG, "precision must be CFD_PRECISION_FP64, CFD_PRECISION_FP32 or CFD_PRECISION_FP64_FMA");
   }
   h->fp32 = cfg->precision == CFD_PRECISION_FP32;
+  h->fma = cfg->precision == CFD_PRECISION_FP64_FMA;
+  if (h->fma && !fma_scheme_supported(sch)) {
+    delete h;
+    return cfd_fail(CFD_E_UNSUPP, "CFD_PRECISION_FP64_FMA is built for WENO-JS, WENO-Z, TENO5 and TENO-S");
+  }
   if (h->fp32 && cfg->dim == 1) {
     delete h;
     return cfd_fail(CFD_E_UNSUPP, "FP32 precision is available for dim 2 and 3");

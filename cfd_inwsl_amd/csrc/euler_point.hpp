@@ -42,13 +42,24 @@ template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) 
 // ones, bit for bit (except that a zero dividend gives +0 where IEEE may give
 // -0, which no later operation of the path distinguishes).  8 instead of 11
 // and 13 instead of 19 instructions; FP32 keeps the compiler's sequences.
+//
+// The translation units of the opt-in FMA-contracted mode (CFD_FMA_MODE 1)
+// take one Newton step fewer in each: the residual correction that ends both
+// sequences squares the remaining error once more, so the quotient and root
+// stay within an ulp (almost always the correctly rounded one) for two
+// instructions less.
+#ifndef CFD_FMA_MODE
+#define CFD_FMA_MODE 0
+#endif
 template <class T> __device__ __forceinline__ T fdv(T a, T b) { return a / b; }
 template <> __device__ __forceinline__ double fdv<double>(double a, double b) {
   double r = __builtin_amdgcn_rcp(b);
   double e = __builtin_fma(-b, r, 1.0);
   r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-b, r, 1.0);
-  r = __builtin_fma(r, e, r);
+  if (!CFD_FMA_MODE) {
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+  }
   const double q = a * r;
   const double t = __builtin_fma(-b, q, a);
   return __builtin_fma(t, r, q);
@@ -62,8 +73,10 @@ template <> __device__ __forceinline__ double fsq<double>(double x) {
   h = __builtin_fma(h, r, h);
   double d = __builtin_fma(-g, g, x);
   g = __builtin_fma(d, h, g);
-  d = __builtin_fma(-g, g, x);
-  g = __builtin_fma(d, h, g);
+  if (!CFD_FMA_MODE) {
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+  }
   return x == 0.0 ? x : g;
 }
 

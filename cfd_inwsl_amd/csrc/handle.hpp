@@ -22,6 +22,7 @@ struct cfd_handle {
   void *Ua, *Ub, *R;     // stage states and rhs: element type of the sweeps
   float* U0f;            // FP32 shadow of U0 (FP32 precision only)
   bool fp32;
+  bool fma;              // CFD_PRECISION_FP64_FMA: the FMA-contracted FP64 sweep kernels
   size_t esz;            // bytes per element of Ua / Ub / R
   // device scalars: [0] dt, [1..3] max wave speed per direction (bits), [4] time
   // elapsed in the current call, [5] max |rhs| (bits), [6] absolute time,

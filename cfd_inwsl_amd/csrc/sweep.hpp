@@ -113,8 +113,10 @@ template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T
 // * the 3D FP64 WENO-JS / WENO-Z / TENO-A kernels would otherwise take more
 //   than 256 registers and run one wave per SIMD; bounded to two they spill
 //   a few registers and run 17-41 % faster at 256^3 (without the prefetch,
-//   which bounded kernels do not take).  TENO5 and TENO-S already fit two
-//   waves and lose 12-15 % when bounded.
+//   which bounded kernels do not take).  The 3D TENO5 kernel (its keep-all
+//   test costs registers) is bounded to two waves too but keeps the
+//   prefetch: without it the same bound is 15 % slower.  TENO-S fits two
+//   waves by itself and loses 12-15 % when bounded.
 template <int D, int AX, int SCH, int FLX, class T> struct SweepOcc {
   static constexpr bool dbl = sizeof(T) == sizeof(double);
   static constexpr int waves = (D == 2 && AX == 0 && FLX == FLX_ROE && dbl)                               ? 3
@@ -525,6 +527,18 @@ __global__ __launch_bounds__(256, (SweepOcc<D, AX, SCH, FLX, T>::waves)) void sw
   if (a.halt && *a.halt != 0.0) return;
   sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, SweepPrefetch<T>::value && SweepOcc<D, AX, SCH, FLX, T>::prefetch>(
       a, blockIdx.x, blockIdx.y);
+}
+
+// The same kernel for the FMA-contracted FP64 mode (CFD_PRECISION_FP64_FMA):
+// instantiated only in the translation units compiled with
+// -ffp-contract=fast (sweep_fma_d*.hip), where every a*b + c of the path may
+// be evaluated with one rounding; its own name keeps those kernels apart from
+// the exact ones.
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE>
+__global__ __launch_bounds__(256, (SweepOcc<D, AX, SCH, FLX, double>::waves)) void sweep_kernel_fma(SweepArgs a) {
+  if (a.halt && *a.halt != 0.0) return;
+  sweep_block<D, AX, SCH, FLX, CHARW, TILE, double, SweepOcc<D, AX, SCH, FLX, double>::prefetch>(a, blockIdx.x,
+                                                                                              blockIdx.y);
 }
 
 // ---- batched zones ------------------------------------------------------------

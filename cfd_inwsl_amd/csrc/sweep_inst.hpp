@@ -67,9 +67,46 @@ static hipError_t launch_sch(int sch, int flx, bool cw, int tile, bool fp32, con
   }
 }
 
+// Launch tables of the FMA-contracted FP64 kernels (single zones): the
+// schemes of the benchmark configurations -- WENO-JS, WENO-Z, TENO5 (with its
+// cut-off constants), TENO-S -- with HLLC or Roe, characteristic- or
+// component-wise.  Instantiated only by the sweep_fma_d*.hip units.
+template <int D, int AX, int SCH, int FLX, bool CW>
+static hipError_t launch_fma_one(int tile, const SweepArgs* a, dim3 grid, hipStream_t s) {
+  if (tile == TilePair<AX>::A)
+    hipLaunchKernelGGL((sweep_kernel_fma<D, AX, SCH, FLX, CW, TilePair<AX>::A>), grid, dim3(256), 0, s, *a);
+  else if (tile == TilePair<AX>::B)
+    hipLaunchKernelGGL((sweep_kernel_fma<D, AX, SCH, FLX, CW, TilePair<AX>::B>), grid, dim3(256), 0, s, *a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+template <int D, int AX, int SCH>
+static hipError_t launch_fma_flx(int flx, bool cw, int tile, const SweepArgs* a, dim3 grid, hipStream_t s) {
+  if (flx == FLX_HLLC)
+    return cw ? launch_fma_one<D, AX, SCH, FLX_HLLC, true>(tile, a, grid, s)
+              : launch_fma_one<D, AX, SCH, FLX_HLLC, false>(tile, a, grid, s);
+  return cw ? launch_fma_one<D, AX, SCH, FLX_ROE, true>(tile, a, grid, s)
+            : launch_fma_one<D, AX, SCH, FLX_ROE, false>(tile, a, grid, s);
+}
+template <int D, int AX>
+static hipError_t launch_fma_sch(int sch, int flx, bool cw, int tile, const SweepArgs* a, dim3 grid, hipStream_t s) {
+  switch (sch) {
+  case SCH_JS: return launch_fma_flx<D, AX, SCH_JS>(flx, cw, tile, a, grid, s);
+  case SCH_Z: return launch_fma_flx<D, AX, SCH_Z>(flx, cw, tile, a, grid, s);
+  case SCH_TENO: return launch_fma_flx<D, AX, SCH_TENO>(flx, cw, tile, a, grid, s);
+  case SCH_TENOS: return launch_fma_flx<D, AX, SCH_TENOS>(flx, cw, tile, a, grid, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+inline bool fma_scheme_supported(int sch) { return sch == SCH_JS || sch == SCH_Z || sch == SCH_TENO || sch == SCH_TENOS; }
+
 // Entry points defined in sweep_d<D>_<axis>.hip
 // fp32: FP32 storage / arithmetic kernels (not compiled for 1D).
 hipError_t launch_sweep_dispatch(int D, int ax, int sch, int flx, bool cw, int tile, bool fp32, const SweepArgs* a,
                                  const BatchArgs* b, dim3 grid, hipStream_t s);
+// the FMA-contracted FP64 kernels (sweep_fma_d<D>_<axis>.hip)
+hipError_t launch_sweep_fma(int D, int ax, int sch, int flx, bool cw, int tile, const SweepArgs* a, dim3 grid,
+                            hipStream_t s);
 
 }  // namespace cfd

@@ -94,7 +94,7 @@ class cfd_config(ctypes.Structure):
     ]
 
 
-PRECISION_FP64, PRECISION_FP32 = 0, 1
+PRECISION_FP64, PRECISION_FP32, PRECISION_FP64_FMA = 0, 1, 2
 
 
 class cfd_zone(ctypes.Structure):

@@ -78,10 +78,11 @@ typedef struct cfd_config {
   int bc[6];            /* per face: x-, x+, y-, y+, z-, z+ */
   double bc_prim[6][5]; /* Dirichlet ghost state per face, primitive variables */
   int device;           /* HIP device ordinal */
-  int precision;        /* CFD_PRECISION_FP64 (reference arithmetic, bitwise) or
+  int precision;        /* CFD_PRECISION_FP64 (reference arithmetic, bitwise),
                            CFD_PRECISION_FP32 (FP32 storage and arithmetic of the
-                           sweeps; U^n and the RK3 stage combination stay FP64);
-                           FP32 is available for dim 2 and 3, single zones */
+                           sweeps; U^n and the RK3 stage combination stay FP64;
+                           dim 2 and 3, single zones) or CFD_PRECISION_FP64_FMA
+                           (below) */
   double origin;        /* cell centres at origin + (i + 0.5) h[0] in every direction
                            (Initializer::initUniformBlock, initializer.cpp:672-677);
                            used by the double-Mach boundaries */
@@ -93,6 +94,16 @@ typedef struct cfd_config {
 
 #define CFD_PRECISION_FP64 0
 #define CFD_PRECISION_FP32 1
+/* FP64 storage and arithmetic with FMA contraction (a*b + c rounded once)
+ * in the sweeps, and divisions / square roots with one Newton step fewer
+ * (within an ulp).  Opt-in: results differ from the reference arithmetic by
+ * rounding (one rhs within 1e-12 of S_v * max(|u| + c) / h per variable,
+ * S_v the variable's magnitude scale; a branch predicate tied at rounding
+ * level -- a TENO cut-off, an HLLC wave region -- may be decided the other
+ * way).  Single zones, WENO-JS / WENO-Z / TENO5 (and its cut-off variants) /
+ * TENO-S, any flux, dimension and recon; no adjoint.  No counterpart in the
+ * reference, which has one arithmetic. */
+#define CFD_PRECISION_FP64_FMA 2
 
 typedef struct cfd_handle cfd_handle;
 

@@ -81,6 +81,26 @@ def test_fp32_rejected_for_1d_and_batches():
     assert lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h)) < 0
 
 
+def test_fma_precision_limited_to_its_schemes():
+    """CFD_PRECISION_FP64_FMA is built for WENO-JS, WENO-Z, TENO5, TENO-S;
+    any other scheme is refused before any GPU call."""
+    lib = solver.load_library()
+    cfg = solver.cfd_config()
+    cfg.dim = 2
+    cfg.n[0], cfg.n[1], cfg.n[2] = 16, 16, 1
+    cfg.h[0] = cfg.h[1] = cfg.h[2] = 0.1
+    cfg.flux_method, cfg.diff_method, cfg.recon = 0, 3, 0
+    for i in range(6):
+        cfg.bc[i] = 7
+    cfg.precision = solver.PRECISION_FP64_FMA
+    h = ctypes.c_void_p()
+    for inter in (solver.InterMethod.NEW_WHFTCNSA, solver.InterMethod.MUCSLIN5, solver.InterMethod.WHFTCNSN):
+        cfg.inter_method = int(inter)
+        assert lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h)) < 0
+        assert "FP64_FMA is built for" in lib.cfd_last_error().decode()
+        assert not h.value
+
+
 def _batch_cfg():
     cfg = solver.cfd_config()
     cfg.dim = 2
