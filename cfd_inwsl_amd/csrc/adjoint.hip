@@ -98,6 +98,10 @@ struct VjpArgs {
   double h;
   int bc_lo, bc_hi;
   const double* bc_prim;  // Dirichlet ghost states of this direction: low face [0..3], high [4..7]
+  // k_rhs_vjp<..., REC = true>: the recording sweep's record of this state
+  // and direction (SweepArgs::rec: QL, QR planes, then the branch words)
+  const double* rec;
+  long rec_plane;
 };
 
 // Forward record of one interface in the forward sweep's own arithmetic
@@ -126,7 +130,6 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
       for (int v = 0; v < 4; v++) c[v] = w.cons(j, v);
       ch[j] = cfd::project<2>(c, l);
     }
-    for (int v = 0; v < 4; v++) f.L[k][v] = l[v];
     if constexpr (IP::kLinear) {
       const unsigned fl = cfd::cut_flag<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, rc);
       const unsigned fr = cfd::cut_flag<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct, rc);
@@ -145,7 +148,6 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
     double r[4];
     es.rcol(k, r);
     for (int i = 0; i < 4; i++) {
-      f.R[i][k] = r[i];
       if (t == 0) {
         f.cL[i] = f.QL[k] * r[i];
         f.cR[i] = f.QR[k] * r[i];
@@ -170,7 +172,10 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
   f.hb = cfdadj::hllc_branch_bits<AX>(f.oL, f.oR);
 }
 
-template <int AX, int CA, int CP, class IP, int SCH>
+// REC: the interface's half-node values and branch bits come from the
+// recording sweep of the same state (iface_from_record) instead of an exact
+// re-evaluation of the interface.
+template <int AX, int CA, int CP, class IP, int SCH, bool REC>
 __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
   __shared__ double wc[8 * RC * CP];  // primitive state, then the conservative state rebuilt from it
@@ -296,9 +301,18 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     }
     // the next chunk's new ring cell
     if (pvalid && k0 + CA - 3 < e1 && k0 + CA + 2 + la <= kmax + 2) fetch(k0 + CA + 2 + la, nf);
-    __syncthreads();
     const int k = k0 + la;
     const bool act = pvalid && k <= kmax;
+    // REC: this chunk's interface record, in flight across the barrier
+    double rq[8];
+    unsigned rbits = 0u;
+    if (REC && act) {
+      const long ri = AX == 0 ? gp * (n + 3) + (k + 1) : (long)(k + 1) * npencil + gp;
+#pragma unroll
+      for (int v = 0; v < 8; v++) rq[v] = a.rec[v * a.rec_plane + ri];
+      rbits = reinterpret_cast<const unsigned*>(a.rec + 8 * a.rec_plane)[ri];
+    }
+    __syncthreads();
     double lw[6][4];
     for (int j = 0; j < 6; j++)
       for (int v = 0; v < 4; v++) lw[j][v] = 0.0;
@@ -307,7 +321,10 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) * ih;
       const Cells cells(wc, k, lp);
       cfdadj::IfaceFwd f;
-      iface_forward_exact<AX, IP, SCH>(cells, f, a.ct, a.rc);
+      if (REC)
+        cfdadj::iface_from_record<AX>(cells, rq, rbits, f);
+      else
+        iface_forward_exact<AX, IP, SCH>(cells, f, a.ct, a.rc);
       cfdadj::iface_backward<AX, IP>(cells, f, lG, lw);
     }
     if (act) {
@@ -460,6 +477,15 @@ struct AdjointStore {
   bool last_in_store = false;  // the stage store holds the last segment (written by the forward)
   double dt = 0.0;
   int vjp_seg[2] = {0, 0};
+  // interface records of the segment's stage states (recording sweeps), one
+  // block per stage state q = 3i (S_i), 3i+1 (u1_i), 3i+2 (u2_i): direction 0
+  // then 1, each 8 planes of rec_plane[ax] doubles and the branch words
+  double* rec = nullptr;
+  int rec_cap = 0;            // stage states the record buffer holds
+  long rec_plane[2] = {0, 0};
+  size_t rec_off[2] = {0, 0};  // doubles from the block start
+  size_t rec_block = 0;        // doubles per stage state
+  double* rec_at(int q, int ax) const { return rec + (size_t)q * rec_block + rec_off[ax]; }
 };
 
 #define AHIP(expr)                                                                      \
@@ -479,6 +505,7 @@ void cfd_adjoint_release(cfd_handle* h) {
   (void)hipFree(s->bcprim);
   (void)hipFree(s->GH[1]);
   (void)hipFree(s->stage);
+  (void)hipFree(s->rec);
   for (double* p : s->ckpt) (void)hipFree(p);
   delete s;
   h->adj = nullptr;
@@ -560,33 +587,59 @@ static int adj_ensure(cfd_handle* h) {
     const int cp = ax == 0 ? VjpTile<0>::CP : VjpTile<1>::CP;
     s->vjp_seg[ax] = vjp_plan_seg(h->cfg.n[ax], (pencils + cp - 1) / cp, ca, cp);
   }
+  size_t off = 0;
+  for (int ax = 0; ax < 2; ax++) {
+    const long P = (long)(h->cfg.n[ax] + 3) * (ax == 0 ? h->cfg.n[1] : h->cfg.n[0]);
+    s->rec_plane[ax] = P;
+    s->rec_off[ax] = off;
+    off += (size_t)(8 * P + (P + 1) / 2 + 63) / 64 * 64;  // 8 planes + the words, 512-byte aligned
+  }
+  s->rec_block = off;
   return 0;
+}
+
+// the record pointers of stage state q (null: no records)
+struct RecPair {
+  void* p[2];
+};
+static RecPair rec_of(const AdjointStore* s, int q) {
+  if (!s->rec) return {{nullptr, nullptr}};
+  return {{s->rec_at(q, 0), s->rec_at(q, 1)}};
 }
 
 // The VJP kernel of a scheme: WENO-JS's reverse pass, or the TENO family's
 // (the flag's stencil, the flag from the scheme's own cut-off).
-template <int AX, int CA, int CP>
-static void launch_vjp(int sch, dim3 grid, hipStream_t s, const VjpArgs& a) {
+template <int AX, int CA, int CP, bool REC>
+static void launch_vjp_rec(int sch, dim3 grid, hipStream_t s, const VjpArgs& a) {
   using cfdadj::JS;
   using cfdadj::Teno;
   switch (sch) {
-  case cfd::SCH_JS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, JS, cfd::SCH_JS>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENO: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENO>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENOS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOS>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENOA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOA>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENOSA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOSA>), grid, dim3(CA * CP), 0, s, a); break;
-  default: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_WHF>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_JS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, JS, cfd::SCH_JS, REC>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENO: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENO, REC>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENOS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOS, REC>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENOA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOA, REC>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_TENOSA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOSA, REC>), grid, dim3(CA * CP), 0, s, a); break;
+  default: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_WHF, REC>), grid, dim3(CA * CP), 0, s, a); break;
   }
+}
+template <int AX, int CA, int CP>
+static void launch_vjp(int sch, dim3 grid, hipStream_t s, const VjpArgs& a) {
+  if (a.rec)
+    launch_vjp_rec<AX, CA, CP, true>(sch, grid, s, a);
+  else
+    launch_vjp_rec<AX, CA, CP, false>(sch, grid, s, a);
 }
 
 // OUT = base + coef * (d rhs / d U)^T MU at the state U, both directions;
 // base = init's combination (null init: the current OUT).
 static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef, double* OUT,
-                   const VjpInit* init) {
+                   const VjpInit* init, const RecPair& rec) {
   AdjointStore* s = h->adj;
   for (int ax = 0; ax < 2; ax++) {
     VjpArgs a;
     memset(&a, 0, sizeof(a));
+    a.rec = static_cast<const double*>(rec.p[ax]);
+    a.rec_plane = s->rec_plane[ax];
     a.U = U;
     a.MU = MU;
     a.OUT = OUT;
@@ -633,16 +686,18 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
   return 0;
 }
 
-// One RK3 step backwards: s->LA holds l3 on entry and l0 on exit.
-static int adj_step_back(cfd_handle* h, const double* u0, const double* u1, const double* u2, double dt) {
+// One RK3 step backwards: s->LA holds l3 on entry and l0 on exit; q0 is the
+// stage-state index of u0 in the record buffer (-1: no records).
+static int adj_step_back(cfd_handle* h, const double* u0, const double* u1, const double* u2, double dt, int q0) {
   AdjointStore* s = h->adj;
+  const RecPair none = {{nullptr, nullptr}};
   int rc;
   const VjpInit i2 = {{s->LA, nullptr, nullptr}, {2.0 / 3.0, 0.0, 0.0}};
-  if ((rc = adj_vjp(h, u2, s->LA, -2.0 / 3.0 * dt, s->LB, &i2))) return rc;
+  if ((rc = adj_vjp(h, u2, s->LA, -2.0 / 3.0 * dt, s->LB, &i2, q0 < 0 ? none : rec_of(s, q0 + 2)))) return rc;
   const VjpInit i1 = {{s->LB, nullptr, nullptr}, {0.25, 0.0, 0.0}};
-  if ((rc = adj_vjp(h, u1, s->LB, -0.25 * dt, s->LC, &i1))) return rc;
+  if ((rc = adj_vjp(h, u1, s->LB, -0.25 * dt, s->LC, &i1, q0 < 0 ? none : rec_of(s, q0 + 1)))) return rc;
   const VjpInit i0 = {{s->LA, s->LB, s->LC}, {1.0 / 3.0, 0.75, 1.0}};
-  if ((rc = adj_vjp(h, u0, s->LC, -dt, s->LA, &i0))) return rc;
+  if ((rc = adj_vjp(h, u0, s->LC, -dt, s->LA, &i0, q0 < 0 ? none : rec_of(s, q0)))) return rc;
   return 0;
 }
 
@@ -656,7 +711,7 @@ int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad) {
   AHIP(hipMemcpyAsync(s->LC, cot, s->state_bytes, hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_aos_to_soa4, dim3(nb), dim3(256), 0, h->stream, s->LC, s->LA, h->N);
   const VjpInit zero = {{nullptr, nullptr, nullptr}, {0.0, 0.0, 0.0}};
-  if (int rc = adj_vjp(h, h->U0, s->LA, 1.0, s->LB, &zero)) return rc;
+  if (int rc = adj_vjp(h, h->U0, s->LA, 1.0, s->LB, &zero, {{nullptr, nullptr}})) return rc;
   hipLaunchKernelGGL(k_soa_to_aos4, dim3(nb), dim3(256), 0, h->stream, s->LB, s->LC, h->N);
   AHIP(hipGetLastError());
   AHIP(hipMemcpyAsync(grad, s->LC, s->state_bytes, hipMemcpyDeviceToHost, h->stream));
@@ -694,6 +749,15 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
     AHIP(hipMalloc(&s->stage, s->state_bytes * need));
     s->stage_cap = need;
   }
+  // interface records of one segment's stage states (not when every stage
+  // state is kept: the VJPs then re-evaluate their interfaces)
+  if (!keep_all && s->rec_cap < 3 * seg) {
+    if (s->rec) AHIP(hipFree(s->rec));
+    s->rec = nullptr;
+    s->rec_cap = 0;
+    AHIP(hipMalloc(&s->rec, sizeof(double) * s->rec_block * 3 * seg));
+    s->rec_cap = 3 * seg;
+  }
   s->every = checkpoint_every;
   s->dt = dt;
   hipLaunchKernelGGL(k_put_dt, dim3(1), dim3(1), 0, h->stream, h->scal, dt);
@@ -728,9 +792,9 @@ int cfd_adjoint_forward(cfd_handle* h, double dt, int nsteps, int checkpoint_eve
     AHIP(hipMemcpyAsync(st, h->U0, s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
     for (int i = 0; i < nsteps - last; i++) {
       double *S0 = st + (size_t)(3 * i) * nd, *S1 = S0 + nd, *S2 = S1 + nd, *S3 = S2 + nd;
-      if ((rc = rk_stage(h, 1, S0, S0, S1, nullptr))) return rc;
-      if ((rc = rk_stage(h, 2, S1, S0, S2, nullptr))) return rc;
-      if ((rc = rk_stage(h, 3, S2, S0, S3, nullptr))) return rc;
+      if ((rc = rk_stage(h, 1, S0, S0, S1, nullptr, rec_of(s, 3 * i).p))) return rc;
+      if ((rc = rk_stage(h, 2, S1, S0, S2, nullptr, rec_of(s, 3 * i + 1).p))) return rc;
+      if ((rc = rk_stage(h, 3, S2, S0, S3, nullptr, rec_of(s, 3 * i + 2).p))) return rc;
     }
     AHIP(hipMemcpyAsync(h->U0, st + (size_t)(3 * (nsteps - last)) * nd, s->state_bytes, hipMemcpyDeviceToDevice,
                         h->stream));
@@ -756,7 +820,7 @@ int cfd_adjoint_backward(cfd_handle* h, const double* cot_final, double* grad_in
   int rc;
   if (s->every == 0) {
     for (int i = s->nsteps - 1; i >= 0; i--)
-      if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt))) return rc;
+      if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt, -1))) return rc;
   } else {
     hipLaunchKernelGGL(k_put_dt, dim3(1), dim3(1), 0, h->stream, h->scal, s->dt);
     const int nck = (int)s->ckpt.size();
@@ -766,15 +830,16 @@ int cfd_adjoint_backward(cfd_handle* h, const double* cot_final, double* grad_in
       if (!(m == nck - 1 && s->last_in_store)) {
         s->last_in_store = false;  // the store now holds another segment
         AHIP(hipMemcpyAsync(S(0), s->ckpt[m], s->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+        // every stage records its interfaces; the last step's third stage
+        // runs for its record only (its state S(len) is not used)
         for (int i = 0; i < len; i++) {
-          if ((rc = rk_stage(h, 1, S(i), S(i), U1(i), nullptr))) return rc;
-          if ((rc = rk_stage(h, 2, U1(i), S(i), U2(i), nullptr))) return rc;
-          if (i + 1 < len)
-            if ((rc = rk_stage(h, 3, U2(i), S(i), S(i + 1), nullptr))) return rc;
+          if ((rc = rk_stage(h, 1, S(i), S(i), U1(i), nullptr, rec_of(s, 3 * i).p))) return rc;
+          if ((rc = rk_stage(h, 2, U1(i), S(i), U2(i), nullptr, rec_of(s, 3 * i + 1).p))) return rc;
+          if ((rc = rk_stage(h, 3, U2(i), S(i), S(i + 1), nullptr, rec_of(s, 3 * i + 2).p))) return rc;
         }
       }
       for (int i = len - 1; i >= 0; i--)
-        if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt))) return rc;
+        if ((rc = adj_step_back(h, S(i), U1(i), U2(i), s->dt, 3 * i))) return rc;
     }
   }
   hipLaunchKernelGGL(k_soa_to_aos4, dim3(nb), dim3(256), 0, h->stream, s->LA, s->LB, h->N);

@@ -508,6 +508,33 @@ template <int AX> struct Roe2 {
     R[2][0] = -nx; R[2][1] = v * ih; R[2][2] = v - ny * c; R[2][3] = v + ny * c;
     R[3][0] = ny * u - nx * v; R[3][1] = ek * ih; R[3][2] = h + ek - Vn * c; R[3][3] = h + ek + Vn * c;
   }
+  // row k of L, column k of R (the entries of mats)
+  CFD_HD void lrow(int k, double* l) const {
+    CFD_CONTRACT
+    const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+    if (k == 0) {
+      l[0] = -ny * u + nx * v; l[1] = ny; l[2] = -nx; l[3] = 0.0;
+    } else if (k == 1) {
+      l[0] = h - ek; l[1] = u; l[2] = v; l[3] = -1.0;
+    } else if (k == 2) {
+      l[0] = 0.5 * (Vn * ic + ek * ih); l[1] = 0.5 * (-nx * ic - u * ih); l[2] = 0.5 * (-ny * ic - v * ih); l[3] = 0.5 * ih;
+    } else {
+      l[0] = 0.5 * (-Vn * ic + ek * ih); l[1] = 0.5 * (nx * ic - u * ih); l[2] = 0.5 * (ny * ic - v * ih); l[3] = 0.5 * ih;
+    }
+  }
+  CFD_HD void rcol(int k, double* r) const {
+    CFD_CONTRACT
+    const double nx = AX == 0 ? 1.0 : 0.0, ny = AX == 1 ? 1.0 : 0.0;
+    if (k == 0) {
+      r[0] = 0.0; r[1] = ny; r[2] = -nx; r[3] = ny * u - nx * v;
+    } else if (k == 1) {
+      r[0] = ih; r[1] = u * ih; r[2] = v * ih; r[3] = ek * ih;
+    } else if (k == 2) {
+      r[0] = 1.0; r[1] = u - nx * c; r[2] = v - ny * c; r[3] = h + ek - Vn * c;
+    } else {
+      r[0] = 1.0; r[1] = u + nx * c; r[2] = v + ny * c; r[3] = h + ek + Vn * c;
+    }
+  }
   // Reverse pass in pieces, for register-lean callers: cotangents of the
   // Roe-averaged quantities are accumulated entry by entry of L and R, then
   // chained back to the two cell states.
@@ -727,7 +754,6 @@ CFD_HD inline GhostSrc ghost_source(int bc, int g, int n) {
 // ---- interface flux as a forward record + reverse pass ----------------------------
 // What the reverse pass of one interface needs from its forward evaluation.
 struct IfaceFwd {
-  double L[4][4], R[4][4];  // left eigenvectors (rows = waves) and right (R[i][k])
   double QL[4], QR[4];      // characteristic half-node values
   double cL[4], cR[4];      // conservative half-node states
   double oL[4], oR[4];      // primitive states given to the flux
@@ -774,12 +800,13 @@ template <int AX, class IP = JS, class W> CFD_HD inline void iface_forward(const
   }
   Roe2<AX> es;
   es.fwd(pl, pr);
-  es.mats(f.L, f.R);
+  double L[4][4], R[4][4];
+  es.mats(L, R);
   f.tflag = 0u;
   for (int k = 0; k < 4; k++) {
     double ch[6];
     for (int j = 0; j < 6; j++)
-      ch[j] = f.L[k][0] * w.cons(j, 0) + f.L[k][1] * w.cons(j, 1) + f.L[k][2] * w.cons(j, 2) + f.L[k][3] * w.cons(j, 3);
+      ch[j] = L[k][0] * w.cons(j, 0) + L[k][1] * w.cons(j, 1) + L[k][2] * w.cons(j, 2) + L[k][3] * w.cons(j, 3);
     double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
     double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
     IP a = proto, b = proto;
@@ -790,8 +817,8 @@ template <int AX, class IP = JS, class W> CFD_HD inline void iface_forward(const
     if constexpr (IP::kLinear) f.tflag |= (a.flag | (b.flag << 3)) << (6 * k);
   }
   for (int i = 0; i < 4; i++) {
-    f.cL[i] = f.R[i][0] * f.QL[0] + f.R[i][1] * f.QL[1] + f.R[i][2] * f.QL[2] + f.R[i][3] * f.QL[3];
-    f.cR[i] = f.R[i][0] * f.QR[0] + f.R[i][1] * f.QR[1] + f.R[i][2] * f.QR[2] + f.R[i][3] * f.QR[3];
+    f.cL[i] = R[i][0] * f.QL[0] + R[i][1] * f.QL[1] + R[i][2] * f.QL[2] + R[i][3] * f.QL[3];
+    f.cR[i] = R[i][0] * f.QR[0] + R[i][1] * f.QR[1] + R[i][2] * f.QR[2] + R[i][3] * f.QR[3];
   }
   c2p(f.cL, f.oL);
   c2p(f.cR, f.oR);
@@ -806,6 +833,52 @@ template <int AX, class IP = JS, class W> CFD_HD inline void iface_forward(const
     f.oR[3] = pr[3];
   }
   f.hb = -1;
+}
+
+// Forward record from what the solver's recording sweep kept of the
+// interface (sweep_kernel_rec): the characteristic half-node values Q[0..3]
+// (left) and Q[4..7] (right) and the branch word (TENO flags in bits 0..23,
+// positivity fallback left / right in 24 / 25, HLLC branch bits from 26).
+// The branches are the solver's own; the states rebuilt from Q only feed
+// derivatives, so they are evaluated in this file's contracted arithmetic.
+template <int AX, class W>
+CFD_HD inline void iface_from_record(const W& w, const double* Q, unsigned bits, IfaceFwd& f) {
+  CFD_CONTRACT
+  double pl[4], pr[4];
+  for (int v = 0; v < 4; v++) {
+    pl[v] = w(2, v);
+    pr[v] = w(3, v);
+  }
+  Roe2<AX> es;
+  es.fwd(pl, pr);
+  for (int i = 0; i < 4; i++) {
+    f.QL[i] = Q[i];
+    f.QR[i] = Q[4 + i];
+    f.cL[i] = 0.0;
+    f.cR[i] = 0.0;
+  }
+  for (int k = 0; k < 4; k++) {
+    double r[4];
+    es.rcol(k, r);
+    for (int i = 0; i < 4; i++) {
+      f.cL[i] += f.QL[k] * r[i];
+      f.cR[i] += f.QR[k] * r[i];
+    }
+  }
+  c2p(f.cL, f.oL);
+  c2p(f.cR, f.oR);
+  f.tflag = bits & 0xffffffu;
+  f.fbL = (bits >> 24) & 1u;
+  f.fbR = (bits >> 25) & 1u;
+  f.hb = (int)(bits >> 26);
+  if (f.fbL)
+    for (int v = 0; v < 4; v++) f.oL[v] = pl[v];
+  if (f.fbR) {
+    f.oR[0] = pr[0];
+    f.oR[1] = pl[1];
+    f.oR[2] = pl[2];
+    f.oR[3] = pr[3];
+  }
 }
 
 // Reverse pass of the interface flux given its forward record: lw[j][v] +=
@@ -847,8 +920,10 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
   double lQL[4], lQR[4];
   CFD_UNROLL
   for (int k = 0; k < 4; k++) {
-    lQL[k] = f.R[0][k] * lcL[0] + f.R[1][k] * lcL[1] + f.R[2][k] * lcL[2] + f.R[3][k] * lcL[3];
-    lQR[k] = f.R[0][k] * lcR[0] + f.R[1][k] * lcR[1] + f.R[2][k] * lcR[2] + f.R[3][k] * lcR[3];
+    double r[4];
+    es.rcol(k, r);
+    lQL[k] = r[0] * lcL[0] + r[1] * lcL[1] + r[2] * lcL[2] + r[3] * lcL[3];
+    lQR[k] = r[0] * lcR[0] + r[1] * lcR[1] + r[2] * lcR[2] + r[3] * lcR[3];
   }
   // conservative cotangents of the six cells, summed over the waves before
   // the transpose of prim -> cons (linear in them) is applied once per cell
@@ -858,6 +933,8 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
     for (int v = 0; v < 4; v++) lc[j][v] = 0.0;
   CFD_UNROLL
   for (int k = 0; k < 4; k++) {
+    double Lk[4];
+    es.lrow(k, Lk);
     double gl[5] = {0, 0, 0, 0, 0}, gr[5] = {0, 0, 0, 0, 0};
     if constexpr (IP::kLinear) {  // the recorded flags' stencils
       IP a, b;
@@ -869,7 +946,7 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
       double ch[6];
       CFD_UNROLL
       for (int j = 0; j < 6; j++)
-        ch[j] = f.L[k][0] * w.cons(j, 0) + f.L[k][1] * w.cons(j, 1) + f.L[k][2] * w.cons(j, 2) + f.L[k][3] * w.cons(j, 3);
+        ch[j] = Lk[0] * w.cons(j, 0) + Lk[1] * w.cons(j, 1) + Lk[2] * w.cons(j, 2) + Lk[3] * w.cons(j, 3);
       double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
       double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
       {
@@ -892,7 +969,7 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
     for (int j = 0; j < 6; j++)
       for (int v = 0; v < 4; v++) {
         gLk[v] += lch[j] * w.cons(j, v);
-        lc[j][v] += lch[j] * f.L[k][v];
+        lc[j][v] += lch[j] * Lk[v];
       }
     es.acc_Lrow(k, gLk, acc);
   }

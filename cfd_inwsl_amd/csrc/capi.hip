@@ -378,7 +378,7 @@ static bool has_source(const cfd_handle* h) {
   return h->cfg.gravity[0] != 0.0 || h->cfg.gravity[1] != 0.0 || h->cfg.gravity[2] != 0.0;
 }
 hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int stage, const double* U0,
-                               void* Uout, int tile, int seg, void* Ushadow) {
+                               void* Uout, int tile, int seg, void* Ushadow, void* rec) {
   SweepArgs a;
   memset(&a, 0, sizeof(a));
   a.U = U;
@@ -420,20 +420,27 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
     h->wpart_blocks = nb * nseg;
     a.wave = h->wpart;
   }
+  if (rec) {
+    if (h->D != 2 || h->fp32 || h->fma || !h->cw || h->flx != FLX_HLLC) return hipErrorInvalidValue;
+    a.rec = rec;
+    a.rec_plane = (long)(h->cfg.n[ax] + 3) * (ax == 0 ? h->cfg.n[1] : h->cfg.n[0]);
+    return launch_sweep_rec(ax, h->sch, tile, &a, dim3((unsigned)nb, (unsigned)nseg), h->stream);
+  }
   if (h->fma) return launch_sweep_fma(h->D, ax, h->sch, h->flx, h->cw, tile, &a, dim3((unsigned)nb, (unsigned)nseg), h->stream);
   return launch_sweep_dispatch(h->D, ax, h->sch, h->flx, h->cw, tile, h->fp32, &a, nullptr,
                                dim3((unsigned)nb, (unsigned)nseg), h->stream);
 }
 
 // One RK stage: sweeps over every direction; the last one applies the stage update.
-int rk_stage(cfd_handle* h, int stage, const void* Ucur, const double* U0, void* Uout, void* Ushadow) {
+int rk_stage(cfd_handle* h, int stage, const void* Ucur, const double* U0, void* Uout, void* Ushadow,
+             void* const* rec) {
   for (int ax = 0; ax < h->D; ax++) {
     int flags = 0;
     if (ax > 0) flags |= SW_READ_R;
     if (ax < h->D - 1 || h->keep_rhs) flags |= SW_WRITE_R;
     if (ax == h->D - 1) flags |= SW_RK;
     if (ax == h->D - 1 && stage == 3 && h->fuse_wave) flags |= SW_WAVE;
-    HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, U0, Uout, -1, -1, Ushadow));
+    HIPCHK(launch_sweep(h, ax, Ucur, flags, stage, U0, Uout, -1, -1, Ushadow, rec ? rec[ax] : nullptr));
   }
   return 0;
 }

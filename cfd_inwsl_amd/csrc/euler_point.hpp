@@ -772,8 +772,12 @@ template <class T> __device__ __forceinline__ void roe1d(const T* L, const T* Rr
 
 // HLLCFlux2D2 (fluxSchemes.hpp:202-327), pressure-based wave speeds, with the
 // averaged Sstar == 0 branch; 3D writes the star tangential momentum as
-// Sstar*n_k + u_k*(sum of the other normal components).
-template <int D, int AX, class T> __device__ __forceinline__ void hllc_md(const T* Lp, const T* Rp, T* res) {
+// Sstar*n_k + u_k*(sum of the other normal components).  A non-null hb
+// receives the branches taken: 1 pressure clipped to 0, 2 left shock factor,
+// 4 right shock factor, region << 3 (0 left state, 1 left star, 2 the
+// averaged Sstar == 0 star, 3 right star, 4 right state).
+template <int D, int AX, class T>
+__device__ __forceinline__ void hllc_md(const T* Lp, const T* Rp, T* res, int* hb = nullptr) {
   constexpr int NV = D + 2, E = NV - 1;
   const T n[3] = {AX == 0 ? T(1.0) : T(0.0), AX == 1 ? T(1.0) : T(0.0), AX == 2 ? T(1.0) : T(0.0)};
   T rl = Lp[0], rr = Rp[0], pl = Lp[E], pr = Rp[E];
@@ -789,11 +793,16 @@ template <int D, int AX, class T> __device__ __forceinline__ void hllc_md(const 
   T cl = fsq(fdv(pl, rl) * T(CFD_GAMMA));
   T cr = fsq(fdv(pr, rr) * T(CFD_GAMMA));
   T rBar = (rl + rr) / 2, cBar = (cl + cr) / 2;
-  T ps = smax(T(0.0), (pl + pr) / 2 - rBar * cBar * (VnR - VnL) / 2);
+  const T psr = (pl + pr) / 2 - rBar * cBar * (VnR - VnL) / 2;
+  T ps = smax(T(0.0), psr);
   T ql = (ps <= pl) ? T(1.0) : fsq(T(1.0) + (T(CFD_GAMMA) + T(1.0)) / (T(2.0) * T(CFD_GAMMA)) * (fdv(ps, pl) - T(1.0)));
   T qr = (ps <= pr) ? T(1.0) : fsq(T(1.0) + (T(CFD_GAMMA) + T(1.0)) / (T(2.0) * T(CFD_GAMMA)) * (fdv(ps, pr) - T(1.0)));
   T SL = VnL - cl * ql, SR = VnR + cr * qr;
   T Sstar = fdv((pr - pl) + (rl * VnL * (SL - VnL) - rr * VnR * (SR - VnR)), rl * (SL - VnL) - rr * (SR - VnR));
+  if (hb) {
+    const int reg = SL > 0 ? 0 : Sstar > 0 ? 1 : Sstar == 0 ? 2 : SR > 0 ? 3 : 4;
+    *hb = (T(0.0) < psr ? 0 : 1) | (ps <= pl ? 0 : 2) | (ps <= pr ? 0 : 4) | (reg << 3);
+  }
 
   auto side = [&](const T* P, T r, T p, T H, T Vn, T* F) {
     F[0] = r * Vn;

@@ -48,8 +48,11 @@ struct cfd_handle {
 };
 
 int cfd_fail(int code, const std::string& msg);
+// rec (2D FP64 char-wise HLLC only): the direction's interface record, written
+// by the recording sweep (SweepArgs::rec); rec[ax] for rk_stage
 hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int stage, const double* U0, void* Uout,
-                        int tile, int seg, void* Ushadow);
-int rk_stage(cfd_handle* h, int stage, const void* Ucur, const double* U0, void* Uout, void* Ushadow);
+                        int tile, int seg, void* Ushadow, void* rec = nullptr);
+int rk_stage(cfd_handle* h, int stage, const void* Ucur, const double* U0, void* Uout, void* Ushadow,
+             void* const* rec = nullptr);
 const void* sweep_state(cfd_handle* h);
 void cfd_adjoint_release(cfd_handle* h);

@@ -81,6 +81,16 @@ struct SweepArgs {
   const double* halt;
   // SW_WAVE: the per-workgroup maxima of c + |u_d| (D x blocks)
   double* wave;
+  // sweep_kernel_rec (2D FP64 characteristic-wise HLLC, the adjoint's
+  // recomputed forward): per interface of the pencil (n + 3 of them, k + 1 =
+  // the interface between cells k - 1 and k) the characteristic half-node
+  // values QL[4], QR[4] (8 planes of rec_plane doubles) and one word of branch
+  // bits (after them): TENO flags (bits 6k..6k+2 left, 6k+3..6k+5 right value
+  // of wave k), positivity fallback left (24) / right (25), HLLC branch bits
+  // (26..31: clip, left shock, right shock, region << 3).  Plane index: x
+  // sweep pencil * (n + 3) + k + 1, y sweep (k + 1) * nx + pencil.
+  void* rec;
+  long rec_plane;
 };
 
 // Tile shapes: CA interfaces per chunk along the sweep axis x CP pencils.
@@ -136,8 +146,16 @@ template <int R> __device__ __forceinline__ int ring_wrap(int x) {
 // c mod R of a non-negative-or-small signed (wave-uniform) position
 template <int R> __device__ __forceinline__ int ring_mod(int c) { return ((c % R) + R) % R; }
 
-template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T, bool PF>
+// Schemes whose half-node value is one of the eight stencil choices of a
+// cut-off flag (teno_pick of cut_flag: what inter5 returns for them).
+template <int SCH> struct TenoFamily {
+  static constexpr bool value = SCH == SCH_TENO || SCH == SCH_TENOS || SCH == SCH_TENOA || SCH == SCH_TENOSA ||
+                                SCH == SCH_WHF;
+};
+
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T, bool PF, bool REC = false>
 __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, const int by) {
+  static_assert(!REC || (D == 2 && CHARW && FLX == FLX_HLLC), "recording sweeps: 2D char-wise HLLC");
   constexpr int NV = D + 2;
   const T* __restrict__ Ut = static_cast<const T*>(a.U);
   T* Rt = static_cast<T*>(a.R);
@@ -360,13 +378,32 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
             for (int j = 0; j < 6; j++)
               for (int v = 0; v < NV; v++) cw[j][v] = cell[cidx(NV + v, sl[j], lp)];
             T QL[NV], QR[NV];
+            unsigned rbits = 0u;  // REC: branch bits of this interface
             for (int k = 0; k < NV; k++) {
               T l[NV];
               es.lrow(k, l);
               T ch[6];
               for (int j = 0; j < 6; j++) ch[j] = project<D>(cw[j], l);
-              QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, &a.aux->rc);
-              QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct, &a.aux->rc);
+              if (REC && TenoFamily<SCH>::value) {
+                // inter5's own evaluation for these schemes, flags kept
+                const unsigned fl = cut_flag<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, &a.aux->rc);
+                const unsigned fr = cut_flag<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct, &a.aux->rc);
+                QL[k] = teno_pick(fl, ch[0], ch[1], ch[2], ch[3], ch[4]);
+                QR[k] = teno_pick(fr, ch[5], ch[4], ch[3], ch[2], ch[1]);
+                rbits |= (fl | (fr << 3)) << (6 * k);
+              } else {
+                QL[k] = inter5<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, &a.aux->rc);
+                QR[k] = inter5<SCH>(ch[5], ch[4], ch[3], ch[2], ch[1], ct, &a.aux->rc);
+              }
+            }
+            long ri = 0;
+            if (REC) {  // the half-node values, stored as soon as they exist
+              ri = AX == 0 ? gp * (n + 3) + m : (long)m * npencil + gp;
+              double* rd = static_cast<double*>(a.rec);
+              for (int k = 0; k < NV; k++) {
+                rd[k * a.rec_plane + ri] = QL[k];
+                rd[(NV + k) * a.rec_plane + ri] = QR[k];
+              }
             }
             // charToCons in the summation order (ch3, ch4, ch2, ch1[, ch5])
             T cL[NV], cR[NV];
@@ -388,25 +425,39 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
             eig_cons_to_prim<D>(cL, outL);
             eig_cons_to_prim<D>(cR, outR);
             // positivity fallback (reconstructor5order.hpp:301-302)
-            if (outL[0] < 0 || outL[NV - 1] < 0)
+            const bool fbL = outL[0] < 0 || outL[NV - 1] < 0;
+            const bool fbR = outR[0] < 0 || outR[NV - 1] < 0;
+            if (fbL)
               for (int v = 0; v < NV; v++) outL[v] = pL[v];
-            if (outR[0] < 0 || outR[NV - 1] < 0) {
+            if (fbR) {
               outR[0] = pR[0];
               for (int v = 1; v < NV - 1; v++) outR[v] = pL[v];
               outR[NV - 1] = pR[NV - 1];
             }
+            if (REC) {
+              rbits |= (fbL ? 1u << 24 : 0u) | (fbR ? 1u << 25 : 0u);
+              int hb = 0;
+              T F[NV];
+              hllc_md<D, AX>(outL, outR, F, &hb);
+              reinterpret_cast<unsigned*>(static_cast<double*>(a.rec) + 2 * NV * a.rec_plane)[ri] =
+                  rbits | ((unsigned)hb << 26);
+              const int fs = ring_wrap<FR>(bF3 + 3 + la);
+              for (int v = 0; v < NV; v++) flux[fidx(v, fs, lp)] = F[v];
+            }
           }
         }
-        T F[NV];
-        if (D == 1) {
-          if (FLX == FLX_HLLC) hllc1d(outL, outR, F);
-          else roe1d(outL, outR, F);
-        } else {
-          if (FLX == FLX_HLLC) hllc_md<D, AX>(outL, outR, F);
-          else roe_md<D, AX>(outL, outR, F);
+        if (!REC) {
+          T F[NV];
+          if (D == 1) {
+            if (FLX == FLX_HLLC) hllc1d(outL, outR, F);
+            else roe1d(outL, outR, F);
+          } else {
+            if (FLX == FLX_HLLC) hllc_md<D, AX>(outL, outR, F);
+            else roe_md<D, AX>(outL, outR, F);
+          }
+          const int fs = ring_wrap<FR>(bF3 + 3 + la);
+          for (int v = 0; v < NV; v++) flux[fidx(v, fs, lp)] = F[v];
         }
-        const int fs = ring_wrap<FR>(bF3 + 3 + la);
-        for (int v = 0; v < NV; v++) flux[fidx(v, fs, lp)] = F[v];
       }
     }
     __syncthreads();
@@ -539,6 +590,16 @@ __global__ __launch_bounds__(256, (SweepOcc<D, AX, SCH, FLX, double>::waves)) vo
   if (a.halt && *a.halt != 0.0) return;
   sweep_block<D, AX, SCH, FLX, CHARW, TILE, double, SweepOcc<D, AX, SCH, FLX, double>::prefetch>(a, blockIdx.x,
                                                                                               blockIdx.y);
+}
+
+// The adjoint's recomputed forward: the 2D FP64 characteristic-wise HLLC
+// sweep that also writes each interface's half-node values and branch bits
+// (SweepArgs::rec); its results are the plain kernel's, bit for bit.
+template <int AX, int SCH, int TILE>
+__global__ __launch_bounds__(256, (SweepOcc<2, AX, SCH, FLX_HLLC, double>::waves)) void sweep_kernel_rec(SweepArgs a) {
+  if (a.halt && *a.halt != 0.0) return;
+  sweep_block<2, AX, SCH, FLX_HLLC, true, TILE, double, SweepOcc<2, AX, SCH, FLX_HLLC, double>::prefetch, true>(
+      a, blockIdx.x, blockIdx.y);
 }
 
 // ---- batched zones ------------------------------------------------------------

@@ -105,6 +105,8 @@ inline bool fma_scheme_supported(int sch) { return sch == SCH_JS || sch == SCH_Z
 // fp32: FP32 storage / arithmetic kernels (not compiled for 1D).
 hipError_t launch_sweep_dispatch(int D, int ax, int sch, int flx, bool cw, int tile, bool fp32, const SweepArgs* a,
                                  const BatchArgs* b, dim3 grid, hipStream_t s);
+// the adjoint's recording 2D sweeps (sweep_rec.hip; SweepArgs::rec)
+hipError_t launch_sweep_rec(int ax, int sch, int tile, const SweepArgs* a, dim3 grid, hipStream_t s);
 // the FMA-contracted FP64 kernels (sweep_fma_d<D>_<axis>.hip)
 hipError_t launch_sweep_fma(int D, int ax, int sch, int flx, bool cw, int tile, const SweepArgs* a, dim3 grid,
                             hipStream_t s);

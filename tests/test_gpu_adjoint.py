@@ -7,7 +7,9 @@
   derivative.
 * The checkpointed RK3 adjoint against the dual-number oracle RK3 on 32
   seeded directions of the initial density, independence of the checkpoint
-  interval (bit for bit), and the torch.autograd harness.
+  interval (bit for bit between intervals, whose VJPs read the recording
+  sweeps' interface records; to rounding against every stage kept, whose
+  VJPs re-evaluate the interfaces), and the torch.autograd harness.
 * Central finite differences of the GPU forward on the full workload
   (512^2, 400 steps, checkpoint every 20).
 """
@@ -118,8 +120,8 @@ def _gpu_gradient(case, U0, dt, nsteps, every, inter=InterMethod.WCNS5):
 def test_rk3_adjoint_matches_dual_oracle_32_directions(inter):
     """WENO-JS and the TENO family -- TENO5, TENO-S, TENO-A, an adaptive
     TENO-S (whf_ai AS) and a chi-type whf scheme (LAD) -- whose stencil
-    flags are branches of the forward, recorded by the VJP's exact forward
-    re-evaluation."""
+    flags are branches of the forward, recorded by the recording sweeps of
+    the recomputed forward."""
     n, nsteps = 64, 60
     case, U0 = _shock(n)
     dt = cases.cfl_dt_2d(U0, n, case["h"])
@@ -138,13 +140,24 @@ def test_rk3_adjoint_matches_dual_oracle_32_directions(inter):
     assert worst < 1e-9
 
 
+def _close(a, b, tol=1e-12):
+    return np.abs(a - b).max() <= tol * np.abs(b).max()
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
+    """With checkpoints the VJPs read the interface records of the
+    recording sweeps (same bits for every interval); with every stage state
+    kept (0) they re-evaluate the interfaces exactly and take the same
+    branches, so the gradient agrees to rounding."""
     n, nsteps = 48, 25
     case, U0 = _shock(n)
     dt = cases.cfl_dt_2d(U0, n, case["h"])
     ref = _gpu_gradient(case, U0, dt, nsteps, 1)[2]
-    for every in (7, 25, 40, 0):  # 0: every stage state kept, no recomputation
+    for every in (7, 25, 40):
         assert np.array_equal(_gpu_gradient(case, U0, dt, nsteps, every)[2], ref), every
+    g0 = _gpu_gradient(case, U0, dt, nsteps, 0)[2]  # every stage state kept, no recomputation
+    _say(f"every stage kept vs checkpoints: max |d| / max |g| {np.abs(g0 - ref).max() / np.abs(ref).max():.2e}")
+    assert _close(g0, ref)
 
 
 @pytest.mark.parametrize("every", [1, 7, 25, 0])
@@ -163,7 +176,8 @@ def test_backward_repeats_after_one_forward(every):
     g2 = s.adjoint_backward(gJ)
     s.close()
     assert np.array_equal(g1, g2)
-    assert np.array_equal(g1, _gpu_gradient(case, U0, dt, nsteps, 1)[2])
+    ref = _gpu_gradient(case, U0, dt, nsteps, 1)[2]
+    assert np.array_equal(g1, ref) if every else _close(g1, ref)
 
 
 def test_torch_harness_equals_direct_call():
