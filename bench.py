@@ -85,6 +85,7 @@ def bench_batch(nzones=96, seed=20261203, steps=3):
         b.set_state(i, c)
     b.autotune(reps=1)
     out = {}
+    finals = {}
     for name, mode in (("graph", BATCH_GRAPH), ("persistent", BATCH_PERSISTENT), ("per_zone", BATCH_PER_ZONE)):
         for i, c in enumerate(states):
             b.set_state(i, c)
@@ -97,11 +98,40 @@ def bench_batch(nzones=96, seed=20261203, steps=3):
         out[name] = {"ms_per_step": round(sec * 1e3 / steps, 3),
                      "mcell_stage_s": round(cells * 3 * steps / sec / 1e6, 1)}
         log(f"[bench] batch {name}: {out[name]}")
+        if mode == BATCH_GRAPH:
+            finals[PRECISION_FP64] = [b.get_state(i) for i in range(nzones)]
     b.close()
+    # FP32 storage (FP32 sweeps, FP64 state and stage combination): the same
+    # 1 + steps graph-replayed steps, timed, and the per-variable difference to
+    # the FP64 batch over every zone (relative to max |U_v| over the batch)
+    b = Batch(2, zones, InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, precision=PRECISION_FP32)
+    for i, c in enumerate(states):
+        b.set_state(i, c)
+    b.autotune(reps=1)
+    for i, c in enumerate(states):
+        b.set_state(i, c)
+    b.steps_cfl(0.5, 1, BATCH_GRAPH)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.steps_cfl(0.5, steps, BATCH_GRAPH)
+    b.synchronize()
+    sec = time.perf_counter() - t0
+    out["graph_fp32"] = {"ms_per_step": round(sec * 1e3 / steps, 3),
+                         "mcell_stage_s": round(cells * 3 * steps / sec / 1e6, 1)}
+    log(f"[bench] batch graph_fp32: {out['graph_fp32']}")
+    f32 = [b.get_state(i) for i in range(nzones)]
+    b.close()
+    f64 = finals[PRECISION_FP64]
+    scale = np.max([np.abs(x).max(axis=0) for x in f64], axis=0)
+    l1 = sum(np.abs(a - x).sum(axis=0) for a, x in zip(f32, f64)) / cells / scale
+    linf = np.max([np.abs(a - x).max(axis=0) for a, x in zip(f32, f64)], axis=0) / scale
+    fp32_err = {"steps": steps + 1, "L1_rel": [float(x) for x in l1], "Linf_rel": [float(x) for x in linf]}
+    log(f"[bench] batch fp32 vs fp64: {fp32_err}")
     return {"workload": f"{nzones} independent 2D four-quadrant Riemann zones, sides U[128,1024], "
-                        "WCNS5 char-wise + HLLC, FP64, per-zone CFL 0.5",
+                        "WCNS5 char-wise + HLLC, FP64 (and FP32 storage: graph_fp32), per-zone CFL 0.5",
             "zones": nzones, "cells": cells, "steps": steps, "modes": out,
-            "speedup_graph_vs_per_zone": round(out["per_zone"]["ms_per_step"] / out["graph"]["ms_per_step"], 2)}
+            "speedup_graph_vs_per_zone": round(out["per_zone"]["ms_per_step"] / out["graph"]["ms_per_step"], 2),
+            "fp32_vs_fp64": fp32_err}
 
 
 def bench_precisions(steps_err=100):
@@ -144,6 +174,44 @@ def bench_precisions(steps_err=100):
                                  "Linf_rel": [float(x) for x in np.abs(a - b).max(axis=0) / scale]}
         out[name] = {"mcell_stage_s": res, "steps": steps_err, "vs_fp64": diff}
         log(f"[bench] precisions {name}: {out[name]}")
+    # FP32 storage on the two other single-zone configurations: the Sod tube
+    # (400 cells, WCNS5-JS component-wise + HLLC, CFL 0.5 to t = 0.2 through
+    # the device output loop) and the adjoint workload's forward map (512^2,
+    # WCNS5-JS char-wise + HLLC, Mach-1.5 shock, 400 fixed-dt steps)
+    def rel(a, b):
+        sc = np.abs(b).max(axis=0)
+        return {"L1_rel": [float(x) for x in np.abs(a - b).mean(axis=0) / sc],
+                "Linf_rel": [float(x) for x in np.abs(a - b).max(axis=0) / sc]}
+
+    sod = {}
+    for prec in (PRECISION_FP64, PRECISION_FP32):
+        c, h = cases.sod(400)
+        s = Solver(1, (400,), h, InterMethod.WCNS5, FluxMethod.HLLC, Recon.COMPONENT, [BC.OUTFLOW] * 6,
+                   precision=prec)
+        s.set_state(c)
+        ot, reached, nst = 0.0, False, 0
+        while not reached:
+            lg, ot, reached = s.steps_to_output(0.5, 0.2, ot, max_steps=256)
+            nst += len(lg)
+        sod[prec] = (s.get_state(), s.get_time(), nst)
+        s.close()
+    out["1d_sod_400"] = {"t": sod[PRECISION_FP32][1], "steps": sod[PRECISION_FP32][2],
+                         "vs_fp64": {"fp32": rel(sod[PRECISION_FP32][0], sod[PRECISION_FP64][0])}}
+    log(f"[bench] precisions 1d_sod_400: {out['1d_sod_400']}")
+    case = cases.adjoint_shock(512)
+    U0 = cases.adjoint_state(case)
+    dt = cases.cfl_dt_2d(U0, 512, case["h"])
+    adj = {}
+    for prec in (PRECISION_FP64, PRECISION_FP32):
+        s = Solver(2, (512, 512), (case["h"], case["h"]), InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR,
+                   [BC.PERIODIC] * 6, precision=prec)
+        s.set_state(U0)
+        for _ in range(400):
+            s.rk3_step(dt)
+        adj[prec] = s.get_state()
+        s.close()
+    out["2d_adjoint_shock_512"] = {"steps": 400, "vs_fp64": {"fp32": rel(adj[PRECISION_FP32], adj[PRECISION_FP64])}}
+    log(f"[bench] precisions 2d_adjoint_shock_512: {out['2d_adjoint_shock_512']}")
     return out
 
 

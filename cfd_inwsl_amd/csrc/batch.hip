@@ -46,7 +46,14 @@ struct cfd_batch {
   double ct;
   std::vector<ZoneDesc> hz;       // host copy of the zone table
   long total;                     // pooled elements per buffer (sum nv*N)
-  double *U0, *Ua, *Ub, *R;
+  // CFD_PRECISION_FP32: the sweeps read and write FP32 pools (Ua, Ub, R and
+  // the shadow U0f of U0); U0 and the stage combination stay FP64.  Ub is
+  // FP64-sized either way: it doubles as the host-transfer staging buffer.
+  bool fp32;
+  size_t esz;                     // bytes per element of Ua, R (sizeof(double) or sizeof(float))
+  double *U0, *Ub;
+  void *Ua, *R;
+  float* U0f;
   ZoneDesc* dz;
   double* zscal;                  // per zone: [0] dt, [1..3] lambda bits, [4] t (stride 8)
   double* dt;                     // per-zone dt (contiguous, for the sweep kernels)
@@ -129,6 +136,10 @@ __global__ void kb_aos_to_soa(const double* __restrict__ in, double* __restrict_
   if (i < N)
     for (int v = 0; v < NV; v++) out[v * N + i] = in[i * NV + v];
 }
+__global__ void kb_to_f32(const double* __restrict__ in, float* __restrict__ out, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
 __global__ void kb_soa_to_aos(const double* __restrict__ in, double* __restrict__ out, long N, int NV) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N)
@@ -205,8 +216,8 @@ static int batch_step_persistent(cfd_batch* b, double cfl) {
     break;
   }
   BCHK(hipGetLastError());
-  const double* ucur[3] = {b->U0, b->Ua, b->Ub};
-  double* uout[3] = {b->Ua, b->Ub, b->U0};
+  const void* ucur[3] = {b->fp32 ? (const void*)b->U0f : (const void*)b->U0, b->Ua, b->Ub};
+  void* uout[3] = {b->Ua, b->Ub, b->U0};
   for (int st = 0; st < 3; st++)
     for (int ax = 0; ax < b->D; ax++) {
       BatchArgs a;
@@ -215,6 +226,7 @@ static int batch_step_persistent(cfd_batch* b, double cfl) {
       a.R = b->R;
       a.U0 = b->U0;
       a.Uout = uout[st];
+      a.Ushadow = (b->fp32 && st == 2) ? (void*)b->U0f : nullptr;
       a.dt = b->dt;
       a.zones = b->dz;
       a.items = b->items[ax];
@@ -224,7 +236,7 @@ static int batch_step_persistent(cfd_batch* b, double cfl) {
       a.stage = st + 1;
       a.ct = b->ct;
       int grid = std::min(b->persist_blocks, b->nitems[ax]);
-      BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], false, nullptr, &a, dim3(grid), b->stream));
+      BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], b->fp32, nullptr, &a, dim3(grid), b->stream));
     }
   return 0;
 }
@@ -259,18 +271,24 @@ static int batch_step_per_zone(cfd_batch* b, double cfl) {
   default: hipLaunchKernelGGL(kb_finish_dt<3>, dim3((b->nz + 63) / 64), dim3(64), 0, b->stream, b->zscal, b->dt, b->nz, cfl); break;
   }
   BCHK(hipGetLastError());
-  const double* ucur[3] = {b->U0, b->Ua, b->Ub};
-  double* uout[3] = {b->Ua, b->Ub, b->U0};
+  // element offsets into pools of the sweeps' type (T) and of FP64
+  auto at = [&](const void* pool, bool t_typed, long off) -> void* {
+    const size_t e = t_typed ? b->esz : sizeof(double);
+    return static_cast<char*>(const_cast<void*>(pool)) + e * off;
+  };
+  const void* ucur[3] = {b->fp32 ? (const void*)b->U0f : (const void*)b->U0, b->Ua, b->Ub};
+  void* uout[3] = {b->Ua, b->Ub, b->U0};
   for (int z = 0; z < b->nz; z++) {
     const ZoneDesc& zd = b->hz[z];
     for (int st = 0; st < 3; st++)
       for (int ax = 0; ax < b->D; ax++) {
         SweepArgs a;
         std::memset(&a, 0, sizeof(a));
-        a.U = ucur[st] + zd.off;
-        a.R = b->R + zd.off;
+        a.U = at(ucur[st], true, zd.off);
+        a.R = at(b->R, true, zd.off);
         a.U0 = b->U0 + zd.off;
-        a.Uout = uout[st] + zd.off;
+        a.Uout = at(uout[st], st != 2, zd.off);
+        a.Ushadow = (b->fp32 && st == 2) ? at(b->U0f, true, zd.off) : nullptr;
         a.dtp = b->dt + z;
         a.N = zd.N;
         for (int i = 0; i < 3; i++) a.n[i] = zd.n[i];
@@ -284,7 +302,7 @@ static int batch_step_per_zone(cfd_batch* b, double cfl) {
         a.aux = &b->dz[z].aux[ax];
         long nb = plan_pencil_blocks(zd.n, ax, b->tile[ax]);
         int nseg = (zd.n[ax] + zd.seg_len[ax] - 1) / zd.seg_len[ax];
-        BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], false, &a, nullptr,
+        BCHK(launch_sweep_dispatch(b->D, ax, b->sch, b->flx, b->cw, b->tile[ax], b->fp32, &a, nullptr,
                                    dim3((unsigned)nb, (unsigned)nseg), b->stream));
       }
   }
@@ -324,7 +342,8 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
     return cfd_set_error_from_batch(CFD_E_ARG, "recon must be CFD_RECON_CHAR or CFD_RECON_COMPONENT");
   int sch;
   double ct;
-  if (c->precision != CFD_PRECISION_FP64) return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches run in FP64");
+  if (c->precision != CFD_PRECISION_FP64 && c->precision != CFD_PRECISION_FP32)
+    return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches run in CFD_PRECISION_FP64 or CFD_PRECISION_FP32");
   if (c->gravity[0] != 0.0 || c->gravity[1] != 0.0 || c->gravity[2] != 0.0)
     return cfd_set_error_from_batch(CFD_E_UNSUPP, "batches take no source term");
   if (cfd_select_scheme(c->inter_method, c->dim, &sch, &ct))
@@ -340,6 +359,8 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
   b->ct = ct;
   b->flx = c->flux_method;
   b->cw = c->recon == CFD_RECON_CHAR;
+  b->fp32 = c->precision == CFD_PRECISION_FP32;
+  b->esz = b->fp32 ? sizeof(float) : sizeof(double);
   b->tile[0] = 0;
   b->tile[1] = 1;
   b->tile[2] = 1;
@@ -384,9 +405,10 @@ int cfd_batch_create(const cfd_config* c, int nzones, const cfd_zone* zones, cfd
   b->total = off;
   hipError_t e = hipSetDevice(c->device);
   if (e == hipSuccess) e = hipMalloc(&b->U0, sizeof(double) * off);
-  if (e == hipSuccess) e = hipMalloc(&b->Ua, sizeof(double) * off);
+  if (e == hipSuccess) e = hipMalloc(&b->Ua, b->esz * off);
   if (e == hipSuccess) e = hipMalloc(&b->Ub, sizeof(double) * off);
-  if (e == hipSuccess) e = hipMalloc(&b->R, sizeof(double) * off);
+  if (e == hipSuccess) e = hipMalloc(&b->R, b->esz * off);
+  if (e == hipSuccess && b->fp32) e = hipMalloc(&b->U0f, sizeof(float) * off);
   if (e == hipSuccess) e = hipMalloc(&b->dz, sizeof(ZoneDesc) * nzones);
   if (e == hipSuccess) e = hipMalloc(&b->zscal, sizeof(double) * 8 * nzones);
   if (e == hipSuccess) e = hipMalloc(&b->dt, sizeof(double) * nzones);
@@ -428,6 +450,7 @@ int cfd_batch_destroy(cfd_batch* b) {
   (void)hipFree(b->Ua);
   (void)hipFree(b->Ub);
   (void)hipFree(b->R);
+  if (b->U0f) (void)hipFree(b->U0f);
   (void)hipFree(b->dz);
   (void)hipFree(b->zscal);
   (void)hipFree(b->dt);
@@ -447,6 +470,9 @@ int cfd_batch_set_state(cfd_batch* b, int zone, const double* cons) {
   BCHK(hipMemcpyAsync(b->Ub + zd.off, cons, sizeof(double) * zd.N * b->NV, hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(kb_aos_to_soa, dim3((unsigned)((zd.N + 255) / 256)), dim3(256), 0, b->stream, b->Ub + zd.off,
                      b->U0 + zd.off, zd.N, b->NV);
+  if (b->fp32)
+    hipLaunchKernelGGL(kb_to_f32, dim3((unsigned)((zd.N * b->NV + 255) / 256)), dim3(256), 0, b->stream,
+                       b->U0 + zd.off, b->U0f + zd.off, zd.N * b->NV);
   BCHK(hipGetLastError());
   BCHK(hipStreamSynchronize(b->stream));
   return CFD_OK;
@@ -525,6 +551,12 @@ int cfd_batch_autotune(cfd_batch* b, int reps) {
         }
         BCHK(hipMemcpy(b->U0, save.data(), sizeof(double) * b->total, hipMemcpyHostToDevice));
       }
+  if (b->fp32) {  // the shadow of the restored state
+    hipLaunchKernelGGL(kb_to_f32, dim3((unsigned)((b->total + 255) / 256)), dim3(256), 0, b->stream, b->U0, b->U0f,
+                       b->total);
+    BCHK(hipGetLastError());
+    BCHK(hipStreamSynchronize(b->stream));
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   for (int d = 0; d < 3; d++) b->tile[d] = bt[d];

@@ -550,10 +550,6 @@ int cfd_create(const cfd_config* cfg, cfd_handle** out) {
     delete h;
     return cfd_fail(CFD_E_UNSUPP, "CFD_PRECISION_FP64_FMA is built for WENO-JS, WENO-Z, TENO5 and TENO-S");
   }
-  if (h->fp32 && cfg->dim == 1) {
-    delete h;
-    return cfd_fail(CFD_E_UNSUPP, "FP32 precision is available for dim 2 and 3");
-  }
   if (h->fp32 && sch == SCH_WHF) {
     delete h;
     return cfd_fail(CFD_E_UNSUPP, "inter_method " + std::to_string(cfg->inter_method) + " runs in FP64 only");

@@ -622,11 +622,15 @@ struct WorkItem {
   int by;  // segment
 };
 
+// Pools are typed like SweepArgs' buffers: U, R and the stage outputs hold
+// the kernel's compute type T; U0 is FP64; with T = float the stage-3 output
+// is the FP64 pool U0 plus its FP32 shadow Ushadow.
 struct BatchArgs {
-  const double* U;
-  double* R;
+  const void* U;
+  void* R;
   const double* U0;
-  double* Uout;
+  void* Uout;
+  void* Ushadow;
   const double* dt;        // per-zone time step
   const ZoneDesc* zones;
   const WorkItem* items;   // work list of this direction, largest zones first
@@ -639,7 +643,7 @@ struct BatchArgs {
 
 // Persistent kernel: each workgroup pulls work items from the queue until it
 // is empty, so one launch covers every zone of the batch.
-template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE>
+template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
 __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kernel(BatchArgs b) {
   __shared__ int s_item;
   for (;;) {
@@ -652,11 +656,16 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     const int z = __builtin_amdgcn_readfirstlane(w.zone);
     const ZoneDesc& zd = b.zones[z];
     SweepArgs a;
-    a.U = b.U + zd.off;
-    a.R = b.R + zd.off;
+    a.U = static_cast<const T*>(b.U) + zd.off;
+    a.R = static_cast<T*>(b.R) + zd.off;
     a.U0 = b.U0 + zd.off;
-    a.Uout = b.Uout + zd.off;
-    a.Ushadow = nullptr;
+    if (sizeof(T) == sizeof(double) || b.stage != 3) {
+      a.Uout = static_cast<T*>(b.Uout) + zd.off;
+      a.Ushadow = nullptr;
+    } else {
+      a.Uout = static_cast<double*>(b.Uout) + zd.off;
+      a.Ushadow = static_cast<T*>(b.Ushadow) + zd.off;
+    }
     a.halt = nullptr;
     a.wave = nullptr;
     a.dtp = b.dt + z;
@@ -672,7 +681,7 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     a.bc_lo = zd.bc[2 * AX];
     a.bc_hi = zd.bc[2 * AX + 1];
     a.aux = &zd.aux[AX];  // batches take neither position-dependent boundaries nor sources
-    sweep_block<D, AX, SCH, FLX, CHARW, TILE, double, false>(a, __builtin_amdgcn_readfirstlane(w.bx),
+    sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, false>(a, __builtin_amdgcn_readfirstlane(w.bx),
                                               __builtin_amdgcn_readfirstlane(w.by));
   }
 }

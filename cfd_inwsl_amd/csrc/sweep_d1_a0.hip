@@ -1,5 +1,5 @@
 // Sweep kernels for 1D Euler, direction 0.
-#define CFD_WITH_FP32 0
+#define CFD_WITH_FP32 1
 #include "sweep_inst.hpp"
 
 namespace cfd {

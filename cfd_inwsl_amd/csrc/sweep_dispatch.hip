@@ -28,7 +28,6 @@ hipError_t launch_sweep_fma(int D, int ax, int sch, int flx, bool cw, int tile, 
 
 hipError_t launch_sweep_dispatch(int D, int ax, int sch, int flx, bool cw, int tile, bool fp32, const SweepArgs* a,
                                  const BatchArgs* b, dim3 grid, hipStream_t s) {
-  if (fp32 && (D == 1 || b)) return hipErrorInvalidValue;
   if (D == 1) return launch_d1_a0(sch, flx, cw, tile, fp32, a, b, grid, s);
   if (D == 2) return ax == 0 ? launch_d2_a0(sch, flx, cw, tile, fp32, a, b, grid, s) : launch_d2_a1(sch, flx, cw, tile, fp32, a, b, grid, s);
   if (ax == 0) return launch_d3_a0(sch, flx, cw, tile, fp32, a, b, grid, s);

@@ -19,7 +19,15 @@ static hipError_t launch_one(bool fp32, const SweepArgs* a, const BatchArgs* b, 
     if (b || fp32) return hipErrorInvalidValue;
     hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW, TL, double>), grid, dim3(256), 0, s, *a);
   } else if (b) {
-    hipLaunchKernelGGL((sweep_batch_kernel<D, AX, SCH, FLX, CW, TL>), grid, dim3(256), 0, s, *b);
+    if (fp32) {
+#if CFD_WITH_FP32
+      hipLaunchKernelGGL((sweep_batch_kernel<D, AX, SCH, FLX, CW, TL, float>), grid, dim3(256), 0, s, *b);
+#else
+      return hipErrorInvalidValue;
+#endif
+    } else {
+      hipLaunchKernelGGL((sweep_batch_kernel<D, AX, SCH, FLX, CW, TL, double>), grid, dim3(256), 0, s, *b);
+    }
   } else if (fp32) {
 #if CFD_WITH_FP32
     hipLaunchKernelGGL((sweep_kernel<D, AX, SCH, FLX, CW, TL, float>), grid, dim3(256), 0, s, *a);

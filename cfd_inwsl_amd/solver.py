@@ -376,9 +376,10 @@ class Batch:
     """Independent zones of uneven sizes advanced together (cfd_batch_*)."""
 
     def __init__(self, dim, zones, inter_method=InterMethod.WCNS5, flux_method=FluxMethod.HLLC,
-                 recon=Recon.CHAR, device=0):
+                 recon=Recon.CHAR, device=0, precision=PRECISION_FP64):
         """zones: list of dicts with keys n (tuple), h (float or tuple), bc (6 codes),
-        optional bc_prim."""
+        optional bc_prim.  precision: PRECISION_FP64 or PRECISION_FP32 (FP32
+        sweeps, FP64 state and stage combination)."""
         self.lib = load_library()
         self.dim = dim
         self.nv = dim + 2
@@ -389,6 +390,7 @@ class Batch:
         cfg.diff_method = DIFF_MND6
         cfg.recon = int(recon)
         cfg.device = device
+        cfg.precision = int(precision)
         arr = (cfd_zone * len(zones))()
         self.ncell = []
         for i, z in enumerate(zones):

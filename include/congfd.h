@@ -81,8 +81,8 @@ typedef struct cfd_config {
   int precision;        /* CFD_PRECISION_FP64 (reference arithmetic, bitwise),
                            CFD_PRECISION_FP32 (FP32 storage and arithmetic of the
                            sweeps; U^n and the RK3 stage combination stay FP64;
-                           dim 2 and 3, single zones) or CFD_PRECISION_FP64_FMA
-                           (below) */
+                           dim 1-3, single zones and batches) or
+                           CFD_PRECISION_FP64_FMA (below; single zones) */
   double origin;        /* cell centres at origin + (i + 0.5) h[0] in every direction
                            (Initializer::initUniformBlock, initializer.cpp:672-677);
                            used by the double-Mach boundaries */
@@ -212,8 +212,9 @@ int cfd_set_launch_plan(cfd_handle* h, const int* tiles, const int* segs);
  * sizes) advanced together.  The reference solves one block per process
  * (BlockSolver); this has no reference counterpart beyond running
  * BlockSolver::stepsLoopCFL on each zone.  Every zone takes its own CFL time
- * step.  Scheme, flux, recon and dim come from `common`; n, h and boundaries
- * from each zone. */
+ * step.  Scheme, flux, recon, dim and precision (CFD_PRECISION_FP64 or
+ * CFD_PRECISION_FP32) come from `common`; n, h and boundaries from each
+ * zone. */
 typedef struct cfd_zone {
   int n[3];
   double h[3];

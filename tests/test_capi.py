@@ -64,7 +64,9 @@ def test_bad_config_is_rejected_before_any_gpu_call(field, value, msg):
     assert not h.value
 
 
-def test_fp32_rejected_for_1d_and_batches():
+def test_precision_codes_checked():
+    """Unknown precision codes are refused by cfd_create, and batches refuse
+    the FMA mode (they run FP64 or FP32), before any GPU call."""
     lib = solver.load_library()
     cfg = solver.cfd_config()
     cfg.dim = 1
@@ -73,12 +75,21 @@ def test_fp32_rejected_for_1d_and_batches():
     cfg.inter_method, cfg.flux_method, cfg.diff_method, cfg.recon = 2, 0, 3, 0
     for i in range(6):
         cfg.bc[i] = 7
-    cfg.precision = 1
     h = ctypes.c_void_p()
-    assert lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h)) < 0
-    assert "FP32" in lib.cfd_last_error().decode()
     cfg.precision = 7
     assert lib.cfd_create(ctypes.byref(cfg), ctypes.byref(h)) < 0
+    assert "precision must be" in lib.cfd_last_error().decode()
+    zones = (solver.cfd_zone * 1)()
+    zones[0].n[0], zones[0].n[1], zones[0].n[2] = 16, 1, 1
+    zones[0].h[0] = zones[0].h[1] = zones[0].h[2] = 0.1
+    for i in range(6):
+        zones[0].bc[i] = 7
+    b = ctypes.c_void_p()
+    for prec in (solver.PRECISION_FP64_FMA, 7):
+        cfg.precision = prec
+        assert lib.cfd_batch_create(ctypes.byref(cfg), 1, zones, ctypes.byref(b)) < 0
+        assert "batches run in" in lib.cfd_last_error().decode()
+        assert not b.value
 
 
 def test_fma_precision_limited_to_its_schemes():

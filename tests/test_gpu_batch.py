@@ -7,8 +7,8 @@ a change of tile shape or segment length must not change any result."""
 import numpy as np
 import pytest
 
-from cfd_inwsl_amd import (BATCH_GRAPH, BATCH_PER_ZONE, BATCH_PERSISTENT, BC, Batch, FluxMethod, InterMethod,
-                           Recon, Solver, cases)
+from cfd_inwsl_amd import (BATCH_GRAPH, BATCH_PER_ZONE, BATCH_PERSISTENT, BC, PRECISION_FP32, PRECISION_FP64, Batch,
+                           FluxMethod, InterMethod, Recon, Solver, cases)
 from oracle import oracle as O
 from tests.helpers import oracle_grid, random_state
 
@@ -141,8 +141,8 @@ def _config_b_zones(nzones, lo, hi, seed):
     return out
 
 
-def _single_zone_run(z, c, steps):
-    s = Solver(2, z["n"], z["h"], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, z["bc"])
+def _single_zone_run(z, c, steps, precision=PRECISION_FP64):
+    s = Solver(2, z["n"], z["h"], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, z["bc"], precision=precision)
     s.set_state(c)
     t = s.rk3_steps_cfl(0.5, steps)
     out = s.get_state()
@@ -202,3 +202,36 @@ def test_config_b_96_zones_128_1024():
         assert np.array_equal(got, single), i
         assert times[i] == t, i
     b.close()
+
+
+def test_batch_fp32_equals_single_zone_fp32_runs():
+    """FP32 batches (FP32 sweeps over FP32 pools, FP64 state and stage
+    combination): every zone equals its FP32 single-zone run bit for bit in
+    all three modes (same kernels, same per-zone dt), and the batch's
+    per-variable L1 / Linf difference to the FP64 batch is bounded."""
+    zs = _config_b_zones(6, 48, 160, seed=777)
+    steps = 3
+    res = {}
+    for mode in (BATCH_GRAPH, BATCH_PERSISTENT, BATCH_PER_ZONE):
+        b = Batch(2, [z for z, _ in zs], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR, precision=PRECISION_FP32)
+        for i, (_, c) in enumerate(zs):
+            b.set_state(i, c)
+        b.steps_cfl(0.5, steps, mode)
+        res[mode] = ([b.get_state(i) for i in range(len(zs))], b.times())
+        b.close()
+    b64 = Batch(2, [z for z, _ in zs], InterMethod.WCNS5, FluxMethod.HLLC, Recon.CHAR)
+    for i, (_, c) in enumerate(zs):
+        b64.set_state(i, c)
+    b64.steps_cfl(0.5, steps, BATCH_GRAPH)
+    f64 = [b64.get_state(i) for i in range(len(zs))]
+    b64.close()
+    for i, (z, c) in enumerate(zs):
+        single, t = _single_zone_run(z, c, steps, precision=PRECISION_FP32)
+        for mode, (states, times) in res.items():
+            assert np.array_equal(states[i], single), (mode, i)
+            assert times[i] == t, (mode, i)
+        scale = np.abs(f64[i]).max(axis=0)
+        l1 = np.abs(single - f64[i]).mean(axis=0) / scale
+        linf = np.abs(single - f64[i]).max(axis=0) / scale
+        print(f"fp32 batch zone {i} {z['n']} vs fp64: L1 {l1} Linf {linf}", flush=True)
+        assert np.all(l1 < 1e-5) and np.all(linf < 1e-3), i

@@ -110,3 +110,42 @@ def test_fp32_teno_a_rhs_against_oracle(inter):
     _say(f"fp32 {inter.name} rhs vortex 128^2: L1", l1, "Linf", linf)
     assert np.all(linf < 1e-4)
     assert np.all(l1 < 1e-5)
+
+
+def _sod_run(prec, recon, n=400, t_end=0.2):
+    """Config S: Sod tube, 400 cells, WCNS5-JS + HLLC, CFL 0.5 to t = 0.2
+    (stepsLoopCFL's loop on the device, dt clamped at the output time)."""
+    cons, h = cases.sod(n)
+    s = Solver(1, (n,), h, InterMethod.WCNS5, FluxMethod.HLLC, recon, [BC.OUTFLOW] * 6, precision=prec)
+    s.set_state(cons)
+    out_t, reached, steps = 0.0, False, 0
+    while not reached:
+        log, out_t, reached = s.steps_to_output(0.5, t_end, out_t, max_steps=256)
+        steps += len(log)
+    got, t = s.get_state(), s.get_time()
+    s.close()
+    return got, t, steps
+
+
+@pytest.mark.parametrize("recon", [Recon.COMPONENT, Recon.CHAR])
+def test_fp32_1d_sod_against_fp64(recon):
+    """FP32 sweeps in 1D (component-wise, and characteristic-wise with the
+    positivity limiter): config S to t = 0.2 against the FP64 run, which
+    equals the oracle bit for bit; both runs' density error against the exact
+    Riemann solution is printed and the FP32 one may exceed FP64's by 1 %."""
+    r64, t64, n64 = _sod_run(PRECISION_FP64, recon)
+    r32, t32, n32 = _sod_run(PRECISION_FP32, recon)
+    # the clock advances as the reference's ((t + dt) - dt/2 - dt/2) + dt,
+    # so the output time is reached to rounding
+    assert abs(t64 - 0.2) < 1e-12 and abs(t32 - 0.2) < 1e-12
+    l1, linf = _errs(r32, r64)
+    n = r64.shape[0]
+    x = (np.arange(n) + 0.5) / n
+    rho_ex = cases.sod_exact(x, 0.2)[:, 0]
+    e64 = np.abs(r64[:, 0] - rho_ex).mean()
+    e32 = np.abs(r32[:, 0] - rho_ex).mean()
+    _say(f"fp32 1D Sod {recon.name}: steps {n32} (fp64 {n64}), L1 {l1} Linf {linf}; "
+         f"L1(rho - exact) fp64 {e64:.6e} fp32 {e32:.6e}")
+    assert np.all(l1 < 1e-5)
+    assert np.all(linf < 1e-3)
+    assert e32 < 1.01 * e64
