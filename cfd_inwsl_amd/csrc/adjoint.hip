@@ -123,12 +123,12 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
   f.tflag = 0u;
   for (int k = 0; k < 4; k++) {
     double l[4];
-    es.lrow(k, l);
+    if (k != 0) es.lrow(k, l);
     double ch[6];
     for (int j = 0; j < 6; j++) {
       double c[4];
       for (int v = 0; v < 4; v++) c[v] = w.cons(j, v);
-      ch[j] = cfd::project<2>(c, l);
+      ch[j] = es.proj(k, l, c);
     }
     if constexpr (IP::kLinear) {
       const unsigned fl = cfd::cut_flag<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, rc);
@@ -142,21 +142,7 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
     }
   }
   // charToCons in the summation order (ch3, ch4, ch2, ch1)
-  const int order[4] = {2, 3, 1, 0};
-  for (int t = 0; t < 4; t++) {
-    const int k = order[t];
-    double r[4];
-    es.rcol(k, r);
-    for (int i = 0; i < 4; i++) {
-      if (t == 0) {
-        f.cL[i] = f.QL[k] * r[i];
-        f.cR[i] = f.QR[k] * r[i];
-      } else {
-        f.cL[i] = f.cL[i] + f.QL[k] * r[i];
-        f.cR[i] = f.cR[i] + f.QR[k] * r[i];
-      }
-    }
-  }
+  es.back(f.QL, f.QR, f.cL, f.cR);
   cfd::eig_cons_to_prim<2>(f.cL, f.oL);
   cfd::eig_cons_to_prim<2>(f.cR, f.oR);
   f.fbL = f.oL[0] < 0 || f.oL[3] < 0;

@@ -541,6 +541,8 @@ template <int D, class T> __device__ __forceinline__ void eig_cons_to_prim(const
   }
 }
 
+template <int D, class T> __device__ __forceinline__ T project(const T* c, const T* l);
+
 // ---- Roe-averaged characteristic system (2D / 3D) --------------------------
 // eigensystemEuler2D(priml, primr, norm) (eigenSystem.cpp:13-86).  The 3D
 // system adds a second shear wave last; for x/y normals the first shear wave
@@ -574,13 +576,16 @@ template <int D, int AX, class T> struct RoeSys {
     u = (c1 * ul + c2 * ur) * divisor;
     v = (c1 * vl + c2 * vr) * divisor;
     T ht = (c1 * HL + c2 * HR) * divisor;
+    // Vn = nx u + ny v [+ nz w] of eigenSystem.cpp with the constant normal:
+    // the zero terms add zeros to the one nonzero product, so Vn is the
+    // normal component itself (the sign of an exact zero aside)
     if (D == 2) {
       w = T(0.0);
-      Vn = nx * u + ny * v;
+      Vn = AX == 0 ? u : v;
       ek = (u * u + v * v) / 2;
     } else {
       w = (c1 * wl + c2 * wr) * divisor;
-      Vn = nx * u + ny * v + nz * w;
+      Vn = AX == 0 ? u : (AX == 1 ? v : w);
       ek = (u * u + v * v + w * w) / 2;
     }
     h = ht - ek;
@@ -602,16 +607,18 @@ template <int D, int AX, class T> struct RoeSys {
       if (D == 3) l[3] = w;
       l[NV - 1] = -T(1.0);
     } else if (k == 2) {
+      // (n_d / c - u_d / h) / 2 with n_d = -1 on the normal axis; on the
+      // others 0 / c is +0 and (+0 - u_d / h) / 2 = -(u_d / h) / 2
       l[0] = (fdv(Vn, c) + fdv(ek, h)) / 2;
-      l[1] = (fdv(-nx, c) - fdv(u, h)) / 2;
-      l[2] = (fdv(-ny, c) - fdv(v, h)) / 2;
-      if (D == 3) l[3] = (fdv(-nz, c) - fdv(w, h)) / 2;
+      l[1] = AX == 0 ? (fdv(T(-1.0), c) - fdv(u, h)) / 2 : -fdv(u, h) / 2;
+      l[2] = AX == 1 ? (fdv(T(-1.0), c) - fdv(v, h)) / 2 : -fdv(v, h) / 2;
+      if (D == 3) l[3] = AX == 2 ? (fdv(T(-1.0), c) - fdv(w, h)) / 2 : -fdv(w, h) / 2;
       l[NV - 1] = fdv(T(1.0), T(2.0) * h);
     } else if (k == 3) {
       l[0] = (fdv(-Vn, c) + fdv(ek, h)) / 2;
-      l[1] = (fdv(nx, c) - fdv(u, h)) / 2;
-      l[2] = (fdv(ny, c) - fdv(v, h)) / 2;
-      if (D == 3) l[3] = (fdv(nz, c) - fdv(w, h)) / 2;
+      l[1] = AX == 0 ? (fdv(T(1.0), c) - fdv(u, h)) / 2 : -fdv(u, h) / 2;
+      l[2] = AX == 1 ? (fdv(T(1.0), c) - fdv(v, h)) / 2 : -fdv(v, h) / 2;
+      if (D == 3) l[3] = AX == 2 ? (fdv(T(1.0), c) - fdv(w, h)) / 2 : -fdv(w, h) / 2;
       l[NV - 1] = fdv(T(1.0), T(2.0) * h);
     } else {  // k == 4, 3D only: shear 2 along t2
       if (AX == 2) { l[0] = -v; l[1] = T(0.0); l[2] = T(1.0); l[3] = T(0.0); }
@@ -646,6 +653,69 @@ template <int D, int AX, class T> struct RoeSys {
       r[0] = T(0.0);
       if (AX == 2) { r[1] = T(0.0); r[2] = T(1.0); r[3] = T(0.0); r[4] = v; }
       else { r[1] = T(0.0); r[2] = T(0.0); r[3] = T(1.0); r[4] = w; }
+    }
+  }
+
+  // Left row k applied to a conservative state: primToChar's sum in
+  // project<D>'s order, with the constant zeros and units of the shear rows
+  // folded (a product c_i * 0 adds a zero to a nonzero partial sum, c_i * -1
+  // is -c_i exactly), so the value is the full sum's, bit for bit, up to the
+  // sign of an exact-zero result.  l holds lrow(k) for the other rows.
+  //   k = 0, x normal: (v, 0, -1, 0[, 0]);  y normal, 3D z normal: (-u, 1, 0, 0[, 0])
+  //   k = 4 (3D), z normal: (-v, 0, 1, 0, 0);  x / y normal: (-w, 0, 0, 1, 0)
+  __device__ __forceinline__ T proj(int k, const T* l, const T* cc) const {
+    if (k == 0) return AX == 0 ? cc[0] * v - cc[2] : cc[1] - cc[0] * u;
+    if (D == 3 && k == 4) return AX == 2 ? cc[2] - cc[0] * v : cc[3] - cc[0] * w;
+    return project<D>(cc, l);
+  }
+
+  // charToCons: o_i = sum of Q_k r_k[i] over k = 2, 3, 1, 0[, 4] (the
+  // reference's summation order), the right vectors' constant zeros and
+  // units folded as in proj:
+  //   r_2 = (1, u - n_u c, v - n_v c[, w - n_w c], h + ek - Vn c), r_3 the same with + c
+  //   r_1 = (1, u, v[, w], ek) / h
+  //   r_0: x normal (0, 0, -1[, 0], -v); y normal, 3D z normal (0, 1, 0[, 0], u)
+  //   r_4 (3D): z normal (0, 0, 1, 0, v); x / y normal (0, 0, 0, 1, w)
+  __device__ __forceinline__ void back(const T* QL, const T* QR, T* oL, T* oR) const {
+    constexpr int E = NV - 1;
+    auto both = [&](int i, int k, T r) {  // o_i = Q_k r (first term)
+      oL[i] = QL[k] * r;
+      oR[i] = QR[k] * r;
+    };
+    auto add = [&](int i, int k, T r) {  // o_i += Q_k r
+      oL[i] = oL[i] + QL[k] * r;
+      oR[i] = oR[i] + QR[k] * r;
+    };
+    oL[0] = QL[2];
+    oR[0] = QR[2];
+    both(1, 2, AX == 0 ? u - c : u);
+    both(2, 2, AX == 1 ? v - c : v);
+    if (D == 3) both(3, 2, AX == 2 ? w - c : w);
+    both(E, 2, h + ek - Vn * c);
+    oL[0] = oL[0] + QL[3];
+    oR[0] = oR[0] + QR[3];
+    add(1, 3, AX == 0 ? u + c : u);
+    add(2, 3, AX == 1 ? v + c : v);
+    if (D == 3) add(3, 3, AX == 2 ? w + c : w);
+    add(E, 3, h + ek + Vn * c);
+    add(0, 1, fdv(T(1.0), h));
+    add(1, 1, fdv(u, h));
+    add(2, 1, fdv(v, h));
+    if (D == 3) add(3, 1, fdv(w, h));
+    add(E, 1, fdv(ek, h));
+    if (AX == 0) {
+      oL[2] = oL[2] - QL[0];
+      oR[2] = oR[2] - QR[0];
+      add(E, 0, -v);
+    } else {
+      oL[1] = oL[1] + QL[0];
+      oR[1] = oR[1] + QR[0];
+      add(E, 0, u);
+    }
+    if (D == 3) {
+      oL[AX == 2 ? 2 : 3] = oL[AX == 2 ? 2 : 3] + QL[4];
+      oR[AX == 2 ? 2 : 3] = oR[AX == 2 ? 2 : 3] + QR[4];
+      add(4, 4, AX == 2 ? v : w);
     }
   }
 };
@@ -779,7 +849,6 @@ template <class T> __device__ __forceinline__ void roe1d(const T* L, const T* Rr
 template <int D, int AX, class T>
 __device__ __forceinline__ void hllc_md(const T* Lp, const T* Rp, T* res, int* hb = nullptr) {
   constexpr int NV = D + 2, E = NV - 1;
-  const T n[3] = {AX == 0 ? T(1.0) : T(0.0), AX == 1 ? T(1.0) : T(0.0), AX == 2 ? T(1.0) : T(0.0)};
   T rl = Lp[0], rr = Rp[0], pl = Lp[E], pr = Rp[E];
   T HL, HR;
   if (D == 2) {
@@ -804,9 +873,11 @@ __device__ __forceinline__ void hllc_md(const T* Lp, const T* Rp, T* res, int* h
     *hb = (T(0.0) < psr ? 0 : 1) | (ps <= pl ? 0 : 2) | (ps <= pr ? 0 : 4) | (reg << 3);
   }
 
+  // products with the normal's zero components left out (zeros added to
+  // nonzero terms: the same values up to the sign of an exact zero)
   auto side = [&](const T* P, T r, T p, T H, T Vn, T* F) {
     F[0] = r * Vn;
-    for (int k = 0; k < D; k++) F[1 + k] = r * P[1 + k] * Vn + p * n[k];
+    for (int k = 0; k < D; k++) F[1 + k] = k == AX ? r * P[1 + k] * Vn + p : r * P[1 + k] * Vn;
     F[E] = r * H * Vn;
   };
   auto star = [&](const T* P, T r, T p, T H, T Vn, T S, T* out) {
@@ -816,10 +887,9 @@ __device__ __forceinline__ void hllc_md(const T* Lp, const T* Rp, T* res, int* h
     T UE = r * H - p;
     out[0] = F[0] + S * (Usf * r - r);
     for (int k = 0; k < D; k++) {
-      T other = T(0.0);
-      for (int m = 0; m < D; m++)
-        if (m != k) other += n[m];
-      T Us = Usf * (Sstar * n[k] + P[1 + k] * other) * r;
+      // Sstar n_k + u_k (sum of the other normal components): Sstar on the
+      // normal axis, u_k on the others
+      T Us = Usf * (k == AX ? Sstar : P[1 + k]) * r;
       out[1 + k] = F[1 + k] + S * (Us - r * P[1 + k]);
     }
     T UsE = Usf * (UE + (Sstar - Vn) * (r * Sstar + fdv(p, S - Vn)));
@@ -845,7 +915,6 @@ __device__ __forceinline__ void hllc_md(const T* Lp, const T* Rp, T* res, int* h
 // form sums the shear energy over the tangential components in (u, v, w) order.
 template <int D, int AX, class T> __device__ __forceinline__ void roe_md(const T* Lp, const T* Rp, T* res) {
   constexpr int NV = D + 2, E = NV - 1;
-  const T n[3] = {AX == 0 ? T(1.0) : T(0.0), AX == 1 ? T(1.0) : T(0.0), AX == 2 ? T(1.0) : T(0.0)};
   T rl = Lp[0], rr = Rp[0], pl = Lp[E], pr = Rp[E];
   T HL, HR;
   if (D == 2) {
@@ -882,19 +951,22 @@ template <int D, int AX, class T> __device__ __forceinline__ void roe_md(const T
     shear = first ? t : shear + t;
     first = false;
   }
+  // The products with the constant normal's zero components (c n_k, p n_k,
+  // rhoAvg * 0 for the normal momentum) only add zeros to nonzero terms and
+  // are left out: the same values, up to the sign of an exact-zero result.
   // component 0
   {
     T fd = lam0 * (cf1 * 1);
     fd += lam2 * (cf * 1);
-    fd += lam1 * (cf2 * T(1.0) + rhoAvg * T(0.0));
+    fd += lam1 * cf2;
     res[0] = (rl * VnL + rr * VnR - fd) / 2;
   }
   for (int k = 0; k < D; k++) {
-    T fd = lam0 * (cf1 * (va[k] - cAvg * n[k]));
-    fd += lam2 * (cf * (va[k] + cAvg * n[k]));
-    fd += lam1 * (cf2 * va[k] + rhoAvg * (k == AX ? T(0.0) : (Rp[1 + k] - Lp[1 + k])));
-    T fl = rl * Lp[1 + k] * VnL + pl * n[k];
-    T fr = rr * Rp[1 + k] * VnR + pr * n[k];
+    T fd = lam0 * (cf1 * (k == AX ? va[k] - cAvg : va[k]));
+    fd += lam2 * (cf * (k == AX ? va[k] + cAvg : va[k]));
+    fd += lam1 * (k == AX ? cf2 * va[k] : cf2 * va[k] + rhoAvg * (Rp[1 + k] - Lp[1 + k]));
+    T fl = k == AX ? rl * Lp[1 + k] * VnL + pl : rl * Lp[1 + k] * VnL;
+    T fr = k == AX ? rr * Rp[1 + k] * VnR + pr : rr * Rp[1 + k] * VnR;
     res[1 + k] = (fl + fr - fd) / 2;
   }
   {
@@ -915,18 +987,18 @@ template <int D, int AX, class T> __device__ __forceinline__ void node_flux(cons
     return;
   }
   constexpr int NV = D + 2, E = NV - 1;
-  const T n[3] = {AX == 0 ? T(1.0) : T(0.0), AX == 1 ? T(1.0) : T(0.0), AX == 2 ? T(1.0) : T(0.0)};
   T r = p[0], pp = p[E];
-  T Vn, k2;
+  // Vn = sum p_k n_k and n_k pp with the constant normal's zero products
+  // left out (the same values up to the sign of an exact zero)
+  T k2;
+  const T Vn = p[1 + AX];
   if (D == 2) {
-    Vn = (p[1] * n[0] + p[2] * n[1]);
     k2 = (p[1] * p[1] + p[2] * p[2]);
   } else {
-    Vn = (p[1] * n[0] + p[2] * n[1] + p[3] * n[2]);
     k2 = (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
   }
   F[0] = r * Vn;
-  for (int k = 0; k < D; k++) F[1 + k] = r * p[1 + k] * Vn + n[k] * pp;
+  for (int k = 0; k < D; k++) F[1 + k] = k == AX ? r * p[1 + k] * Vn + pp : r * p[1 + k] * Vn;
   F[E] = (k2 / 2 * r + T(CFD_GAMMA) / (T(CFD_GAMMA) - T(1)) * pp) * Vn;
 }
 

@@ -115,25 +115,29 @@ __host__ __device__ constexpr int tile_cp(int t) { return t == 0 ? 8 : (t == 1 ?
 // the operands are loaded in the cell phase as before.
 template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T) == sizeof(double); };
 // Waves per SIMD the compiler must leave room for (launch bounds), from
-// same-box A/B runs:
-// * the 2D FP64 Roe x sweep fits 168 VGPRs without spills and runs faster at
-//   three waves (0.256 -> 0.237 ms at 2048^2); its y sweep spills there and
-//   loses more than it gains, and so do the HLLC kernels.  The three-wave
-//   kernel drops the prefetch (the first sweep has nothing to prefetch);
-// * the 3D FP64 WENO-JS / WENO-Z / TENO-A kernels would otherwise take more
-//   than 256 registers and run one wave per SIMD; bounded to two they spill
-//   a few registers and run 17-41 % faster at 256^3 (without the prefetch,
-//   which bounded kernels do not take).  The 3D TENO5 kernel (its keep-all
-//   test costs registers) is bounded to two waves too but keeps the
-//   prefetch: without it the same bound is 15 % slower.  TENO-S fits two
-//   waves by itself and loses 12-15 % when bounded.
+// same-box A/B runs.  Without a bound the FP64 characteristic-wise kernels
+// that need ~256 registers take a few accumulation registers on top and run
+// one wave per SIMD; bounded to two they run 2-40 % faster:
+// * every FP64 x sweep of 2D / 3D (the first sweep of a stage reads neither
+//   R nor U^n, so it has nothing to prefetch and no prefetch registers):
+//   2D WENO-JS + HLLC 0.317 -> 0.233 ms at 2048^2, 3D TENO-S 1.72 -> 1.06 ms
+//   at 256^3; the 2D Roe x sweep spills 76 registers at three waves and is
+//   7 % slower there;
+// * the 3D WENO-JS / WENO-Z / TENO-A / TENO5 y and z sweeps (17-41 % at
+//   256^3; TENO5 keeps the prefetch, 15 % faster than without it), and the
+//   research variants' kernels.
+// The other y / z sweeps fit two waves with the prefetch (TENO-S 3D loses
+// 20-30 % in y / z when bounded).
 template <int D, int AX, int SCH, int FLX, class T> struct SweepOcc {
   static constexpr bool dbl = sizeof(T) == sizeof(double);
-  static constexpr int waves = (D == 2 && AX == 0 && FLX == FLX_ROE && dbl)                               ? 3
-                               : (D == 3 && dbl && (SCH == SCH_JS || SCH == SCH_Z || SCH == SCH_TENOA || SCH == SCH_TENO)) ? 2
-                                                                                                          : 1;
+  static constexpr int waves = !dbl ? 1
+                               : (D >= 2 && AX == 0) ? 2
+                               : (D == 3 && (SCH == SCH_JS || SCH == SCH_Z || SCH == SCH_TENOA || SCH == SCH_TENO)) ? 2
+                               : (D >= 2 && (SCH == SCH_TENOSA || SCH == SCH_WHF)) ? 2
+                                                                                  : 1;
   // the cell phase's operands are prefetched where the registers allow it
-  static constexpr bool prefetch = waves == 1 || (D == 3 && dbl && SCH == SCH_TENO);
+  // (never in the first sweep of a 2D / 3D stage: nothing to prefetch)
+  static constexpr bool prefetch = (D == 1 || AX != 0) && (waves == 1 || (D == 3 && dbl && SCH == SCH_TENO));
 };
 
 // One workgroup's share of a sweep: pencil group bx, segment by.
@@ -381,9 +385,9 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
             unsigned rbits = 0u;  // REC: branch bits of this interface
             for (int k = 0; k < NV; k++) {
               T l[NV];
-              es.lrow(k, l);
+              if (k != 0 && !(D == 3 && k == 4)) es.lrow(k, l);
               T ch[6];
-              for (int j = 0; j < 6; j++) ch[j] = project<D>(cw[j], l);
+              for (int j = 0; j < 6; j++) ch[j] = es.proj(k, l, cw[j]);
               if (REC && TenoFamily<SCH>::value) {
                 // inter5's own evaluation for these schemes, flags kept
                 const unsigned fl = cut_flag<SCH>(ch[0], ch[1], ch[2], ch[3], ch[4], ct, &a.aux->rc);
@@ -407,21 +411,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
             }
             // charToCons in the summation order (ch3, ch4, ch2, ch1[, ch5])
             T cL[NV], cR[NV];
-            const int order[5] = {2, 3, 1, 0, 4};
-            for (int t = 0; t < NV; t++) {
-              const int k = order[t];
-              T r[NV];
-              es.rcol(k, r);
-              for (int i = 0; i < NV; i++) {
-                if (t == 0) {
-                  cL[i] = QL[k] * r[i];
-                  cR[i] = QR[k] * r[i];
-                } else {
-                  cL[i] = cL[i] + QL[k] * r[i];
-                  cR[i] = cR[i] + QR[k] * r[i];
-                }
-              }
-            }
+            es.back(QL, QR, cL, cR);
             eig_cons_to_prim<D>(cL, outL);
             eig_cons_to_prim<D>(cR, outR);
             // positivity fallback (reconstructor5order.hpp:301-302)
