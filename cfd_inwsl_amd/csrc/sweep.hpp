@@ -127,10 +127,13 @@ template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T
 //   256^3; TENO5 keeps the prefetch, 15 % faster than without it), and the
 //   research variants' kernels.
 // The other y / z sweeps fit two waves with the prefetch (TENO-S 3D loses
-// 20-30 % in y / z when bounded).
+// 20-30 % in y / z when bounded).  The FP32 3D TENO-family kernels are
+// bounded to three waves without the prefetch (x: 168 VGPRs, no spills;
+// y / z then take ~120 and run four): 3D 256^3 TENO5 x sweep 0.91 -> 0.74 ms,
+// +8 % per step; the FP32 WENO-JS / WENO-Z x sweeps would spill there.
 template <int D, int AX, int SCH, int FLX, class T> struct SweepOcc {
   static constexpr bool dbl = sizeof(T) == sizeof(double);
-  static constexpr int waves = !dbl ? 1
+  static constexpr int waves = !dbl ? (D == 3 && SCH != SCH_JS && SCH != SCH_Z ? 3 : 1)
                                : (D >= 2 && AX == 0) ? 2
                                : (D == 3 && (SCH == SCH_JS || SCH == SCH_Z || SCH == SCH_TENOA || SCH == SCH_TENO)) ? 2
                                : (D >= 2 && (SCH == SCH_TENOSA || SCH == SCH_WHF)) ? 2
