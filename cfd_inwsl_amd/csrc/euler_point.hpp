@@ -65,6 +65,19 @@ template <> __device__ __forceinline__ double fdv<double>(double a, double b) {
   return __builtin_fma(t, r, q);
 }
 template <class T> __device__ __forceinline__ T fsq(T x) { return sqrt(x); }
+// FP32 (the storage variant, compared with the FP64 path by its L1 / Linf
+// error, not bit for bit): the hardware reciprocal (1 ulp) with one residual
+// correction, and the hardware square root (1 ulp), in place of the
+// compiler's correctly rounded sequences -- about a quarter of the FP32
+// sweeps' instructions.  Operands are nonzero normal divisors and
+// non-negative normal (or zero) square-root arguments.
+template <> __device__ __forceinline__ float fdv<float>(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  const float t = __builtin_fmaf(-b, q, a);
+  return __builtin_fmaf(t, r, q);
+}
+template <> __device__ __forceinline__ float fsq<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double fsq<double>(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = y * 0.5;
@@ -220,12 +233,12 @@ __device__ __forceinline__ unsigned teno5_flag(T q0, T q1, T q2, T q3, T q4, T c
     T base = T(1.0);
     T num = tau;
     if (tau > sm) {
-      base = sm / tau;
+      base = fdv(sm, tau);
       num = sm;
     }
-    t0 = base + num / s0;
-    t1 = base + num / s1;
-    t2 = base + num / s2;
+    t0 = base + fdv(num, s0);
+    t1 = base + fdv(num, s1);
+    t2 = base + fdv(num, s2);
   }
   t0 = t0 * t0;
   t1 = t1 * t1;
@@ -273,12 +286,12 @@ __device__ __forceinline__ unsigned cut_flag(T q0, T q1, T q2, T q3, T q4, T ct,
       T base = T(1.0);
       T num = tau;
       if (tau > sm) {
-        base = sm / tau;
+        base = fdv(sm, tau);
         num = sm;
       }
-      t0 = base + num / s0;
-      t1 = base + num / s1;
-      t2 = base + num / s2;
+      t0 = base + fdv(num, s0);
+      t1 = base + fdv(num, s1);
+      t2 = base + fdv(num, s2);
     }
     t0 = t0 * t0;
     t1 = t1 * t1;
@@ -456,12 +469,12 @@ __device__ __forceinline__ T inter5(T q0, T q1, T q2, T q3, T q4, T ct, const Re
       T base = T(1.0);
       T num = tau;
       if (tau > sm) {
-        base = sq(sm / tau);
+        base = sq(fdv(sm, tau));
         num = sm;
       }
-      a0 = T(0.0625) * (base + sq(num / s0));
-      a1 = T(0.625) * (base + sq(num / s1));
-      a2 = T(0.3125) * (base + sq(num / s2));
+      a0 = T(0.0625) * (base + sq(fdv(num, s0)));
+      a1 = T(0.625) * (base + sq(fdv(num, s1)));
+      a2 = T(0.3125) * (base + sq(fdv(num, s2)));
     }
     T s = a0 + a1 + a2;
     T r = a0 * u0 + a1 * u1 + a2 * u2;
