@@ -103,16 +103,14 @@ template <> struct Tile<2> { static constexpr int CA = 8, CP = 32; };
 __host__ __device__ constexpr int tile_ca(int t) { return t == 0 ? 32 : (t == 1 ? 16 : 8); }
 __host__ __device__ constexpr int tile_cp(int t) { return t == 0 ? 8 : (t == 1 ? 16 : 32); }
 
-#ifndef CFD_SWEEP_WAVES_PER_EU
-#define CFD_SWEEP_WAVES_PER_EU 1
-#endif
 // Prefetch (template flag PF of sweep_block): the cell phase's global
 // operands (R, U^n, U^(k)) are loaded when the chunk starts and held in
 // registers across the interface phase.  At two waves per SIMD (the FP64
 // kernels) loads issued in the cell phase itself stall it (3D 256^3 y/z
 // sweeps 1.47/1.62 -> 1.27/1.31 ms); where the extra registers would cost
-// occupancy (FP32 kernels: 4 -> 3 waves; the persistent batch kernel: 2 -> 1)
-// the operands are loaded in the cell phase as before.
+// occupancy (the FP32 kernels, the bounded kernels; in the persistent batch
+// kernel every sweep but the 2D FP64 y sweep) the operands are loaded in the
+// cell phase.
 template <class T> struct SweepPrefetch { static constexpr bool value = sizeof(T) == sizeof(double); };
 // Waves per SIMD the compiler must leave room for (launch bounds), from
 // same-box A/B runs.  Without a bound the FP64 characteristic-wise kernels
@@ -635,9 +633,11 @@ struct BatchArgs {
 };
 
 // Persistent kernel: each workgroup pulls work items from the queue until it
-// is empty, so one launch covers every zone of the batch.
+// is empty, so one launch covers every zone of the batch.  Two waves per
+// SIMD; the 2D FP64 y sweep prefetches its cell-phase operands (96 uneven
+// zones, WCNS5 + HLLC: 7160 -> 8145 Mcell-stage/s, same bits).
 template <int D, int AX, int SCH, int FLX, bool CHARW, int TILE, class T>
-__global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kernel(BatchArgs b) {
+__global__ __launch_bounds__(256, 2) void sweep_batch_kernel(BatchArgs b) {
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)atomicAdd(b.counter, 1u);
@@ -674,8 +674,8 @@ __global__ __launch_bounds__(256, CFD_SWEEP_WAVES_PER_EU) void sweep_batch_kerne
     a.bc_lo = zd.bc[2 * AX];
     a.bc_hi = zd.bc[2 * AX + 1];
     a.aux = &zd.aux[AX];  // batches take neither position-dependent boundaries nor sources
-    sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, false>(a, __builtin_amdgcn_readfirstlane(w.bx),
-                                              __builtin_amdgcn_readfirstlane(w.by));
+    sweep_block<D, AX, SCH, FLX, CHARW, TILE, T, (D == 2 && AX != 0 && sizeof(T) == sizeof(double))>(
+        a, __builtin_amdgcn_readfirstlane(w.bx), __builtin_amdgcn_readfirstlane(w.by));
   }
 }
 
