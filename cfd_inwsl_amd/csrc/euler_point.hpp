@@ -93,6 +93,14 @@ template <> __device__ __forceinline__ double fsq<double>(double x) {
   return x == 0.0 ? x : g;
 }
 
+// a / (2 b) as (a / b) / 2: 2 b is exact and a correctly rounded quotient
+// scales exactly by 1/2 (operands far from the exponent limits), so this is
+// the reference's a / (2 * b) bit for bit and shares the refined reciprocal
+// of b with the other divisions by b (Roe: 2 c^2 and c^2, eigenvectors:
+// 2 h and h).  FP32 keeps the plain form.
+template <class T> __device__ __forceinline__ T half_quot(T a, T b) { return fdv(a, T(2.0) * b); }
+template <> __device__ __forceinline__ double half_quot<double>(double a, double b) { return fdv(a, b) * 0.5; }
+
 // Constants of the research TENO variants, from the host's libm pow as the
 // reference computes them: ctp[k-5] = pow(1.5 c / (1 - c), 1/6) with
 // c = pow(10, -k), cta[k-5] = pow(10, -k) (k = 5..10), ctp_s the first formula
@@ -626,13 +634,13 @@ template <int D, int AX, class T> struct RoeSys {
       l[1] = AX == 0 ? (fdv(T(-1.0), c) - fdv(u, h)) / 2 : -fdv(u, h) / 2;
       l[2] = AX == 1 ? (fdv(T(-1.0), c) - fdv(v, h)) / 2 : -fdv(v, h) / 2;
       if (D == 3) l[3] = AX == 2 ? (fdv(T(-1.0), c) - fdv(w, h)) / 2 : -fdv(w, h) / 2;
-      l[NV - 1] = fdv(T(1.0), T(2.0) * h);
+      l[NV - 1] = half_quot(T(1.0), h);
     } else if (k == 3) {
       l[0] = (fdv(-Vn, c) + fdv(ek, h)) / 2;
       l[1] = AX == 0 ? (fdv(T(1.0), c) - fdv(u, h)) / 2 : -fdv(u, h) / 2;
       l[2] = AX == 1 ? (fdv(T(1.0), c) - fdv(v, h)) / 2 : -fdv(v, h) / 2;
       if (D == 3) l[3] = AX == 2 ? (fdv(T(1.0), c) - fdv(w, h)) / 2 : -fdv(w, h) / 2;
-      l[NV - 1] = fdv(T(1.0), T(2.0) * h);
+      l[NV - 1] = half_quot(T(1.0), h);
     } else {  // k == 4, 3D only: shear 2 along t2
       if (AX == 2) { l[0] = -v; l[1] = T(0.0); l[2] = T(1.0); l[3] = T(0.0); }
       else { l[0] = -w; l[1] = T(0.0); l[2] = T(0.0); l[3] = T(1.0); }
@@ -841,8 +849,8 @@ template <class T> __device__ __forceinline__ void roe1d(const T* L, const T* Rr
   for (int i = 0; i < 3; i++)
     if (lam[i] < eps) lam[i] = fdv(lam[i] * lam[i] + eps * eps, T(2.0) * eps);
   T a0 = lam[0] * (dr - fdv(dp, cBar2));
-  T a1 = fdv(lam[1], T(2.0) * cBar2) * (dp + rBar * cBar * du);
-  T a2 = fdv(lam[2], T(2.0) * cBar2) * (dp - rBar * cBar * du);
+  T a1 = half_quot(lam[1], cBar2) * (dp + rBar * cBar * du);
+  T a2 = half_quot(lam[2], cBar2) * (dp - rBar * cBar * du);
   T a3 = a0 + a1 + a2;
   T a4 = cBar * (a1 - a2);
   T FL[3] = {rl * ul, rl * ul * ul + pl, rl * HL * ul};
@@ -953,8 +961,8 @@ template <int D, int AX, class T> __device__ __forceinline__ void roe_md(const T
   if (lam1 < eps) lam1 = fdv(lam1 * lam1 + eps * eps, T(2.0) * eps);
   if (lam2 < eps) lam2 = fdv(lam2 * lam2 + eps * eps, T(2.0) * eps);
   T dP = pr - pl, dVn = VnR - VnL, dRho = rr - rl;
-  T cf1 = fdv(dP - rhoAvg * cAvg * dVn, T(2.0) * cAvg * cAvg);
-  T cf = fdv(dP + rhoAvg * cAvg * dVn, T(2.0) * cAvg * cAvg);
+  T cf1 = half_quot(dP - rhoAvg * cAvg * dVn, cAvg * cAvg);
+  T cf = half_quot(dP + rhoAvg * cAvg * dVn, cAvg * cAvg);
   T cf2 = dRho - fdv(dP, cAvg * cAvg);
   T shear = T(0.0);
   bool first = true;
