@@ -26,7 +26,11 @@
 // are done.  Ghost-cell cotangents go to a small
 // per-pencil buffer; the x sweep's are folded onto their source cells by the
 // y sweep's cell phase (stream order makes them visible), the y sweep's by a
-// small kernel after it.
+// small kernel after it.  A direction periodic on both faces has no ghosts:
+// its pencils are marched as rings (indices mod n; the interfaces the forward
+// evaluates twice across the seam, with ghost copies, are the same function
+// of the same cells, so the ring's single interface carries their summed
+// cotangent).
 //
 // RK3 (BlockSolver::RK3_SSP, zone_FD/src/src/blockSolver.cpp:43-87):
 //   u1 = u0 - dt R(u0); u2 = 3/4 u0 + 1/4 u1 - 1/4 dt R(u1);
@@ -97,6 +101,10 @@ struct VjpArgs {
   int seg_len;
   double h;
   int bc_lo, bc_hi;
+  // both faces of this direction periodic: the pencil is marched as a ring
+  // (cells and interfaces indexed mod n), so no ghost cell and no ghost
+  // cotangent exists -- every interface's cotangents land on real cells
+  int ring;
   const double* bc_prim;  // Dirichlet ghost states of this direction: low face [0..3], high [4..7]
   // k_rhs_vjp<..., REC = true>: the recording sweep's record of this state
   // and direction (SweepArgs::rec: QL, QR planes, then the branch words)
@@ -187,9 +195,13 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   const long base = pvalid ? (AX == 0 ? gp * nx : gp) : 0;
   const int c0 = blockIdx.y * a.seg_len;
   const int c1 = min(c0 + a.seg_len, n);
-  const int e0 = c0 == 0 ? -4 : c0;  // cells finalised here (ghosts at the ends)
-  const int e1 = c1 == n ? n + 4 : c1;
-  const int kmin = max(e0 - 2, -1), kmax = min(e1 + 2, n + 1);
+  const bool ring = a.ring != 0;
+  // cells finalised here (ghosts at the pencil ends, none on a ring)
+  const int e0 = c0 == 0 && !ring ? -4 : c0;
+  const int e1 = c1 == n && !ring ? n + 4 : c1;
+  const int kmin = ring ? e0 - 2 : max(e0 - 2, -1), kmax = ring ? e1 + 2 : min(e1 + 2, n + 1);
+  // ring index of a cell or interface in [-8, n + 8) (n >= 8)
+  auto wrap = [n](int c) -> int { return c < 0 ? c + n : (c >= n ? c - n : c); };
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
   const double ih = 1.0 / a.h;
 
@@ -200,9 +212,9 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     double u[4], m[4];
   };
   auto fetch = [&](int c, Fetch& f) {
-    const bool in = c >= 0 && c < n;
+    const bool in = ring || (c >= 0 && c < n);
     const cfdadj::GhostSrc gs = cfdadj::ghost_source(c < 0 ? a.bc_lo : a.bc_hi, c, n);
-    const long i = base + (long)(in ? c : gs.fixed ? 0 : gs.src) * sa;
+    const long i = base + (long)(ring ? wrap(c) : in ? c : gs.fixed ? 0 : gs.src) * sa;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       f.u[v] = a.U[v * a.N + i];
@@ -213,7 +225,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   auto commit = [&](int c, const Fetch& f) {
     const int s = slot_of(c);
     double p[4], m[4] = {0.0, 0.0, 0.0, 0.0};
-    if (c >= 0 && c < n) {
+    if (ring || (c >= 0 && c < n)) {
       cfd::cons_to_prim<2>(f.u, p);
 #pragma unroll
       for (int v = 0; v < 4; v++) m[v] = a.coef * f.m[v];
@@ -267,7 +279,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     // this chunk's finalised cell: its state and base terms, loaded now
     const int cf = k0 - 3 + la;
     const bool fin = pvalid && cf >= e0 && cf < e1;
-    const bool fin_in = fin && cf >= 0 && cf < n;
+    const bool fin_in = fin && (ring || (cf >= 0 && cf < n));
     const long fi = base + (long)cf * sa;
     double fu[4], fx[3][4];
     if (fin_in) {
@@ -293,7 +305,8 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     double rq[8];
     unsigned rbits = 0u;
     if (REC && act) {
-      const long ri = AX == 0 ? gp * (n + 3) + (k + 1) : (long)(k + 1) * npencil + gp;
+      const int kr = ring ? wrap(k) : k;  // a ring interface's record: its copy inside the pencil
+      const long ri = AX == 0 ? gp * (n + 3) + (kr + 1) : (long)(kr + 1) * npencil + gp;
 #pragma unroll
       for (int v = 0; v < 8; v++) rq[v] = a.rec[v * a.rec_plane + ri];
       rbits = reinterpret_cast<const unsigned*>(a.rec + 8 * a.rec_plane)[ri];
@@ -307,11 +320,14 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       for (int v = 0; v < 4; v++) lG[v] = (w0 * (M(k - 1, v) - M(k, v)) + w2 * (M(k - 2, v) - M(k + 1, v))) * ih;
       const Cells cells(wc, k, lp);
       cfdadj::IfaceFwd f;
-      if (REC)
-        cfdadj::iface_from_record<AX>(cells, rq, rbits, f);
-      else
+      if (REC) {
+        cfdadj::Roe2<AX> es;
+        cfdadj::iface_from_record<AX>(cells, rq, rbits, f, es);
+        cfdadj::iface_backward<AX, IP>(cells, f, lG, lw, es);
+      } else {
         iface_forward_exact<AX, IP, SCH>(cells, f, a.ct, a.rc);
-      cfdadj::iface_backward<AX, IP>(cells, f, lG, lw);
+        cfdadj::iface_backward<AX, IP>(cells, f, lG, lw);
+      }
     }
     if (act) {
 #pragma unroll
@@ -350,7 +366,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       const int s = slot_of(c);
       double lwc[4];
       for (int v = 0; v < 4; v++) lwc[v] = ac[idx(v, s, lp)];
-      if (c >= -1 && c <= n) {  // node fluxes exist on cells -1 .. n
+      if (ring || (c >= -1 && c <= n)) {  // node fluxes exist on cells -1 .. n
         double p[4], lf[4];
         for (int v = 0; v < 4; v++) {
           p[v] = wc[idx(v, s, lp)];
@@ -516,11 +532,11 @@ static int adj_supported(cfd_handle* h) {
 
 // Segment length of the VJP sweep along a direction of n cells.  A segment of
 // L cells runs interfaces over L + 5 cells (L + 9 at a pencil end, which also
-// finalises the ghosts), in chunks of CA; lengths m*CA - 5 keep every chunk of
+// finalises the ghosts; L + 5 everywhere on a periodic ring), in chunks of CA; lengths m*CA - 5 keep every chunk of
 // an interior segment full.  Cost = lane slots / the fraction of the last
 // round of resident workgroups that is busy (the chip holds 256 * occupancy
 // workgroups of 256 lanes at once).
-static int vjp_plan_seg(int n, long pencil_blocks, int ca, int cp) {
+static int vjp_plan_seg(int n, long pencil_blocks, int ca, int cp, bool ring) {
   const long resident = 256L * CFD_ADJ_OCC;
   double best = 0.0;
   int best_seg = n;
@@ -532,7 +548,7 @@ static int vjp_plan_seg(int n, long pencil_blocks, int ca, int cp) {
     long chunks = 0;
     for (int c0 = 0; c0 < n; c0 += seg) {
       const int len = std::min(seg, n - c0);
-      const int extra = 5 + (c0 == 0 ? 2 : 0) + (c0 + len == n ? 2 : 0);
+      const int extra = 5 + (c0 == 0 && !ring ? 2 : 0) + (c0 + len == n && !ring ? 2 : 0);
       chunks += (len + extra + ca - 1) / ca;
     }
     const long blocks = pencil_blocks * ((n + seg - 1) / seg);
@@ -571,7 +587,8 @@ static int adj_ensure(cfd_handle* h) {
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
     const int ca = ax == 0 ? VjpTile<0>::CA : VjpTile<1>::CA;
     const int cp = ax == 0 ? VjpTile<0>::CP : VjpTile<1>::CP;
-    s->vjp_seg[ax] = vjp_plan_seg(h->cfg.n[ax], (pencils + cp - 1) / cp, ca, cp);
+    const bool ring = h->cfg.bc[2 * ax] == CFD_BC_PERIODIC && h->cfg.bc[2 * ax + 1] == CFD_BC_PERIODIC;
+    s->vjp_seg[ax] = vjp_plan_seg(h->cfg.n[ax], (pencils + cp - 1) / cp, ca, cp, ring);
   }
   size_t off = 0;
   for (int ax = 0; ax < 2; ax++) {
@@ -633,11 +650,13 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     a.GHX = s->GH[0];
     a.gx_src = 0u;
     a.gx_flip = 0u;
+    const bool xring = h->cfg.bc[0] == CFD_BC_PERIODIC && h->cfg.bc[1] == CFD_BC_PERIODIC;
     for (int q = 0; q < 8; q++) {
       const int nx = h->cfg.n[0];
       const cfdadj::GhostSrc o =
           cfdadj::ghost_source(q < 4 ? h->cfg.bc[0] : h->cfg.bc[1], q < 4 ? q - 4 : nx + q - 4, nx);
-      const unsigned code = o.fixed ? 15u : (unsigned)(o.src < 4 ? o.src : o.src - nx + 8);
+      // 15: no source (a fixed state, or no x ghosts at all on an x ring)
+      const unsigned code = o.fixed || xring ? 15u : (unsigned)(o.src < 4 ? o.src : o.src - nx + 8);
       a.gx_src |= code << (4 * q);
       a.gx_flip |= (unsigned)(o.flip > 0 ? o.flip : 0) << (2 * q);
     }
@@ -653,6 +672,7 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     a.h = h->cfg.h[ax];
     a.bc_lo = h->cfg.bc[2 * ax];
     a.bc_hi = h->cfg.bc[2 * ax + 1];
+    a.ring = a.bc_lo == CFD_BC_PERIODIC && a.bc_hi == CFD_BC_PERIODIC;
     a.bc_prim = s->bcprim + 8 * ax;
     const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
@@ -665,7 +685,8 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
       constexpr int CA = VjpTile<1>::CA, CP = VjpTile<1>::CP;
       const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
       launch_vjp<1, CA, CP>(h->sch, grid, h->stream, a);
-      hipLaunchKernelGGL(k_ghost_fold_y, dim3((unsigned)((pencils * 8 + 255) / 256)), dim3(256), 0, h->stream, a);
+      if (!a.ring)
+        hipLaunchKernelGGL(k_ghost_fold_y, dim3((unsigned)((pencils * 8 + 255) / 256)), dim3(256), 0, h->stream, a);
     }
     AHIP(hipGetLastError());
   }

@@ -56,6 +56,25 @@ CFD_HD inline double rcp(double x) {
 #endif
 }
 
+// Square root for the reverse pass (positive arguments: densities, sound
+// speeds, shock factors): on the device the hardware reciprocal square root
+// refined by one Newton step and one residual correction (within an ulp), on
+// the host the library root.
+CFD_HD inline double rt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return x == 0.0 ? x : g;
+#else
+  return sqrt(x);
+#endif
+}
+
 // ---- WENO-JS half-node value and its gradient ---------------------------------
 struct JS {
   static constexpr bool kLinear = false;
@@ -245,7 +264,7 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   const double HL = 0.5 * (ul * ul + vl * vl) + GG1 * pl * irl;
   const double HR = 0.5 * (ur * ur + vr * vr) + GG1 * pr * irr;
   const double VnL = AX == 0 ? ul : vl, VnR = AX == 0 ? ur : vr;
-  const double cl = sqrt(G * pl * irl), cr = sqrt(G * pr * irr);
+  const double cl = rt(G * pl * irl), cr = rt(G * pr * irr);
   const double rBar = 0.5 * (rl + rr), cBar = 0.5 * (cl + cr);
   const double psr = 0.5 * (pl + pr) - 0.5 * rBar * cBar * (VnR - VnL);
   const bool clip = force >= 0 ? (force & HB_CLIP) != 0 : !(0.0 < psr);
@@ -253,8 +272,8 @@ template <int AX> CFD_HD inline void hllc(const double* L, const double* R, doub
   const double kap = (G + 1.0) / (2.0 * G);
   const bool shl = force >= 0 ? (force & HB_SHL) != 0 : !(ps <= pl);
   const bool shr = force >= 0 ? (force & HB_SHR) != 0 : !(ps <= pr);
-  const double ql = shl ? sqrt(1.0 + kap * (ps * ipl - 1.0)) : 1.0;
-  const double qr = shr ? sqrt(1.0 + kap * (ps * ipr - 1.0)) : 1.0;
+  const double ql = shl ? rt(1.0 + kap * (ps * ipl - 1.0)) : 1.0;
+  const double qr = shr ? rt(1.0 + kap * (ps * ipr - 1.0)) : 1.0;
   const double SL = VnL - cl * ql, SR = VnR + cr * qr;
   const double aL = SL - VnL, aR = SR - VnR;
   const double num = (pr - pl) + (rl * VnL * aL - rr * VnR * aR);
@@ -481,8 +500,8 @@ template <int AX> struct Roe2 {
     CFD_CONTRACT
     HL = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2]) + GG1 * pl[3] * rcp(pl[0]);
     HR = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2]) + GG1 * pr[3] * rcp(pr[0]);
-    c1 = sqrt(pl[0]);
-    c2 = sqrt(pr[0]);
+    c1 = rt(pl[0]);
+    c2 = rt(pr[0]);
     D = c1 + c2;
     iD = rcp(D);
     u = (c1 * pl[1] + c2 * pr[1]) * iD;
@@ -491,7 +510,7 @@ template <int AX> struct Roe2 {
     Vn = AX == 0 ? u : v;
     ek = 0.5 * (u * u + v * v);
     h = ht - ek;
-    c = sqrt(G1 * h);
+    c = rt(G1 * h);
     ic = rcp(c);
     ih = rcp(h);
   }
@@ -841,15 +860,16 @@ template <int AX, class IP = JS, class W> CFD_HD inline void iface_forward(const
 // positivity fallback left / right in 24 / 25, HLLC branch bits from 26).
 // The branches are the solver's own; the states rebuilt from Q only feed
 // derivatives, so they are evaluated in this file's contracted arithmetic.
+// es: the Roe-averaged eigensystem of the interface's two cells, built here
+// and handed on to the reverse pass.
 template <int AX, class W>
-CFD_HD inline void iface_from_record(const W& w, const double* Q, unsigned bits, IfaceFwd& f) {
+CFD_HD inline void iface_from_record(const W& w, const double* Q, unsigned bits, IfaceFwd& f, Roe2<AX>& es) {
   CFD_CONTRACT
   double pl[4], pr[4];
   for (int v = 0; v < 4; v++) {
     pl[v] = w(2, v);
     pr[v] = w(3, v);
   }
-  Roe2<AX> es;
   es.fwd(pl, pr);
   for (int i = 0; i < 4; i++) {
     f.QL[i] = Q[i];
@@ -887,7 +907,8 @@ CFD_HD inline void iface_from_record(const W& w, const double* Q, unsigned bits,
 // eigenvector cotangents are folded into six scalars as they appear, and the
 // projection cotangents go straight through p2c's transpose wave by wave.
 template <int AX, class IP = JS, class W>
-CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF, double lw[6][4]) {
+CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF, double lw[6][4],
+                                  const Roe2<AX>& es) {
   CFD_CONTRACT
   double loL[4] = {0, 0, 0, 0}, loR[4] = {0, 0, 0, 0};
   {
@@ -913,8 +934,6 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
     pl[v] = w(2, v);
     pr[v] = w(3, v);
   }
-  Roe2<AX> es;
-  es.fwd(pl, pr);
   typename Roe2<AX>::Acc acc;
   es.acc_R([&](int i, int k) { return f.QL[k] * lcL[i] + f.QR[k] * lcR[i]; }, acc);
   double lQL[4], lQR[4];
@@ -980,6 +999,17 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
     p2c_vjp(p, lc[j], lw[j]);
   }
   es.finish(acc, pl, pr, lw[2], lw[3]);
+}
+template <int AX, class IP = JS, class W>
+CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* lF, double lw[6][4]) {
+  double pl[4], pr[4];
+  for (int v = 0; v < 4; v++) {
+    pl[v] = w(2, v);
+    pr[v] = w(3, v);
+  }
+  Roe2<AX> es;
+  es.fwd(pl, pr);
+  iface_backward<AX, IP>(w, f, lF, lw, es);
 }
 
 // Same derivative as interface<AX>(w, F, lw, lF), from the two passes above.
