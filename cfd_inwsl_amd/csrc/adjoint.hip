@@ -17,13 +17,12 @@
 //
 // Kernel: one workgroup owns CP pencils x one segment of cells and marches
 // along the pencil in chunks of CA interfaces.  Per chunk: load cells
-// (primitive from U, scaled cotangent, zeroed accumulator) into LDS rings;
-// each lane takes one interface through the reverse pass and stages its six
-// cell cotangents in LDS; after one barrier each lane adds the staged
-// contributions to its own cells in a fixed interface order (no two lanes
-// write one cell and the summation order is fixed: results are
-// deterministic); finally each lane finalises one cell whose six interfaces
-// are done.  Ghost-cell cotangents go to a small
+// (primitive from U, scaled cotangent) into LDS rings; each lane takes one
+// interface through the reverse pass and stages its six cell cotangents in
+// LDS (two staging buffers, this chunk's and the previous chunk's); after one
+// barrier each lane finalises one cell whose six interfaces are done, summing
+// their staged contributions in a fixed interface order (no two lanes write
+// one cell and the summation order is fixed: results are deterministic).  Ghost-cell cotangents go to a small
 // per-pencil buffer; the x sweep's are folded onto their source cells by the
 // y sweep's cell phase (stream order makes them visible), the y sweep's by a
 // small kernel after it.  A direction periodic on both faces has no ghosts:
@@ -174,8 +173,9 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
   __shared__ double wc[8 * RC * CP];  // primitive state, then the conservative state rebuilt from it
   __shared__ double mc[4 * RC * CP];  // coef * cotangent
-  __shared__ double ac[4 * RC * CP];  // accumulated primitive cotangent
-  __shared__ double st[24 * CA * CP];  // the chunk's six cell cotangents per interface
+  // six cell cotangents per interface, for this chunk and the one before
+  // (double-buffered by chunk parity: a cell's six interfaces span both)
+  __shared__ double st[2 * 24 * CA * CP];
   auto idx = [](int v, int slot, int p) -> int {
     return AX == 0 ? (v * CP + p) * RC + slot : (v * RC + slot) * CP + p;
   };
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       f.m[v] = in ? a.MU[v * a.N + i] : 0.0;
     }
   };
-  // primitive state, coef * cotangent and a zero accumulator into the ring
+  // primitive state, its rebuilt conservative state and coef * cotangent into the ring
   auto commit = [&](int c, const Fetch& f) {
     const int s = slot_of(c);
     double p[4], m[4] = {0.0, 0.0, 0.0, 0.0};
@@ -247,7 +247,6 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       wc[idx(v, s, lp)] = p[v];
       wc[idx(4 + v, s, lp)] = cc[v];
       mc[idx(v, s, lp)] = m[v];
-      ac[idx(v, s, lp)] = 0.0;
     }
   };
   auto M = [&](int c, int v) -> double { return mc[idx(v, slot_of(c), lp)]; };
@@ -276,15 +275,13 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   if (pvalid && kmin + 2 + la <= kmax + 2) fetch(kmin + 2 + la, nf);
   for (int k0 = kmin; k0 - 3 < e1; k0 += CA) {
     if (pvalid && k0 + 2 + la <= kmax + 2) commit(k0 + 2 + la, nf);
-    // this chunk's finalised cell: its state and base terms, loaded now
+    // this chunk's finalised cell: its base terms, loaded now
     const int cf = k0 - 3 + la;
     const bool fin = pvalid && cf >= e0 && cf < e1;
     const bool fin_in = fin && (ring || (cf >= 0 && cf < n));
     const long fi = base + (long)cf * sa;
-    double fu[4], fx[3][4];
+    double fx[3][4];
     if (fin_in) {
-#pragma unroll
-      for (int v = 0; v < 4; v++) fu[v] = a.U[v * a.N + fi];
       if (a.use_init) {
 #pragma unroll
         for (int t = 0; t < 3; t++) {
@@ -329,43 +326,32 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
         cfdadj::iface_backward<AX, IP>(cells, f, lG, lw);
       }
     }
+    const int par = ((k0 - kmin) / CA) & 1;  // this chunk's staging buffer
+    double* const st_cur = st + par * 24 * CA * CP;
+    const double* const st_prev = st + (par ^ 1) * 24 * CA * CP;
     if (act) {
 #pragma unroll
       for (int j = 0; j < 6; j++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) st[sidx(j * 4 + v, la, lp)] = lw[j][v];
+        for (int v = 0; v < 4; v++) st_cur[sidx(j * 4 + v, la, lp)] = lw[j][v];
     }
     __syncthreads();
-    // Each lane adds the chunk's contributions to its own cells: cell k0-3+la
-    // (finalised below by the same lane) and, for la < 5, cell k0+CA-3+la
-    // (finalised by the same lane one chunk later).  Interfaces come in the
-    // order k = c+3, c+2, .., c-2, so every cell's sum has a fixed order.
-    if (pvalid) {
-      const int kend = min(k0 + CA - 1, kmax);
-      auto gather = [&](int c) {
-        const int s = slot_of(c);
-        double acc[4];
-#pragma unroll
-        for (int v = 0; v < 4; v++) acc[v] = ac[idx(v, s, lp)];
-#pragma unroll
-        for (int j = 0; j < 6; j++) {
-          const int kk = c + 3 - j;
-          if (kk >= k0 && kk <= kend) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) acc[v] += st[sidx(j * 4 + v, kk - k0, lp)];
-          }
-        }
-#pragma unroll
-        for (int v = 0; v < 4; v++) ac[idx(v, s, lp)] = acc[v];
-      };
-      gather(k0 - 3 + la);
-      if (la < 5) gather(k0 + CA - 3 + la);
-    }
     if (fin) {
+      // The cell's six interfaces k = c+3, c+2, .., c-2 (in this order, so
+      // every cell's sum has a fixed order): this chunk's staging for k >= k0,
+      // the previous chunk's for k < k0.
       const int c = cf;
       const int s = slot_of(c);
-      double lwc[4];
-      for (int v = 0; v < 4; v++) lwc[v] = ac[idx(v, s, lp)];
+      double lwc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        const int kk = c + 3 - j;
+        if (kk >= kmin && kk <= kmax) {
+          const double* src = kk >= k0 ? st_cur + sidx(j * 4, kk - k0, lp) : st_prev + sidx(j * 4, kk - k0 + CA, lp);
+#pragma unroll
+          for (int v = 0; v < 4; v++) lwc[v] += src[v * CA * CP];
+        }
+      }
       if (ring || (c >= -1 && c <= n)) {  // node fluxes exist on cells -1 .. n
         double p[4], lf[4];
         for (int v = 0; v < 4; v++) {
@@ -388,7 +374,12 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
           }
       }
       if (fin_in) {
-        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        // the conservative state the ring rebuilt from the cell's primitive
+        // state (the cell's own state up to rounding, which a derivative
+        // does not see)
+        double fu[4], g[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < 4; v++) fu[v] = wc[idx(4 + v, s, lp)];
         cfdadj::c2p_vjp(fu, lwc, g);
         for (int v = 0; v < 4; v++) {
           double b;
