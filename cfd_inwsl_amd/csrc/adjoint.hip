@@ -68,6 +68,7 @@ namespace {
 #ifndef CFD_ADJ_OCC
 #define CFD_ADJ_OCC 1
 #endif
+
 template <int AX> struct VjpTile {
   static constexpr int CA = AX == 0 ? CFD_ADJ_CA0 : CFD_ADJ_CA1;
   static constexpr int CP = AX == 0 ? CFD_ADJ_CP0 : CFD_ADJ_CP1;

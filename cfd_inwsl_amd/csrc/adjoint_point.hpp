@@ -91,10 +91,16 @@ struct JS {
     u[1] = -0.125 * q[1] + 0.75 * q[2] + 0.375 * q[3];
     u[2] = 0.375 * q[2] + 0.75 * q[3] - 0.125 * q[4];
     const double d[3] = {0.0625, 0.625, 0.3125};
-    for (int k = 0; k < 3; k++) {
-      e[k] = rcp(1e-6 + b[k]);
-      a[k] = d[k] * e[k] * e[k];
-    }
+    // e_k = 1 / (1e-6 + b_k) from one reciprocal of the product (each factor
+    // lies in [1e-6, ~1e100] for finite data)
+    const double x0 = 1e-6 + b[0], x1 = 1e-6 + b[1], x2 = 1e-6 + b[2];
+    const double x01 = x0 * x1;
+    const double r = rcp(x01 * x2);
+    const double r01 = x2 * r;  // 1 / (x0 x1)
+    e[0] = x1 * r01;
+    e[1] = x0 * r01;
+    e[2] = x01 * r;
+    for (int k = 0; k < 3; k++) a[k] = d[k] * e[k] * e[k];
     S = a[0] + a[1] + a[2];
     iS = rcp(S);
     out = (a[0] * u[0] + a[1] * u[1] + a[2] * u[2]) * iS;
@@ -955,6 +961,13 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
     double Lk[4];
     es.lrow(k, Lk);
     double gl[5] = {0, 0, 0, 0, 0}, gr[5] = {0, 0, 0, 0, 0};
+    // row 0 (the shear wave) is (v, 0, -1, 0) for an x normal and (-u, 1, 0, 0)
+    // for a y normal: its zero entries are left out of the products below
+    auto proj = [&](int j) -> double {
+      if (k == 0)
+        return AX == 0 ? Lk[0] * w.cons(j, 0) - w.cons(j, 2) : Lk[0] * w.cons(j, 0) + w.cons(j, 1);
+      return Lk[0] * w.cons(j, 0) + Lk[1] * w.cons(j, 1) + Lk[2] * w.cons(j, 2) + Lk[3] * w.cons(j, 3);
+    };
     if constexpr (IP::kLinear) {  // the recorded flags' stencils
       IP a, b;
       a.flag = (f.tflag >> (6 * k)) & 7u;
@@ -964,8 +977,7 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
     } else {
       double ch[6];
       CFD_UNROLL
-      for (int j = 0; j < 6; j++)
-        ch[j] = Lk[0] * w.cons(j, 0) + Lk[1] * w.cons(j, 1) + Lk[2] * w.cons(j, 2) + Lk[3] * w.cons(j, 3);
+      for (int j = 0; j < 6; j++) ch[j] = proj(j);
       double ql[5] = {ch[0], ch[1], ch[2], ch[3], ch[4]};
       double qr[5] = {ch[5], ch[4], ch[3], ch[2], ch[1]};
       {
@@ -985,11 +997,18 @@ CFD_HD inline void iface_backward(const W& w, const IfaceFwd& f, const double* l
     lch[5] = gr[0];
     double gLk[4] = {0, 0, 0, 0};
     CFD_UNROLL
-    for (int j = 0; j < 6; j++)
-      for (int v = 0; v < 4; v++) {
-        gLk[v] += lch[j] * w.cons(j, v);
-        lc[j][v] += lch[j] * Lk[v];
+    for (int j = 0; j < 6; j++) {
+      for (int v = 0; v < 4; v++) gLk[v] += lch[j] * w.cons(j, v);  // row 0: only gLk[0] is used
+      if (k == 0) {
+        lc[j][0] += lch[j] * Lk[0];
+        if (AX == 0)
+          lc[j][2] -= lch[j];
+        else
+          lc[j][1] += lch[j];
+      } else {
+        for (int v = 0; v < 4; v++) lc[j][v] += lch[j] * Lk[v];
       }
+    }
     es.acc_Lrow(k, gLk, acc);
   }
   CFD_UNROLL
