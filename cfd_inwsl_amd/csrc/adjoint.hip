@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,8 @@ template <int AX> struct VjpTile {
   static constexpr int CA = AX == 0 ? CFD_ADJ_CA0 : CFD_ADJ_CA1;
   static constexpr int CP = AX == 0 ? CFD_ADJ_CP0 : CFD_ADJ_CP1;
 };
+// whole ring pencils per workgroup (FULL): two pencils x 128 interfaces
+constexpr int kFullCA = 128, kFullCP = 2;
 
 // OUT = base + coef * (d rhs_AX / d U)^T MU, with base = OUT, or (init) the
 // combination sum_i b_i X_i of up to three SoA arrays (null X_i: absent).
@@ -169,7 +172,14 @@ __device__ __forceinline__ void iface_forward_exact(const W& w, cfdadj::IfaceFwd
 // REC: the interface's half-node values and branch bits come from the
 // recording sweep of the same state (iface_from_record) instead of an exact
 // re-evaluation of the interface.
-template <int AX, int CA, int CP, class IP, int SCH, bool REC>
+//
+// FULL (a ring direction whose pencils are each marched by one workgroup,
+// seg_len = n): the n interfaces are evaluated once each, in chunks from
+// interface 0.  Cells -3 .. 1 of the first chunk (n-3 .. n-1, 0, 1) still miss
+// interfaces the last chunk evaluates: their partial sums are carried in LDS
+// and completed after the last chunk, so no interface is evaluated twice
+// (a segment of L cells otherwise evaluates L + 5).
+template <int AX, int CA, int CP, class IP, int SCH, bool REC, bool FULL>
 __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   constexpr int RC = CA + 6;  // ring slots: cells k0-4 .. k0+CA+1
   __shared__ double wc[8 * RC * CP];  // primitive state, then the conservative state rebuilt from it
@@ -177,6 +187,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   // six cell cotangents per interface, for this chunk and the one before
   // (double-buffered by chunk parity: a cell's six interfaces span both)
   __shared__ double st[2 * 24 * CA * CP];
+  __shared__ double carry[FULL ? 20 * CP : 1];  // FULL: cells -3 .. 1 after the first chunk
   auto idx = [](int v, int slot, int p) -> int {
     return AX == 0 ? (v * CP + p) * RC + slot : (v * RC + slot) * CP + p;
   };
@@ -191,16 +202,24 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
   const int n = a.n[AX];
   const long sa = AX == 0 ? 1 : nx;
   const long npencil = AX == 0 ? ny : nx;
-  const long gp = (long)blockIdx.x * CP + lp;
+  // FULL y sweeps (two x-adjacent pencils per workgroup): consecutive
+  // workgroups of one XCD (dispatch round-robins workgroups over the 8 XCDs)
+  // take adjacent pencil pairs, so that the row segments they read share that
+  // XCD's L2 lines
+  const unsigned nb = gridDim.x;
+  const unsigned bx = (FULL && AX == 1 && nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+  const long gp = (long)bx * CP + lp;
   const bool pvalid = gp < npencil;
   const long base = pvalid ? (AX == 0 ? gp * nx : gp) : 0;
   const int c0 = blockIdx.y * a.seg_len;
   const int c1 = min(c0 + a.seg_len, n);
-  const bool ring = a.ring != 0;
-  // cells finalised here (ghosts at the pencil ends, none on a ring)
-  const int e0 = c0 == 0 && !ring ? -4 : c0;
-  const int e1 = c1 == n && !ring ? n + 4 : c1;
-  const int kmin = ring ? e0 - 2 : max(e0 - 2, -1), kmax = ring ? e1 + 2 : min(e1 + 2, n + 1);
+  const bool ring = FULL || a.ring != 0;
+  // cells finalised in the chunk loop (ghosts at the pencil ends, none on a
+  // ring; FULL: cells 2 .. n-4, the others after it)
+  const int e0 = FULL ? 2 : c0 == 0 && !ring ? -4 : c0;
+  const int e1 = FULL ? n - 3 : c1 == n && !ring ? n + 4 : c1;
+  const int kmin = FULL ? 0 : ring ? e0 - 2 : max(e0 - 2, -1);
+  const int kmax = FULL ? n - 1 : ring ? e1 + 2 : min(e1 + 2, n + 1);
   // ring index of a cell or interface in [-8, n + 8) (n >= 8)
   auto wrap = [n](int c) -> int { return c < 0 ? c + n : (c >= n ? c - n : c); };
   const double w0 = 3.0 / 2.0, w1 = -3.0 / 10.0, w2 = 1.0 / 30.0;
@@ -267,6 +286,92 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     __device__ double cons(int j, int v) const { return at(j, 4 + v); }
   };
 
+  // the base terms of the output at cell index fi (init's combination, or OUT)
+  auto load_base = [&](long fi, double fx[3][4]) {
+    if (a.use_init) {
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const double* X = a.init.X[t];
+#pragma unroll
+        for (int v = 0; v < 4; v++) fx[t][v] = X ? X[v * a.N + fi] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; v++) fx[0][v] = a.OUT[v * a.N + fi];
+    }
+  };
+  // Adds to lwc the staged cotangents of cell c from its interfaces k = c+3,
+  // c+2, .., c-2 (in this order, so that every cell's sum has a fixed order):
+  // the staging of the chunk starting at k0 for k >= k0, the previous chunk's
+  // for k < k0.
+  auto gather = [&](int c, int k0, double* lwc) {
+    const int par = ((k0 - kmin) / CA) & 1;
+    const double* const st_cur = st + par * 24 * CA * CP;
+    const double* const st_prev = st + (par ^ 1) * 24 * CA * CP;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      const int kk = c + 3 - j;
+      if (kk >= kmin && kk <= kmax) {
+        const double* src = kk >= k0 ? st_cur + sidx(j * 4, kk - k0, lp) : st_prev + sidx(j * 4, kk - k0 + CA, lp);
+#pragma unroll
+        for (int v = 0; v < 4; v++) lwc[v] += src[v * CA * CP];
+      }
+    }
+  };
+  // Finalises cell c (ring index; interior: output index fi with base terms
+  // fx) from the summed interface cotangents lwc: node flux, the x ghosts of
+  // its row (y sweep), consToPrim's transpose; a ghost cell's cotangent goes
+  // to GH.
+  auto finish = [&](int c, bool interior, long fi, const double fx[3][4], double* lwc) {
+    const int s = slot_of(c);
+    const int cr = ring ? wrap(c) : c;
+    if (ring || (c >= -1 && c <= n)) {  // node fluxes exist on cells -1 .. n
+      double p[4], lf[4];
+      for (int v = 0; v < 4; v++) {
+        p[v] = wc[idx(v, s, lp)];
+        lf[v] = w1 * (M(c - 1, v) - M(c + 1, v)) * ih;
+      }
+      cfdadj::node_flux_vjp<AX>(p, lf, lwc);
+    }
+    if (AX == 1 && cr >= 0 && cr < n && (gp < 4 || gp >= nx - 4)) {
+      // x ghosts of row cr whose source is this column (sources lie within
+      // four cells of the row ends)
+      const unsigned col = gp < 4 ? (unsigned)gp : (unsigned)(gp - nx + 8);
+      for (int q = 0; q < 8; q++)
+        if (((a.gx_src >> (4 * q)) & 15u) == col) {
+          const int flip = (int)((a.gx_flip >> (2 * q)) & 3u);
+          for (int v = 0; v < 4; v++) {
+            const double x = a.GHX[((long)cr * 8 + q) * 4 + v];
+            lwc[v] += (flip != 0 && v == flip) ? -x : x;
+          }
+        }
+    }
+    if (interior) {
+      // the conservative state the ring rebuilt from the cell's primitive
+      // state (the cell's own state up to rounding, which a derivative
+      // does not see)
+      double fu[4], g[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) fu[v] = wc[idx(4 + v, s, lp)];
+      cfdadj::c2p_vjp(fu, lwc, g);
+      for (int v = 0; v < 4; v++) {
+        double b;
+        if (a.use_init) {
+          b = 0.0;
+#pragma unroll
+          for (int t = 0; t < 3; t++)
+            if (a.init.X[t]) b += a.init.b[t] * fx[t][v];
+        } else {
+          b = fx[0][v];
+        }
+        a.OUT[v * a.N + fi] = b + g[v];
+      }
+    } else {
+      const int gs = c < 0 ? c + 4 : c - n + 4;  // 0..7
+      for (int v = 0; v < 4; v++) a.GH[(gp * 8 + gs) * 4 + v] = lwc[v];
+    }
+  };
+
   if (pvalid && la < 5) {
     Fetch f;
     fetch(kmin - 3 + la, f);
@@ -282,19 +387,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     const bool fin_in = fin && (ring || (cf >= 0 && cf < n));
     const long fi = base + (long)cf * sa;
     double fx[3][4];
-    if (fin_in) {
-      if (a.use_init) {
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-          const double* X = a.init.X[t];
-#pragma unroll
-          for (int v = 0; v < 4; v++) fx[t][v] = X ? X[v * a.N + fi] : 0.0;
-        }
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; v++) fx[0][v] = a.OUT[v * a.N + fi];
-      }
-    }
+    if (fin_in) load_base(fi, fx);
     // the next chunk's new ring cell
     if (pvalid && k0 + CA - 3 < e1 && k0 + CA + 2 + la <= kmax + 2) fetch(k0 + CA + 2 + la, nf);
     const int k = k0 + la;
@@ -328,9 +421,8 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
       }
     }
     const int par = ((k0 - kmin) / CA) & 1;  // this chunk's staging buffer
-    double* const st_cur = st + par * 24 * CA * CP;
-    const double* const st_prev = st + (par ^ 1) * 24 * CA * CP;
     if (act) {
+      double* const st_cur = st + par * 24 * CA * CP;
 #pragma unroll
       for (int j = 0; j < 6; j++)
 #pragma unroll
@@ -338,68 +430,38 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     }
     __syncthreads();
     if (fin) {
-      // The cell's six interfaces k = c+3, c+2, .., c-2 (in this order, so
-      // every cell's sum has a fixed order): this chunk's staging for k >= k0,
-      // the previous chunk's for k < k0.
-      const int c = cf;
-      const int s = slot_of(c);
       double lwc[4] = {0.0, 0.0, 0.0, 0.0};
+      gather(cf, k0, lwc);
+      finish(cf, fin_in, fi, fx, lwc);
+    } else if (FULL && k0 == 0 && pvalid && la < 5) {
+      double lwc[4] = {0.0, 0.0, 0.0, 0.0};
+      gather(cf, k0, lwc);
 #pragma unroll
-      for (int j = 0; j < 6; j++) {
-        const int kk = c + 3 - j;
-        if (kk >= kmin && kk <= kmax) {
-          const double* src = kk >= k0 ? st_cur + sidx(j * 4, kk - k0, lp) : st_prev + sidx(j * 4, kk - k0 + CA, lp);
-#pragma unroll
-          for (int v = 0; v < 4; v++) lwc[v] += src[v * CA * CP];
-        }
-      }
-      if (ring || (c >= -1 && c <= n)) {  // node fluxes exist on cells -1 .. n
-        double p[4], lf[4];
-        for (int v = 0; v < 4; v++) {
-          p[v] = wc[idx(v, s, lp)];
-          lf[v] = w1 * (M(c - 1, v) - M(c + 1, v)) * ih;
-        }
-        cfdadj::node_flux_vjp<AX>(p, lf, lwc);
-      }
-      if (AX == 1 && c >= 0 && c < n && (gp < 4 || gp >= nx - 4)) {
-        // x ghosts of row c whose source is this column (sources lie within
-        // four cells of the row ends)
-        const unsigned col = gp < 4 ? (unsigned)gp : (unsigned)(gp - nx + 8);
-        for (int q = 0; q < 8; q++)
-          if (((a.gx_src >> (4 * q)) & 15u) == col) {
-            const int flip = (int)((a.gx_flip >> (2 * q)) & 3u);
-            for (int v = 0; v < 4; v++) {
-              const double x = a.GHX[((long)c * 8 + q) * 4 + v];
-              lwc[v] += (flip != 0 && v == flip) ? -x : x;
-            }
-          }
-      }
-      if (fin_in) {
-        // the conservative state the ring rebuilt from the cell's primitive
-        // state (the cell's own state up to rounding, which a derivative
-        // does not see)
-        double fu[4], g[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int v = 0; v < 4; v++) fu[v] = wc[idx(4 + v, s, lp)];
-        cfdadj::c2p_vjp(fu, lwc, g);
-        for (int v = 0; v < 4; v++) {
-          double b;
-          if (a.use_init) {
-            b = 0.0;
-#pragma unroll
-            for (int t = 0; t < 3; t++)
-              if (a.init.X[t]) b += a.init.b[t] * fx[t][v];
-          } else {
-            b = fx[0][v];
-          }
-          a.OUT[v * a.N + fi] = b + g[v];
-        }
-      } else {
-        const int gs = c < 0 ? c + 4 : c - n + 4;  // 0..7
-        for (int v = 0; v < 4; v++) a.GH[(gp * 8 + gs) * 4 + v] = lwc[v];
-      }
+      for (int v = 0; v < 4; v++) carry[(la * 4 + v) * CP + lp] = lwc[v];
     }
     __syncthreads();
+  }
+  if (FULL) {
+    // cell n+2 (node flux of cell n+1), then cells n-3 .. n+1 (n-3 .. n-1, 0,
+    // 1): the last chunk's staged interfaces plus the carried first-chunk sums
+    if (pvalid && la == 0) {
+      Fetch f;
+      fetch(n + 2, f);
+      commit(n + 2, f);
+    }
+    __syncthreads();
+    if (pvalid && la < 5) {
+      const int c = n - 3 + la, cr = wrap(c);
+      const int k0 = ((n - 1) / CA) * CA;  // the last chunk
+      double lwc[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) lwc[v] = carry[(la * 4 + v) * CP + lp];
+      gather(c, k0, lwc);
+      const long fi = base + (long)cr * sa;
+      double fx[3][4];
+      load_base(fi, fx);
+      finish(c, true, fi, fx, lwc);
+    }
   }
 }
 
@@ -471,6 +533,7 @@ struct AdjointStore {
   bool last_in_store = false;  // the stage store holds the last segment (written by the forward)
   double dt = 0.0;
   int vjp_seg[2] = {0, 0};
+  bool vjp_full[2] = {false, false};  // ring direction marched one pencil per workgroup lane group
   // interface records of the segment's stage states (recording sweeps), one
   // block per stage state q = 3i (S_i), 3i+1 (u1_i), 3i+2 (u2_i): direction 0
   // then 1, each 8 planes of rec_plane[ax] doubles and the branch words
@@ -580,7 +643,22 @@ static int adj_ensure(cfd_handle* h) {
     const int ca = ax == 0 ? VjpTile<0>::CA : VjpTile<1>::CA;
     const int cp = ax == 0 ? VjpTile<0>::CP : VjpTile<1>::CP;
     const bool ring = h->cfg.bc[2 * ax] == CFD_BC_PERIODIC && h->cfg.bc[2 * ax + 1] == CFD_BC_PERIODIC;
-    s->vjp_seg[ax] = vjp_plan_seg(h->cfg.n[ax], (pencils + cp - 1) / cp, ca, cp, ring);
+    const int n = h->cfg.n[ax];
+    s->vjp_seg[ax] = vjp_plan_seg(n, (pencils + cp - 1) / cp, ca, cp, ring);
+    // Whole ring pencils (no interface evaluated twice) where that takes no
+    // more rounds x chunks of resident workgroups than the segments do.
+    const char* env = getenv("CFD_ADJ_FULL");  // 0: never, 1: wherever allowed
+    if (ring && n >= 8 && (!env || env[0] != '0')) {
+      const long resident = 256L * CFD_ADJ_OCC;
+      const long seg = s->vjp_seg[ax];
+      long seg_chunks = 0;  // the longest workgroup's chunks
+      for (int c0 = 0; c0 < n; c0 += (int)seg) seg_chunks = std::max(seg_chunks, (std::min<long>(seg, n - c0) + 5 + ca - 1) / ca);
+      const long seg_blocks = (pencils + cp - 1) / cp * ((n + seg - 1) / seg);
+      const long seg_cost = (seg_blocks + resident - 1) / resident * seg_chunks;
+      const long full_blocks = (pencils + kFullCP - 1) / kFullCP;
+      const long full_cost = (full_blocks + resident - 1) / resident * ((n + kFullCA - 1) / kFullCA);
+      s->vjp_full[ax] = (env && env[0] == '1') || full_cost < seg_cost;
+    }
   }
   size_t off = 0;
   for (int ax = 0; ax < 2; ax++) {
@@ -604,25 +682,36 @@ static RecPair rec_of(const AdjointStore* s, int q) {
 
 // The VJP kernel of a scheme: WENO-JS's reverse pass, or the TENO family's
 // (the flag's stencil, the flag from the scheme's own cut-off).
-template <int AX, int CA, int CP, bool REC>
+template <int AX, int CA, int CP, bool REC, bool FULL>
 static void launch_vjp_rec(int sch, dim3 grid, hipStream_t s, const VjpArgs& a) {
   using cfdadj::JS;
   using cfdadj::Teno;
+  const dim3 blk(CA * CP);
   switch (sch) {
-  case cfd::SCH_JS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, JS, cfd::SCH_JS, REC>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENO: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENO, REC>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENOS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOS, REC>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENOA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOA, REC>), grid, dim3(CA * CP), 0, s, a); break;
-  case cfd::SCH_TENOSA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOSA, REC>), grid, dim3(CA * CP), 0, s, a); break;
-  default: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_WHF, REC>), grid, dim3(CA * CP), 0, s, a); break;
+  case cfd::SCH_JS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, JS, cfd::SCH_JS, REC, FULL>), grid, blk, 0, s, a); break;
+  case cfd::SCH_TENO: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENO, REC, FULL>), grid, blk, 0, s, a); break;
+  case cfd::SCH_TENOS: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOS, REC, FULL>), grid, blk, 0, s, a); break;
+  case cfd::SCH_TENOA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOA, REC, FULL>), grid, blk, 0, s, a); break;
+  case cfd::SCH_TENOSA: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_TENOSA, REC, FULL>), grid, blk, 0, s, a); break;
+  default: hipLaunchKernelGGL((k_rhs_vjp<AX, CA, CP, Teno, cfd::SCH_WHF, REC, FULL>), grid, blk, 0, s, a); break;
   }
 }
-template <int AX, int CA, int CP>
-static void launch_vjp(int sch, dim3 grid, hipStream_t s, const VjpArgs& a) {
+template <int AX>
+static void launch_vjp(int sch, bool full, long pencils, unsigned nseg, hipStream_t s, const VjpArgs& a) {
+  if (full) {
+    const dim3 grid((unsigned)((pencils + kFullCP - 1) / kFullCP), 1);
+    if (a.rec)
+      launch_vjp_rec<AX, kFullCA, kFullCP, true, true>(sch, grid, s, a);
+    else
+      launch_vjp_rec<AX, kFullCA, kFullCP, false, true>(sch, grid, s, a);
+    return;
+  }
+  constexpr int CA = VjpTile<AX>::CA, CP = VjpTile<AX>::CP;
+  const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
   if (a.rec)
-    launch_vjp_rec<AX, CA, CP, true>(sch, grid, s, a);
+    launch_vjp_rec<AX, CA, CP, true, false>(sch, grid, s, a);
   else
-    launch_vjp_rec<AX, CA, CP, false>(sch, grid, s, a);
+    launch_vjp_rec<AX, CA, CP, false, false>(sch, grid, s, a);
 }
 
 // OUT = base + coef * (d rhs / d U)^T MU at the state U, both directions;
@@ -668,15 +757,13 @@ static int adj_vjp(cfd_handle* h, const double* U, const double* MU, double coef
     a.bc_prim = s->bcprim + 8 * ax;
     const int n = h->cfg.n[ax];
     const long pencils = ax == 0 ? h->cfg.n[1] : h->cfg.n[0];
+    const bool full = s->vjp_full[ax];
+    if (full) a.seg_len = n;
     const unsigned nseg = (unsigned)((n + a.seg_len - 1) / a.seg_len);
     if (ax == 0) {
-      constexpr int CA = VjpTile<0>::CA, CP = VjpTile<0>::CP;
-      const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
-      launch_vjp<0, CA, CP>(h->sch, grid, h->stream, a);
+      launch_vjp<0>(h->sch, full, pencils, nseg, h->stream, a);
     } else {
-      constexpr int CA = VjpTile<1>::CA, CP = VjpTile<1>::CP;
-      const dim3 grid((unsigned)((pencils + CP - 1) / CP), nseg);
-      launch_vjp<1, CA, CP>(h->sch, grid, h->stream, a);
+      launch_vjp<1>(h->sch, full, pencils, nseg, h->stream, a);
       if (!a.ring)
         hipLaunchKernelGGL(k_ghost_fold_y, dim3((unsigned)((pencils * 8 + 255) / 256)), dim3(256), 0, h->stream, a);
     }

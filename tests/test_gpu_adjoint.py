@@ -54,6 +54,50 @@ def test_rhs_vjp_matches_host(lib, name, inter):  # noqa: F811
         assert err < 1e-12
 
 
+@pytest.mark.parametrize("inter", [InterMethod.WCNS5, InterMethod.TCNS5])
+@pytest.mark.parametrize("bcy", [BC.PERIODIC, BC.OUTFLOW])
+def test_rhs_vjp_whole_ring_pencils(lib, monkeypatch, bcy, inter):  # noqa: F811
+    """Ring directions marched one whole pencil per workgroup (first-chunk
+    sums carried to the end) against the host statement and against the
+    segmented march: one chunk, several chunks with a short last one, and the
+    smallest pencil."""
+    bc = [BC.PERIODIC, BC.PERIODIC, bcy, bcy]
+    teno_ct = 1e-5 if inter == InterMethod.TCNS5 else None
+    for n, seed in (((9, 8), 5), ((40, 36), 3), ((130, 70), 4), ((260, 200), 6)):
+        h = (0.07, 0.09)
+        cons = random_state(2, n, seed=seed, jumps=True)
+        mu = np.random.default_rng(seed).normal(size=cons.shape)
+        got = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("CFD_ADJ_FULL", mode)
+            s = _solver(n, h, bc, None, inter)
+            s.set_state(cons)
+            got[mode] = s.rhs_vjp(mu)
+            s.close()
+        ref = host_vjp(lib, n, h, bc, np.zeros((6, 5)), cons, mu, teno_ct)
+        scale = np.abs(ref).max()
+        err = np.abs(got["1"] - ref).max() / scale
+        d = np.abs(got["1"] - got["0"]).max() / scale
+        _say(f"rhs_vjp whole-ring {inter.name} y={bcy.name} {n}: vs host {err:.2e}, vs segmented {d:.2e}")
+        assert err < 1e-12 and d < 1e-13
+
+
+def test_adjoint_whole_ring_pencils_equal_segments(monkeypatch):
+    """The checkpointed RK3 adjoint with whole-ring VJPs equals the segmented
+    one to rounding (their sums differ only in order)."""
+    n, nsteps = 200, 30
+    case, U0 = _shock(n)
+    dt = cases.cfl_dt_2d(U0, n, case["h"])
+    grads = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("CFD_ADJ_FULL", mode)
+        _, gJ, gU = _gpu_gradient(case, U0, dt, nsteps, 10)
+        grads[mode] = gU
+    d = np.abs(grads["1"] - grads["0"]).max() / np.abs(grads["0"]).max()
+    _say(f"adjoint whole-ring vs segmented VJPs ({n}^2, {nsteps} steps): {d:.2e}")
+    assert d < 1e-12
+
+
 def _shock(n, amp=1e-2):
     case = cases.adjoint_shock(n, amp=amp)
     U0 = cases.adjoint_state(case)
