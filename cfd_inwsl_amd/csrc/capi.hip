@@ -204,9 +204,9 @@ __global__ void k_zero_lambda(double* scal) {
 // of stage 3 (SW_WAVE), divided by h_d, into scal[1..D] -- the maxima of the
 // per-cell quotients bit for bit, the division being monotone; nothing for the
 // queued steps after an output time
-template <int D> __global__ __launch_bounds__(256) void k_wave_parts(const double* __restrict__ part, long nb,
-                                                                     double h0, double h1, double h2, double* scal) {
-  if (halted(scal)) return;
+template <int D>
+__device__ __forceinline__ void wave_parts_block(const double* __restrict__ part, long nb, double h0, double h1,
+                                                 double h2, double* scal, double* lam_out) {
   __shared__ double red[3][256];
   for (int d = 0; d < D; d++) {
     double m = 0.0;
@@ -227,8 +227,15 @@ template <int D> __global__ __launch_bounds__(256) void k_wave_parts(const doubl
   }
   if (threadIdx.x < D) {
     const double hh[3] = {h0, h1, h2};
-    scal[1 + threadIdx.x] = fdv(red[threadIdx.x][0], hh[threadIdx.x]);
+    const double l = fdv(red[threadIdx.x][0], hh[threadIdx.x]);
+    scal[1 + threadIdx.x] = l;
+    if (lam_out) lam_out[threadIdx.x] = l;
   }
+}
+template <int D> __global__ __launch_bounds__(256) void k_wave_parts(const double* __restrict__ part, long nb,
+                                                                     double h0, double h1, double h2, double* scal) {
+  if (halted(scal)) return;
+  wave_parts_block<D>(part, nb, h0, h1, h2, scal, nullptr);
 }
 
 // max over cells of (sqrt(gamma p / rho) + |u_d|) / h_d per direction
@@ -289,9 +296,9 @@ __global__ __launch_bounds__(256) void k_abs_max(const E* __restrict__ x, long N
 
 // dt = cfl / ((l0 + l1) + l2); with `advance` (the step follows on the
 // device) the clocks move on by dt as well
-template <int D> __global__ void k_finish_dt(double* scal, double cfl, int advance) {
+template <int D> __device__ __forceinline__ void finish_dt_from(const double* l, double* scal, double cfl, int advance) {
   double lam = 0;
-  for (int d = 0; d < D; d++) lam += scal[1 + d];
+  for (int d = 0; d < D; d++) lam += l[d];
   double dt = cfl / lam;
   scal[0] = dt;
   if (advance) {
@@ -299,6 +306,22 @@ template <int D> __global__ void k_finish_dt(double* scal, double cfl, int advan
     scal[7] = scal[6];
     scal[6] = step_time(scal[6], dt);
   }
+}
+template <int D> __global__ void k_finish_dt(double* scal, double cfl, int advance) {
+  finish_dt_from<D>(scal + 1, scal, cfl, advance);
+}
+// k_wave_parts then k_finish_dt in one launch (the step loop's per-step dt)
+template <int D>
+__global__ __launch_bounds__(256) void k_wave_parts_dt(const double* __restrict__ part, long nb, double h0, double h1,
+                                                       double h2, double* scal, double cfl, int advance) {
+  __shared__ double lam[3];
+  if (!halted(scal)) {  // uniform: every thread reads the same flag
+    wave_parts_block<D>(part, nb, h0, h1, h2, scal, lam);
+  } else if (threadIdx.x < D) {
+    lam[threadIdx.x] = scal[1 + threadIdx.x];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) finish_dt_from<D>(lam, scal, cfl, advance);
 }
 
 // stepsLoopCFL's dt (blockSolver.cpp:178-190): the CFL dt clamped to the next
@@ -469,11 +492,13 @@ static void wave_from_parts(cfd_handle* h) {
 
 // dt from the previous step's partial maxima (fused into its last sweep)
 static int finish_dt(cfd_handle* h, double cfl, int advance) {
-  wave_from_parts(h);
+  const double* hh = h->cfg.h;
+  const double* part = h->wave_scratch ? h->wave_scratch : h->wpart;
+  const long nb = h->wpart_blocks;
   switch (h->D) {
-  case 1: hipLaunchKernelGGL(k_finish_dt<1>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance); break;
-  case 2: hipLaunchKernelGGL(k_finish_dt<2>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance); break;
-  default: hipLaunchKernelGGL(k_finish_dt<3>, dim3(1), dim3(1), 0, h->stream, h->scal, cfl, advance); break;
+  case 1: hipLaunchKernelGGL(k_wave_parts_dt<1>, dim3(1), dim3(256), 0, h->stream, part, nb, hh[0], hh[1], hh[2], h->scal, cfl, advance); break;
+  case 2: hipLaunchKernelGGL(k_wave_parts_dt<2>, dim3(1), dim3(256), 0, h->stream, part, nb, hh[0], hh[1], hh[2], h->scal, cfl, advance); break;
+  default: hipLaunchKernelGGL(k_wave_parts_dt<3>, dim3(1), dim3(256), 0, h->stream, part, nb, hh[0], hh[1], hh[2], h->scal, cfl, advance); break;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -902,17 +927,22 @@ int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   for (int ax = 0; ax < h->D; ax++) {
-    PlanCandidate cand[16];
-    int nc = plan_candidates(h->cfg.n, ax, cand, 16);
+    PlanCandidate cand[24];
+    int nc = plan_candidates(h->cfg.n, ax, cand, 24);
     double best = 1e30;
     int bt = h->tile[ax], bs = h->seg[ax];
-    const int flags = SW_WRITE_R | (ax > 0 ? SW_READ_R : 0);
+    // each direction as a step runs it: the last sweep reads R, applies a
+    // (stage-2) update from U^n and the stage state into a scratch state and
+    // writes no R; the others accumulate R
+    const bool last = ax == h->D - 1;
+    const int flags = last ? (SW_RK | (ax > 0 ? SW_READ_R : 0)) : (SW_WRITE_R | (ax > 0 ? SW_READ_R : 0));
+    const int stage = last ? 2 : 1;
     for (int c = 0; c < nc; c++) {
       // warm-up launch, then timed launches
-      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg, nullptr));
+      HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, stage, h->U0, h->Ub, cand[c].tile, cand[c].seg, nullptr));
       HIPCHK(hipEventRecord(e0, h->stream));
       for (int r = 0; r < reps; r++)
-        HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, 1, h->U0, h->Ua, cand[c].tile, cand[c].seg, nullptr));
+        HIPCHK(launch_sweep(h, ax, sweep_state(h), flags, stage, h->U0, h->Ub, cand[c].tile, cand[c].seg, nullptr));
       HIPCHK(hipEventRecord(e1, h->stream));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0;
