@@ -55,13 +55,15 @@ def test_rhs_vjp_matches_host(lib, name, inter):  # noqa: F811
 
 
 @pytest.mark.parametrize("inter", [InterMethod.WCNS5, InterMethod.TCNS5])
-@pytest.mark.parametrize("bcy", [BC.PERIODIC, BC.OUTFLOW])
-def test_rhs_vjp_whole_ring_pencils(lib, monkeypatch, bcy, inter):  # noqa: F811
+@pytest.mark.parametrize("bcx,bcy", [(BC.PERIODIC, BC.PERIODIC), (BC.PERIODIC, BC.OUTFLOW),
+                                     (BC.OUTFLOW, BC.PERIODIC), (BC.SYMMETRYX, BC.PERIODIC)])
+def test_rhs_vjp_whole_ring_pencils(lib, monkeypatch, bcx, bcy, inter):  # noqa: F811
     """Ring directions marched one whole pencil per workgroup (first-chunk
     sums carried to the end) against the host statement and against the
     segmented march: one chunk, several chunks with a short last one, and the
-    smallest pencil."""
-    bc = [BC.PERIODIC, BC.PERIODIC, bcy, bcy]
+    smallest pencil; with x ghosts (outflow, symmetry) folded in by the y
+    sweep, also in the cells finalised after the last chunk."""
+    bc = [bcx, bcx, bcy, bcy]
     teno_ct = 1e-5 if inter == InterMethod.TCNS5 else None
     for n, seed in (((9, 8), 5), ((40, 36), 3), ((130, 70), 4), ((260, 200), 6)):
         h = (0.07, 0.09)
@@ -78,7 +80,8 @@ def test_rhs_vjp_whole_ring_pencils(lib, monkeypatch, bcy, inter):  # noqa: F811
         scale = np.abs(ref).max()
         err = np.abs(got["1"] - ref).max() / scale
         d = np.abs(got["1"] - got["0"]).max() / scale
-        _say(f"rhs_vjp whole-ring {inter.name} y={bcy.name} {n}: vs host {err:.2e}, vs segmented {d:.2e}")
+        _say(f"rhs_vjp whole-ring {inter.name} x={bcx.name} y={bcy.name} {n}: vs host {err:.2e}, "
+             f"vs segmented {d:.2e}")
         assert err < 1e-12 and d < 1e-13
 
 
