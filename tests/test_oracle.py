@@ -286,6 +286,54 @@ def test_research_variants_reduce_at_the_ends_of_eta(scheme, ends):
         assert not diff, (which, diff[:2])
 
 
+# Integer data whose smoothness indicators (interScheme.hpp's, evaluated
+# exactly for small integers) satisfy beta0 + beta2 = 2 beta1 with every beta
+# positive, found by search: whf_ai_TcnsN_AZS_yuanbao's C = min(1e-5,
+# |beta0 + beta2 - 2 beta1|) is then exactly 0, C_T' = 0, and its TENO-S test
+# keeps every stencil, while the same test with C = 1e-5 (ai_S) cuts one.
+_AZS_FLAT = [(-5, -5, 4, 2, -1), (-4, -4, 5, 3, 0), (-1, 2, 4, -5, -5), (0, -3, -5, 4, 4),
+             (0, 3, 5, -4, -4), (1, -2, -4, 5, 5), (4, 4, -5, -3, 0), (5, 5, -4, -2, 1)]
+
+
+def _quartic_half_node(q):
+    """The quartic through q_0..q_4 (at x = 0..4) evaluated at x = 2.5: the
+    5th-order linear stencil, from Lagrange's weights (exact in binary for
+    small integers)."""
+    from fractions import Fraction as Fr
+    out = Fr(0)
+    for j in range(5):
+        wj = Fr(1)
+        for m in range(5):
+            if m != j:
+                wj *= (Fr(5, 2) - m) / (j - m)
+        out += wj * Fr(q[j])
+    return float(out)
+
+
+def test_azs_keeps_every_stencil_where_its_indicator_vanishes():
+    """The |beta0 + beta2 - 2 beta1| end of the AZS law (the jump end, C = 1e-5,
+    is pinned against ai_S above): on the data above, scaled by powers of two
+    (indicator still exactly 0), AZS returns the linear 5th-order value and
+    ai_S does not.  A slip in the indicator (another combination of the betas,
+    max in place of min) gives C > 0 and a cut stencil.  Where the signed
+    combination is negative and small, C is its absolute value: on the rough
+    data below (tau well above the smallest beta) the positive C_T' cuts a
+    stencil, where a negative C (no absolute value) would give a NaN C_T',
+    fail every cut comparison and keep all stencils."""
+    for q0 in _AZS_FLAT:
+        for k in (0, 10, 20):
+            q = [x * 2.0 ** -k for x in q0]
+            lin = _quartic_half_node(q0) * 2.0 ** -k
+            assert O.inter5(O.S_AI_AZS, q) == lin, (q0, k)
+            assert O.inter5(O.S_AI_S, q) != lin, (q0, k)
+    # integer data with beta0 + beta2 - 2 beta1 = -7/2, -39/4, -39/4 (exact),
+    # scaled by 2^-10 so that |.| < 1e-5
+    for q0 in [(-3, -3, 0, -1, -3), (-3, -1, 3, 3, 2), (-2, -3, -3, 1, 3)]:
+        q = [x * 2.0 ** -10 for x in q0]
+        v = O.inter5(O.S_AI_AZS, q)
+        assert math.isfinite(v) and v != _quartic_half_node(q0) * 2.0 ** -10, q0
+
+
 def test_whf_s_forms_equal_at_k5_and_lad_bounds():
     """whf_TcnsN_AS at k = 5 (eta of the raw values tiny: a zero next to a
     jump) is whf_TcnsN_S (C_T' of 1e-5); whf_TcnsN_LAD on data with
