@@ -251,7 +251,11 @@ int cfd_batch_synchronize(cfd_batch* b);
  * codes; other configurations return CFD_E_UNSUPP.  Branches (HLLC wave
  * pattern, pressure clipping, positivity fallback, TENO stencil flags) take
  * the derivative of the branch the forward evaluation takes.  Arrays are host
- * AoS [cell][4]. */
+ * AoS [cell][4].  A direction periodic on both faces is differentiated as a
+ * ring, marched in segments or (where the planner finds it cheaper) one whole
+ * pencil per lane group; the environment variable CFD_ADJ_FULL = 0 or 1, read
+ * when a handle's first adjoint call sets up its buffers, forces one march
+ * (both give the same gradient up to rounding). */
 
 /* grad = (d rhs / d U)^T cot at the current state. */
 int cfd_rhs_vjp(cfd_handle* h, const double* cot, double* grad);
