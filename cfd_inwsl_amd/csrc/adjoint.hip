@@ -397,7 +397,7 @@ __global__ __launch_bounds__(CA* CP, CFD_ADJ_OCC) void k_rhs_vjp(VjpArgs a) {
     unsigned rbits = 0u;
     if (REC && act) {
       const int kr = ring ? wrap(k) : k;  // a ring interface's record: its copy inside the pencil
-      const long ri = AX == 0 ? gp * (n + 3) + (kr + 1) : (long)(kr + 1) * npencil + gp;
+      const long ri = cfd::rec_index<AX>(gp, kr + 1, n);
 #pragma unroll
       for (int v = 0; v < 8; v++) rq[v] = a.rec[v * a.rec_plane + ri];
       rbits = reinterpret_cast<const unsigned*>(a.rec + 8 * a.rec_plane)[ri];
@@ -662,7 +662,7 @@ static int adj_ensure(cfd_handle* h) {
   }
   size_t off = 0;
   for (int ax = 0; ax < 2; ax++) {
-    const long P = (long)(h->cfg.n[ax] + 3) * (ax == 0 ? h->cfg.n[1] : h->cfg.n[0]);
+    const long P = cfd::rec_plane_of(h->cfg.n, ax);
     s->rec_plane[ax] = P;
     s->rec_off[ax] = off;
     off += (size_t)(8 * P + (P + 1) / 2 + 63) / 64 * 64;  // 8 planes + the words, 512-byte aligned

@@ -446,7 +446,7 @@ hipError_t launch_sweep(cfd_handle* h, int ax, const void* U, int flags, int sta
   if (rec) {
     if (h->D != 2 || h->fp32 || h->fma || !h->cw || h->flx != FLX_HLLC) return hipErrorInvalidValue;
     a.rec = rec;
-    a.rec_plane = (long)(h->cfg.n[ax] + 3) * (ax == 0 ? h->cfg.n[1] : h->cfg.n[0]);
+    a.rec_plane = rec_plane_of(h->cfg.n, ax);
     return launch_sweep_rec(ax, h->sch, tile, &a, dim3((unsigned)nb, (unsigned)nseg), h->stream);
   }
   if (h->fma) return launch_sweep_fma(h->D, ax, h->sch, h->flx, h->cw, tile, &a, dim3((unsigned)nb, (unsigned)nseg), h->stream);

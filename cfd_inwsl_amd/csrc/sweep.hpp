@@ -87,11 +87,24 @@ struct SweepArgs {
   // values QL[4], QR[4] (8 planes of rec_plane doubles) and one word of branch
   // bits (after them): TENO flags (bits 6k..6k+2 left, 6k+3..6k+5 right value
   // of wave k), positivity fallback left (24) / right (25), HLLC branch bits
-  // (26..31: clip, left shock, right shock, region << 3).  Plane index: x
-  // sweep pencil * (n + 3) + k + 1, y sweep (k + 1) * nx + pencil.
+  // (26..31: clip, left shock, right shock, region << 3).  Plane index
+  // (rec_index): x sweep pencil * (n + 3) + k + 1; y sweep pencil pairs, i.e.
+  // ((pencil / 2) * (n + 3) + k + 1) * 2 + pencil % 2, so that the y sweep's
+  // rows of x-adjacent pencils write 64-byte runs and the adjoint's whole-pencil
+  // VJP (two x-adjacent pencils per workgroup) reads contiguous runs.
   void* rec;
   long rec_plane;
 };
+
+// doubles per record plane of direction ax (the y planes hold an even number
+// of pencils)
+inline long rec_plane_of(const int* n, int ax) {
+  return (long)(n[ax] + 3) * (ax == 0 ? (long)n[1] : (long)((n[0] + 1) & ~1));
+}
+// plane index of interface m (0 .. n+2) of pencil gp
+template <int AX> __host__ __device__ __forceinline__ long rec_index(long gp, int m, int n) {
+  return AX == 0 ? gp * (n + 3) + m : (((gp >> 1) * (n + 3) + m) << 1) + (gp & 1);
+}
 
 // Tile shapes: CA interfaces per chunk along the sweep axis x CP pencils.
 // For AX = 0 lanes run along the pencil (CA >= 16 keeps loads coalesced);
@@ -403,7 +416,7 @@ __device__ __forceinline__ void sweep_block(const SweepArgs& a, const long bx, c
             }
             long ri = 0;
             if (REC) {  // the half-node values, stored as soon as they exist
-              ri = AX == 0 ? gp * (n + 3) + m : (long)m * npencil + gp;
+              ri = rec_index<AX>(gp, m, n);
               double* rd = static_cast<double*>(a.rec);
               for (int k = 0; k < NV; k++) {
                 rd[k * a.rec_plane + ri] = QL[k];
