@@ -195,7 +195,9 @@ int cfd_copy_probe(cfd_handle* h, int reps);
 
 /* Chooses the tile shape and segment length of each direction's sweep by
  * timing the candidates of the launch model on the current state (reps
- * launches each).  tiles_out / segs_out (3 ints each, may be NULL) receive the
+ * launches each, each direction with the work it does in a step: the last
+ * sweep with a stage update into a scratch state, which leaves the state
+ * unchanged).  tiles_out / segs_out (3 ints each, may be NULL) receive the
  * choice.  No reference counterpart. */
 int cfd_autotune(cfd_handle* h, int reps, int* tiles_out, int* segs_out);
 
